@@ -1,0 +1,258 @@
+// Internal header of libhmat_hip (gfx950).  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hm_hip.h"
+#include "hmat_lib/error.h"
+
+namespace hm {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+const char *hip_name(hipError_t e);
+
+#define HM_HIP(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      hm::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,      \
+                    hm::hip_name(e_));                                      \
+      return ALLOCATION_FAILURE;                                            \
+    }                                                                       \
+  } while (0)
+
+#define HM_OK(expr)                                                         \
+  do {                                                                      \
+    int r_ = (expr);                                                        \
+    if (r_ != SUCCESS) return r_;                                           \
+  } while (0)
+
+#define HM_LAUNCH_CHECK() HM_HIP(hipGetLastError())
+
+int ensure_init();
+hipStream_t stream_of(void *s);
+
+// -------------------------------------------------------- kernel timing
+// Optional HIP-event bracketing of named kernels (bench instrumentation).
+struct Timed {
+  int slot = -1;
+};
+Timed timing_begin(const char *name, hipStream_t s);
+void timing_end(const Timed &t, hipStream_t s);
+
+// ------------------------------------------------------------ memory
+int dmalloc(void **p, size_t bytes);
+template <class T>
+int dalloc(T **p, size_t count) {
+  return dmalloc(reinterpret_cast<void **>(p), count * sizeof(T) + 16);
+}
+void dfree(void *p);
+
+// -------------------------------------------------------- device tree
+// One off-diagonal block: rows [r0, r0+m) x cols [c0, c0+n) of the matrix,
+// factors U (m x s, ld m) at U + uoff, V (n x s, ld n) at V + voff.
+struct BlockMeta {
+  int r0, m, c0, n;
+  int s, level, j, pad;
+  long long uoff, voff;
+};
+
+// Product scratch layout.  For leaf c and level l (1..h) let
+// q = c >> (h - l).  In Y = H X the leaf's rows are multiplied into block
+// (l, q^1) [input side] and receive block (l, q) [output side]; in
+// Y = H^T X it is (l, q) and (l, q^1).  P_c holds, level after level, the
+// input-side partial products F_in^T X_c of leaf c (S_in(c) x R).
+struct ProductPlan {
+  std::vector<long long> pofs[2];  // per leaf prefix of S_in (rows), x R
+  std::vector<int> proff[2];       // [c * h + (l-1)] row offset inside P_c
+  std::vector<long long> wofs;     // per block prefix of s, x R
+  int s_in_max[2] = {0, 0};
+  int s_out_max[2] = {0, 0};
+  long long *d_pofs[2] = {nullptr, nullptr};
+  int *d_proff[2] = {nullptr, nullptr};
+  long long *d_wofs = nullptr;
+};
+
+}  // namespace hm
+
+struct hm_dtree {
+  int height = 0;
+  int n = 0;
+  int nleaf = 0;
+  int nblock = 0;
+  int max_rank = 0;
+  int max_leaf = 0;
+  std::vector<int> leaf_off;          // nleaf + 1
+  std::vector<long long> diag_off;    // nleaf + 1
+  std::vector<hm::BlockMeta> blk;     // nblock, index (1 << l) - 2 + j
+  double *d_diag = nullptr;
+  double *d_U = nullptr;
+  double *d_V = nullptr;
+  int *d_leaf_off = nullptr;
+  long long *d_diag_off = nullptr;
+  hm::BlockMeta *d_blk = nullptr;
+  hm::ProductPlan plan;
+  double *d_scratch = nullptr;        // P then W, grown on demand
+  size_t scratch_elems = 0;
+};
+
+namespace hm {
+
+__host__ __device__ inline int blk_index(int level, int j) { return (1 << level) - 2 + j; }
+
+// Fill sizes/offsets of a tree from its leaf sizes and block ranks;
+// allocates and uploads metadata and the product plan (not the data pools).
+int tree_finalize_meta(hm_dtree *t, hipStream_t s);
+int tree_alloc_pools(hm_dtree *t);
+int ensure_scratch(hm_dtree *t, int R);
+void tree_free(hm_dtree *t);
+
+// ----------------------------------------------------------- MFMA tiles
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4_f64: lane L holds A[L%16][L/16] and B[L/16][L%16];
+// the 16x16 result lives as 4 values per lane, value v of lane L being
+// D[4*(L/16) + v][L%16].
+__device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// A workgroup of WM*WN waves computes a TM x TN tile C += A B over
+// K-chunks of KC, staging each chunk of A (TM x KC) and B (KC x TN) in LDS.
+// Loaders: la(i, k) and lb(k, j) return the (bounds-checked) element.
+// A_MFAST: consecutive threads fetch consecutive rows i (A column-major);
+// otherwise consecutive k.  B_NFAST: consecutive threads fetch consecutive
+// columns j; otherwise consecutive k (B column-major).
+template <int WM, int WN, int FM, int FN, int KC>
+struct MfmaTile {
+  static constexpr int NW = WM * WN;
+  static constexpr int NT = NW * 64;
+  static constexpr int TM = WM * FM * 16;
+  static constexpr int TN = WN * FN * 16;
+  static constexpr int LDA = TM + 1;
+  static constexpr int LDB = TN + 1;
+  static constexpr int LDS_DOUBLES = KC * LDA + KC * LDB;
+  static constexpr int AREG = (TM * KC + NT - 1) / NT;
+  static constexpr int BREG = (TN * KC + NT - 1) / NT;
+
+  d4 acc[FM][FN];
+  double ra[AREG], rb[BREG];
+
+  __device__ void zero() {
+#pragma unroll
+    for (int f = 0; f < FM; f++)
+#pragma unroll
+      for (int g = 0; g < FN; g++) acc[f][g] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+
+  template <bool A_MFAST, bool B_NFAST, class LA, class LB>
+  __device__ void fetch(int k0, const LA &la, const LB &lb) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < AREG; e++) {
+      int idx = t + e * NT;
+      int i, k;
+      if (A_MFAST) { i = idx % TM; k = idx / TM; }
+      else         { k = idx % KC; i = idx / KC; }
+      ra[e] = (idx < TM * KC) ? la(i, k0 + k) : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < BREG; e++) {
+      int idx = t + e * NT;
+      int j, k;
+      if (B_NFAST) { j = idx % TN; k = idx / TN; }
+      else         { k = idx % KC; j = idx / KC; }
+      rb[e] = (idx < TN * KC) ? lb(k0 + k, j) : 0.0;
+    }
+  }
+
+  template <bool A_MFAST, bool B_NFAST>
+  __device__ void stash(double *lds) {
+    double *As = lds, *Bs = lds + KC * LDA;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < AREG; e++) {
+      int idx = t + e * NT;
+      int i, k;
+      if (A_MFAST) { i = idx % TM; k = idx / TM; }
+      else         { k = idx % KC; i = idx / KC; }
+      if (idx < TM * KC) As[k * LDA + i] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < BREG; e++) {
+      int idx = t + e * NT;
+      int j, k;
+      if (B_NFAST) { j = idx % TN; k = idx / TN; }
+      else         { k = idx % KC; j = idx / KC; }
+      if (idx < TN * KC) Bs[k * LDB + j] = rb[e];
+    }
+  }
+
+  __device__ void compute(const double *lds) {
+    const double *As = lds, *Bs = lds + KC * LDA;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = w % WM, wn = w / WM;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ks++) {
+      double a[FM], b[FN];
+#pragma unroll
+      for (int f = 0; f < FM; f++)
+        a[f] = As[(ks * 4 + lk) * LDA + (wm * FM + f) * 16 + li];
+#pragma unroll
+      for (int g = 0; g < FN; g++)
+        b[g] = Bs[(ks * 4 + lk) * LDB + (wn * FN + g) * 16 + li];
+#pragma unroll
+      for (int f = 0; f < FM; f++)
+#pragma unroll
+        for (int g = 0; g < FN; g++) acc[f][g] = mfma_f64(a[f], b[g], acc[f][g]);
+    }
+  }
+
+  // C += A[:, 0:K) B[0:K, :), software-pipelined over K-chunks.
+  template <bool A_MFAST, bool B_NFAST, class LA, class LB>
+  __device__ void run(double *lds, int K, const LA &la, const LB &lb) {
+    if (K <= 0) return;
+    fetch<A_MFAST, B_NFAST>(0, la, lb);
+    for (int k0 = 0; k0 < K; k0 += KC) {
+      __syncthreads();
+      stash<A_MFAST, B_NFAST>(lds);
+      __syncthreads();
+      if (k0 + KC < K) fetch<A_MFAST, B_NFAST>(k0 + KC, la, lb);
+      compute(lds);
+    }
+    __syncthreads();
+  }
+
+  // st(i, j, value) for every element of the tile (caller bounds-checks).
+  template <class ST>
+  __device__ void store(const ST &st) const {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = w % WM, wn = w / WM;
+#pragma unroll
+    for (int f = 0; f < FM; f++)
+#pragma unroll
+      for (int g = 0; g < FN; g++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          int i = (wm * FM + f) * 16 + 4 * (lane >> 4) + v;
+          int j = (wn * FN + g) * 16 + (lane & 15);
+          st(i, j, acc[f][g][v]);
+        }
+  }
+};
+
+// ----------------------------------------------------- small reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace hm

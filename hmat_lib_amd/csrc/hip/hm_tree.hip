@@ -1,0 +1,341 @@
+// Device-resident HODLR trees: metadata, product plan, upload from and
+// download to the host pointer tree of the C API.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+#include "hm_common.h"
+#include "hmat_lib/hodlr.h"
+
+namespace hm {
+
+void tree_free(hm_dtree *t) {
+  if (!t) return;
+  dfree(t->d_diag);
+  dfree(t->d_U);
+  dfree(t->d_V);
+  dfree(t->d_leaf_off);
+  dfree(t->d_diag_off);
+  dfree(t->d_blk);
+  for (int k = 0; k < 2; k++) {
+    dfree(t->plan.d_pofs[k]);
+    dfree(t->plan.d_proff[k]);
+  }
+  dfree(t->plan.d_wofs);
+  dfree(t->d_scratch);
+  delete t;
+}
+
+template <class T>
+static int upload_vec(T **d, const std::vector<T> &h, hipStream_t s) {
+  HM_OK(dalloc(d, h.size()));
+  if (!h.empty())
+    HM_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T),
+                          hipMemcpyHostToDevice, s));
+  return SUCCESS;
+}
+
+// Requires leaf_off (nleaf+1) and blk[g].{r0,m,c0,n,s,level,j}.
+int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
+  const int h = t->height, nl = t->nleaf;
+  t->n = t->leaf_off[nl];
+  t->diag_off.assign(nl + 1, 0);
+  t->max_leaf = 0;
+  for (int c = 0; c < nl; c++) {
+    long long m = t->leaf_off[c + 1] - t->leaf_off[c];
+    t->diag_off[c + 1] = t->diag_off[c] + m * m;
+    t->max_leaf = std::max<int>(t->max_leaf, (int)m);
+  }
+  long long uo = 0, vo = 0;
+  t->max_rank = 0;
+  for (auto &b : t->blk) {
+    b.uoff = uo;
+    b.voff = vo;
+    uo += (long long)b.m * b.s;
+    vo += (long long)b.n * b.s;
+    t->max_rank = std::max(t->max_rank, b.s);
+  }
+  ProductPlan &p = t->plan;
+  for (int tr = 0; tr < 2; tr++) {
+    p.pofs[tr].assign(nl + 1, 0);
+    p.proff[tr].assign((size_t)nl * h, 0);
+    p.s_in_max[tr] = 0;
+    p.s_out_max[tr] = 0;
+    for (int c = 0; c < nl; c++) {
+      int run_in = 0, run_out = 0;
+      for (int l = 1; l <= h; l++) {
+        int q = c >> (h - l);
+        int bin = tr ? q : (q ^ 1);
+        int bout = tr ? (q ^ 1) : q;
+        p.proff[tr][(size_t)c * h + (l - 1)] = run_in;
+        run_in += t->blk[blk_index(l, bin)].s;
+        run_out += t->blk[blk_index(l, bout)].s;
+      }
+      p.pofs[tr][c + 1] = p.pofs[tr][c] + run_in;
+      p.s_in_max[tr] = std::max(p.s_in_max[tr], run_in);
+      p.s_out_max[tr] = std::max(p.s_out_max[tr], run_out);
+    }
+  }
+  p.wofs.assign(t->nblock + 1, 0);
+  for (int g = 0; g < t->nblock; g++) p.wofs[g + 1] = p.wofs[g] + t->blk[g].s;
+
+  HM_OK(upload_vec(&t->d_leaf_off, t->leaf_off, s));
+  HM_OK(upload_vec(&t->d_diag_off, t->diag_off, s));
+  HM_OK(upload_vec(&t->d_blk, t->blk, s));
+  for (int tr = 0; tr < 2; tr++) {
+    HM_OK(upload_vec(&p.d_pofs[tr], p.pofs[tr], s));
+    HM_OK(upload_vec(&p.d_proff[tr], p.proff[tr], s));
+  }
+  HM_OK(upload_vec(&p.d_wofs, p.wofs, s));
+  HM_HIP(hipStreamSynchronize(s));
+  return SUCCESS;
+}
+
+int tree_alloc_pools(hm_dtree *t) {
+  long long u = 0, v = 0;
+  for (auto &b : t->blk) {
+    u += (long long)b.m * b.s;
+    v += (long long)b.n * b.s;
+  }
+  HM_OK(dalloc(&t->d_diag, (size_t)t->diag_off[t->nleaf]));
+  HM_OK(dalloc(&t->d_U, (size_t)u));
+  HM_OK(dalloc(&t->d_V, (size_t)v));
+  return SUCCESS;
+}
+
+int ensure_scratch(hm_dtree *t, int R) {
+  const ProductPlan &p = t->plan;
+  long long need_p = std::max(p.pofs[0][t->nleaf], p.pofs[1][t->nleaf]);
+  long long need = (need_p + p.wofs[t->nblock]) * (long long)R + 64;
+  if ((size_t)need <= t->scratch_elems) return SUCCESS;
+  dfree(t->d_scratch);
+  t->d_scratch = nullptr;
+  t->scratch_elems = 0;
+  HM_OK(dalloc(&t->d_scratch, (size_t)need));
+  t->scratch_elems = (size_t)need;
+  return SUCCESS;
+}
+
+// Internal nodes of a host subtree per depth, left to right.
+static std::vector<const HODLRInternalNode *> host_levels(
+    const HODLRInternalNode *root, int height) {
+  std::vector<const HODLRInternalNode *> all((1u << height) - 1);
+  all[0] = root;
+  for (int d = 0; d + 1 < height; d++) {
+    size_t base = (1u << d) - 1, nbase = (1u << (d + 1)) - 1;
+    for (size_t i = 0; i < (1u << d); i++) {
+      all[nbase + 2 * i] = all[base + i]->children[0].internal;
+      all[nbase + 2 * i + 1] = all[base + i]->children[3].internal;
+    }
+  }
+  return all;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" {
+
+int hm_hip_tree_upload(const HODLRInternalNode *node, int height,
+                       hm_dtree **out, void *stream) {
+  *out = nullptr;
+  if (!node || height < 1 || height > 24) {
+    set_error("hm_hip_tree_upload: bad node/height (%d)", height);
+    return INPUT_ERROR;
+  }
+  HM_OK(ensure_init());
+  hipStream_t s = stream_of(stream);
+  auto lev = host_levels(node, height);
+  hm_dtree *t = new hm_dtree();
+  t->height = height;
+  t->nleaf = 1 << height;
+  t->nblock = (1 << (height + 1)) - 2;
+  t->leaf_off.assign(t->nleaf + 1, 0);
+  const HODLRInternalNode *const *bottom = &lev[(1u << (height - 1)) - 1];
+  std::vector<const double *> leaf_data(t->nleaf);
+  for (int q = 0; q < t->nleaf / 2; q++) {
+    for (int c = 0; c < 2; c++) {
+      const NodeDiagonal &dg = bottom[q]->children[c ? 3 : 0].leaf->data.diagonal;
+      if (dg.m < 0 || (dg.m > 0 && !dg.data)) {
+        set_error("hm_hip_tree_upload: diagonal leaf %d has no data", 2 * q + c);
+        tree_free(t);
+        return INPUT_ERROR;
+      }
+      t->leaf_off[2 * q + c + 1] = dg.m;
+      leaf_data[2 * q + c] = dg.data;
+    }
+  }
+  for (int c = 0; c < t->nleaf; c++) t->leaf_off[c + 1] += t->leaf_off[c];
+  t->blk.resize(t->nblock);
+  std::vector<const double *> hu(t->nblock), hv(t->nblock);
+  for (int l = 1; l <= height; l++) {
+    const HODLRInternalNode *const *parents = &lev[(1u << (l - 1)) - 1];
+    int span = 1 << (height - l);
+    for (int j = 0; j < (1 << l); j++) {
+      const NodeOffDiagonal &od =
+          parents[j >> 1]->children[(j & 1) ? 2 : 1].leaf->data.off_diagonal;
+      BlockMeta &b = t->blk[blk_index(l, j)];
+      b.r0 = t->leaf_off[j * span];
+      b.m = t->leaf_off[(j + 1) * span] - b.r0;
+      int jc = j ^ 1;
+      b.c0 = t->leaf_off[jc * span];
+      b.n = t->leaf_off[(jc + 1) * span] - b.c0;
+      b.level = l;
+      b.j = j;
+      b.pad = 0;
+      b.s = od.s < 0 ? 0 : od.s;
+      if (od.m != b.m || od.n != b.n || (b.s > 0 && (!od.u || !od.v))) {
+        set_error("hm_hip_tree_upload: block level %d index %d inconsistent "
+                  "(m=%d/%d n=%d/%d s=%d)", l, j, od.m, b.m, od.n, b.n, od.s);
+        tree_free(t);
+        return INPUT_ERROR;
+      }
+      hu[blk_index(l, j)] = od.u;
+      hv[blk_index(l, j)] = od.v;
+    }
+  }
+  int rc = tree_finalize_meta(t, s);
+  if (rc == SUCCESS) rc = tree_alloc_pools(t);
+  if (rc != SUCCESS) {
+    tree_free(t);
+    return rc;
+  }
+  // pack on the host, one copy per pool
+  std::vector<double> stage((size_t)t->diag_off[t->nleaf] + 1);
+  for (int c = 0; c < t->nleaf; c++) {
+    long long cnt = t->diag_off[c + 1] - t->diag_off[c];
+    if (cnt) memcpy(stage.data() + t->diag_off[c], leaf_data[c], cnt * sizeof(double));
+  }
+  hipError_t e = hipMemcpy(t->d_diag, stage.data(),
+                           t->diag_off[t->nleaf] * sizeof(double),
+                           hipMemcpyHostToDevice);
+  long long utot = 0, vtot = 0;
+  for (auto &b : t->blk) {
+    utot += (long long)b.m * b.s;
+    vtot += (long long)b.n * b.s;
+  }
+  std::vector<double> su(utot + 1), sv(vtot + 1);
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    if (b.s == 0) continue;
+    memcpy(su.data() + b.uoff, hu[g], (size_t)b.m * b.s * sizeof(double));
+    memcpy(sv.data() + b.voff, hv[g], (size_t)b.n * b.s * sizeof(double));
+  }
+  if (e == hipSuccess && utot)
+    e = hipMemcpy(t->d_U, su.data(), utot * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && vtot)
+    e = hipMemcpy(t->d_V, sv.data(), vtot * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error("hm_hip_tree_upload: copy failed: %s", hipGetErrorString(e));
+    tree_free(t);
+    return ALLOCATION_FAILURE;
+  }
+  *out = t;
+  return SUCCESS;
+}
+
+int hm_hip_tree_download(const hm_dtree *t, TreeHODLR *host,
+                         void *(*alloc)(size_t), void *stream) {
+  if (!t || !host || host->height != t->height || !host->root) {
+    set_error("hm_hip_tree_download: tree/height mismatch");
+    return INPUT_ERROR;
+  }
+  HM_OK(ensure_init());
+  hipStream_t s = stream_of(stream);
+  if (!alloc) alloc = malloc;
+  const int h = t->height;
+  auto lev = host_levels(host->root, h);
+  // internal node sizes
+  for (int d = 0; d < h; d++) {
+    HODLRInternalNode *const *nodes =
+        const_cast<HODLRInternalNode *const *>(&lev[(1u << d) - 1]);
+    int span = 1 << (h - d);
+    for (int i = 0; i < (1 << d); i++)
+      nodes[i]->m = t->leaf_off[(i + 1) * span] - t->leaf_off[i * span];
+  }
+  std::vector<double> stage((size_t)t->diag_off[t->nleaf] + 1);
+  HM_HIP(hipMemcpyAsync(stage.data(), t->d_diag,
+                        t->diag_off[t->nleaf] * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+  long long utot = 0, vtot = 0;
+  for (auto &b : t->blk) {
+    utot += (long long)b.m * b.s;
+    vtot += (long long)b.n * b.s;
+  }
+  std::vector<double> su(utot + 1), sv(vtot + 1);
+  if (utot)
+    HM_HIP(hipMemcpyAsync(su.data(), t->d_U, utot * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  if (vtot)
+    HM_HIP(hipMemcpyAsync(sv.data(), t->d_V, vtot * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  HM_HIP(hipStreamSynchronize(s));
+  HODLRInternalNode *const *bottom =
+      const_cast<HODLRInternalNode *const *>(&lev[(1u << (h - 1)) - 1]);
+  for (int q = 0; q < t->nleaf / 2; q++) {
+    for (int c = 0; c < 2; c++) {
+      int leaf = 2 * q + c;
+      NodeDiagonal &dg = bottom[q]->children[c ? 3 : 0].leaf->data.diagonal;
+      int m = t->leaf_off[leaf + 1] - t->leaf_off[leaf];
+      dg.m = m;
+      size_t bytes = (size_t)m * m * sizeof(double);
+      dg.data = static_cast<double *>(alloc(bytes ? bytes : 8));
+      if (!dg.data) {
+        set_error("hm_hip_tree_download: host allocation failed");
+        return ALLOCATION_FAILURE;
+      }
+      if (bytes) memcpy(dg.data, stage.data() + t->diag_off[leaf], bytes);
+    }
+  }
+  for (int l = 1; l <= h; l++) {
+    HODLRInternalNode *const *parents =
+        const_cast<HODLRInternalNode *const *>(&lev[(1u << (l - 1)) - 1]);
+    for (int j = 0; j < (1 << l); j++) {
+      NodeOffDiagonal &od =
+          parents[j >> 1]->children[(j & 1) ? 2 : 1].leaf->data.off_diagonal;
+      const BlockMeta &b = t->blk[blk_index(l, j)];
+      od.m = b.m;
+      od.n = b.n;
+      od.s = b.s;
+      size_t ub = (size_t)b.m * b.s * sizeof(double);
+      size_t vb = (size_t)b.n * b.s * sizeof(double);
+      od.u = static_cast<double *>(alloc(ub ? ub : 8));
+      od.v = static_cast<double *>(alloc(vb ? vb : 8));
+      if (!od.u || !od.v) {
+        set_error("hm_hip_tree_download: host allocation failed");
+        return ALLOCATION_FAILURE;
+      }
+      if (ub) memcpy(od.u, su.data() + b.uoff, ub);
+      if (vb) memcpy(od.v, sv.data() + b.voff, vb);
+    }
+  }
+  return SUCCESS;
+}
+
+int hm_hip_tree_destroy(hm_dtree *t) {
+  tree_free(t);
+  return SUCCESS;
+}
+
+int hm_hip_tree_info(const hm_dtree *t, hm_tree_info *info) {
+  if (!t || !info) return INPUT_ERROR;
+  info->height = t->height;
+  info->n = t->n;
+  info->nleaf = t->nleaf;
+  info->nblock = t->nblock;
+  info->max_rank = t->max_rank;
+  long long srd = 0;
+  for (auto &b : t->blk) srd += (long long)(b.m + b.n) * b.s;
+  info->sum_rank_dims = srd;
+  info->diag_elems = t->diag_off[t->nleaf];
+  return SUCCESS;
+}
+
+int hm_hip_tree_ranks(const hm_dtree *t, int *ranks) {
+  if (!t || !ranks) return INPUT_ERROR;
+  for (int g = 0; g < t->nblock; g++) ranks[g] = t->blk[g].s;
+  return SUCCESS;
+}
+
+}  // extern "C"

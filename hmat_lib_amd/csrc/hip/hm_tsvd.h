@@ -1,0 +1,54 @@
+// Batched truncated SVD of dense blocks (internal interface).
+#pragma once
+
+#include "hm_common.h"
+
+namespace hm {
+
+enum TsvdRule { RULE_RELATIVE = 0, RULE_ABSOLUTE = 1 };
+
+// One dense block to factor as U V^T (U carries sigma).
+struct TsvdBlock {
+  const double *A;
+  long long lda;
+  int m, n;
+  int ell;       // sketch width
+  int ntest;     // posterior test vectors (0 = exact width)
+  int rule;
+  int status;    // out: 1 accepted
+  int k;         // out: rank
+  int pad;
+  double thr;
+  double err;    // out: residual estimate
+  double sigma0; // out: sigma_0 lower bound, then exact top singular value
+  long long yoff, ooff, noff, qoff, boff, moff, soff;
+  unsigned long long uid;
+  double *uout;  // out: U (m x k, ld m) and V (n x k, ld n)
+  double *vout;
+};
+
+struct TsvdOptions {
+  int sketch_max = 512;
+  int n_test = 6;
+  double accept = 0.1;
+  unsigned long long seed = 0x9e3779b97f4a7c15ull;
+};
+
+struct TsvdStats {
+  int rounds = 0;
+  int retried = 0;
+  int max_sketch = 0;
+  int exact_blocks = 0;
+  int capped = 0;      // blocks that hit sketch_max without passing the test
+};
+
+// Factor every block.  On entry blocks[i].{A,lda,m,n,ell,rule,thr,uid} are
+// set.  On return .k is set and U/V of block i live at .uout/.vout inside
+// pools allocated here and returned in *u_pool / *v_pool (caller frees).
+// sigma (optional, host) receives the sorted singular values of the final
+// core of each block, up to `sigma_cap` per block.
+int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
+             double **u_pool, double **v_pool, TsvdStats *stats,
+             std::vector<double> *sigma, int sigma_cap, hipStream_t s);
+
+}  // namespace hm

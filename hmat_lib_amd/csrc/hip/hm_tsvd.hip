@@ -1,0 +1,999 @@
+// Batched truncated SVD of dense blocks on the GPU (FP64).
+//
+// The reference factors every off-diagonal block with a full LAPACK dgesdd
+// (src/constructors.c:308-367, src/lapack_wrapper.c:8-59) and keeps the
+// leading singular triplets by its rank rule.  Here each block A (m x n)
+// goes through a randomized range finder with a posterior error test
+// (Halko, Martinsson & Tropp 2011, Alg. 4.2 and Lemma 4.1):
+//
+//   1. Y = A [Omega | Omega_t]        Gaussian sketch, ell + ntest columns
+//   2. Y[:, :ell] = Q R               Householder QR (LAPACK dlarfg signs)
+//   3. err = 10 sqrt(2/pi) max |(I - Q Q^T) A w_t|   for the test columns;
+//      accept when err <= accept * thr * sigma0 (relative rule) or
+//      <= accept * thr (absolute rule); otherwise ell doubles
+//   4. B^T = A^T Q = Q2 R2            second pass over A, Householder QR
+//   5. R2^T = W S Z^T                 one-sided Jacobi SVD (ell x ell)
+//   6. U = Q (R2^T Z)[:, :k] = Q W S,  V = Q2 Z[:, :k]
+//
+// When ell reaches min(m, n) the sketch spans the whole range of A and the
+// result is an exact SVD (no test needed), so small blocks are factored
+// exactly.  sigma_j(B)^2 >= sigma_j(A)^2 - err^2, so with err <= 0.1 thr
+// sigma0 a rank decision can differ from a full SVD only when a singular
+// value lies within 0.5% above the threshold.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "hm_tsvd.h"
+
+namespace hm {
+
+static constexpr int QR_NT = 512;
+static constexpr int MAX_ELL = 1024;
+
+// ------------------------------------------------------------ RNG
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double gauss(unsigned long long key, long long i) {
+  unsigned long long a = mix64(key + 2ull * (unsigned long long)i);
+  unsigned long long b = mix64(key + 2ull * (unsigned long long)i + 1ull);
+  double u1 = ((double)(a >> 11) + 0.5) * 0x1.0p-53;
+  double u2 = (double)(b >> 11) * 0x1.0p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// ------------------------------------------------- block reductions
+template <int NV, int NT>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *red,
+                                          double *out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) red[i * NW + w] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int ww = 0; ww < NW; ww++) s += red[threadIdx.x * NW + ww];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// first row index >= r that thread `tid` owns (rows i with i % NT == tid)
+__device__ __forceinline__ int own_from(int tid, int r, int NT) {
+  if (tid >= r) return tid;
+  return tid + ((r - tid + NT - 1) / NT) * NT;
+}
+
+// --------------------------------------------------- kernels
+struct TsvdBufs {
+  double *Y, *Om, *onorm, *Q, *Bt, *Q2, *M, *Vj, *G, *Z, *Sig;
+};
+
+__global__ __launch_bounds__(256) void k_gen_omega(TsvdBlock *blocks,
+                                                   const int2 *work,
+                                                   TsvdBufs bf,
+                                                   unsigned long long seed) {
+  __shared__ double red[4], out[1];
+  const int2 w = work[blockIdx.x];
+  const TsvdBlock &B = blocks[w.x];
+  const int j = w.y;
+  const int n = B.n;
+  const unsigned long long key = mix64(seed ^ mix64(B.uid * 0x100000001b3ull + (unsigned long long)j));
+  double *col = bf.Om + B.ooff + (long long)j * n;
+  double p[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double g = gauss(key, i);
+    col[i] = g;
+    p[0] += g * g;
+  }
+  block_sum<1, 256>(p, red, out);
+  if (threadIdx.x == 0) bf.onorm[B.noff + j] = sqrt(out[0]);
+}
+
+// Y = A Omega, tiles of 128 rows x 32 sketch columns.
+typedef MfmaTile<4, 1, 2, 2, 16> SketchTile;
+
+__global__ __launch_bounds__(256) void k_sketch(TsvdBlock *blocks,
+                                                const int4 *work, TsvdBufs bf) {
+  __shared__ double lds[SketchTile::LDS_DOUBLES];
+  const int4 w = work[blockIdx.x];
+  const TsvdBlock B = blocks[w.x];
+  const int m = B.m, n = B.n, L = B.ell + B.ntest;
+  const int i0 = w.y * SketchTile::TM, j0 = w.z * SketchTile::TN;
+  const double *A = B.A;
+  const double *Om = bf.Om + B.ooff;
+  auto la = [&](int i, int k) -> double {
+    return (i0 + i < m && k < n) ? A[(long long)(i0 + i) + (long long)k * B.lda] : 0.0;
+  };
+  auto lb = [&](int k, int j) -> double {
+    return (k < n && j0 + j < L) ? Om[k + (long long)(j0 + j) * n] : 0.0;
+  };
+  SketchTile tile;
+  tile.zero();
+  tile.run<true, false>(lds, n, la, lb);
+  double *Y = bf.Y + B.yoff;
+  tile.store([&](int i, int j, double v) {
+    if (i0 + i < m && j0 + j < L) Y[(i0 + i) + (long long)(j0 + j) * m] = v;
+  });
+}
+
+// B^T = A^T Q, tiles of 128 (columns of A) x 32 (ell).
+__global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
+                                                 const int4 *work, TsvdBufs bf) {
+  __shared__ double lds[SketchTile::LDS_DOUBLES];
+  const int4 w = work[blockIdx.x];
+  const TsvdBlock B = blocks[w.x];
+  const int m = B.m, n = B.n, ell = B.ell;
+  const int i0 = w.y * SketchTile::TM, j0 = w.z * SketchTile::TN;
+  const double *A = B.A;
+  const double *Q = bf.Q + B.qoff;
+  auto la = [&](int i, int k) -> double {
+    return (i0 + i < n && k < m) ? A[(long long)k + (long long)(i0 + i) * B.lda] : 0.0;
+  };
+  auto lb = [&](int k, int j) -> double {
+    return (k < m && j0 + j < ell) ? Q[k + (long long)(j0 + j) * m] : 0.0;
+  };
+  SketchTile tile;
+  tile.zero();
+  tile.run<false, false>(lds, m, la, lb);
+  double *Bt = bf.Bt + B.boff;
+  tile.store([&](int i, int j, double v) {
+    if (i0 + i < n && j0 + j < ell) Bt[(i0 + i) + (long long)(j0 + j) * n] = v;
+  });
+}
+
+// Householder QR of a rows x nc panel (in place), explicit Q, and either
+// the posterior residual test (mode 0, sketch) or M = R^T (mode 1).
+__global__ __launch_bounds__(QR_NT) void k_qr(TsvdBlock *blocks,
+                                              const int *list, int mode,
+                                              TsvdBufs bf, double accept) {
+  __shared__ double red[8 * (QR_NT / 64)], out[8];
+  __shared__ double tau_s[MAX_ELL];
+  __shared__ double cvec[MAX_ELL];
+  TsvdBlock &B = blocks[list[blockIdx.x]];
+  const int tid = threadIdx.x;
+  const int nc = B.ell;
+  int rows, nt;
+  double *Y, *Q;
+  if (mode == 0) {
+    rows = B.m;
+    nt = B.ntest;
+    Y = bf.Y + B.yoff;
+    Q = bf.Q + B.qoff;
+  } else {
+    rows = B.n;
+    nt = 0;
+    Y = bf.Bt + B.boff;
+    Q = bf.Q2 + B.boff;
+  }
+  const long long ld = rows;
+  double sigma_lb = 0.0;
+  if (mode == 0 && nt > 0) {
+    const int L = nc + nt;
+    for (int jb = 0; jb < L; jb += 8) {
+      const int nv = min(8, L - jb);
+      double p[8];
+#pragma unroll
+      for (int v = 0; v < 8; v++) p[v] = 0.0;
+      for (int i = tid; i < rows; i += QR_NT) {
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+          if (v < nv) {
+            double y = Y[i + (long long)(jb + v) * ld];
+            p[v] += y * y;
+          }
+      }
+      block_sum<8, QR_NT>(p, red, out);
+      for (int v = 0; v < nv; v++) {
+        double on = bf.onorm[B.noff + jb + v];
+        if (on > 0.0) sigma_lb = fmax(sigma_lb, sqrt(out[v]) / on);
+      }
+    }
+  }
+  // factorization
+  for (int k = 0; k < nc; k++) {
+    double p[1] = {0.0};
+    for (int i = own_from(tid, k + 1, QR_NT); i < rows; i += QR_NT) {
+      double x = Y[i + (long long)k * ld];
+      p[0] += x * x;
+    }
+    block_sum<1, QR_NT>(p, red, out);
+    const double xn2 = out[0];
+    const double alpha = Y[k + (long long)k * ld];
+    double tau = 0.0, beta = alpha, scal = 1.0;
+    if (xn2 > 0.0) {
+      double h = hypot(alpha, sqrt(xn2));
+      beta = (alpha >= 0.0) ? -h : h;
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
+      if (i == k) Y[i + (long long)k * ld] = beta;
+      else if (tau != 0.0) Y[i + (long long)k * ld] *= scal;
+    }
+    if (tid == 0) tau_s[k] = tau;
+    if (tau == 0.0) continue;
+    for (int jb = k + 1; jb < nc; jb += 8) {
+      const int nv = min(8, nc - jb);
+      double q[8];
+#pragma unroll
+      for (int v = 0; v < 8; v++) q[v] = 0.0;
+      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
+        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+          if (v < nv) q[v] += vi * Y[i + (long long)(jb + v) * ld];
+      }
+      block_sum<8, QR_NT>(q, red, out);
+      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
+        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+          if (v < nv) Y[i + (long long)(jb + v) * ld] -= tau * out[v] * vi;
+      }
+    }
+  }
+  __syncthreads();
+  if (mode == 1) {
+    double *M = bf.M + B.moff;
+    for (int e = tid; e < nc * nc; e += QR_NT) {
+      const int j = e % nc, i = e / nc;  // M[j + i*nc] = R[i][j]
+      M[j + (long long)i * nc] = (i <= j) ? Y[i + (long long)j * ld] : 0.0;
+    }
+  }
+  // explicit Q = H_0 ... H_{nc-1} [I; 0]
+  for (int i = tid; i < rows; i += QR_NT)
+    for (int j = 0; j < nc; j++) Q[i + (long long)j * ld] = (i == j) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int k = nc - 1; k >= 0; k--) {
+    const double tau = tau_s[k];
+    if (tau == 0.0) continue;
+    for (int jb = k; jb < nc; jb += 8) {
+      const int nv = min(8, nc - jb);
+      double q[8];
+#pragma unroll
+      for (int v = 0; v < 8; v++) q[v] = 0.0;
+      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
+        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+          if (v < nv) q[v] += vi * Q[i + (long long)(jb + v) * ld];
+      }
+      block_sum<8, QR_NT>(q, red, out);
+      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
+        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+          if (v < nv) Q[i + (long long)(jb + v) * ld] -= tau * out[v] * vi;
+      }
+    }
+  }
+  __syncthreads();
+  if (mode == 1) return;
+  // posterior test on the untouched test columns
+  double errmax = 0.0;
+  for (int t = 0; t < nt; t++) {
+    const double *yt = Y + (long long)(nc + t) * ld;
+    for (int jb = 0; jb < nc; jb += 8) {
+      const int nv = min(8, nc - jb);
+      double q[8];
+#pragma unroll
+      for (int v = 0; v < 8; v++) q[v] = 0.0;
+      for (int i = tid; i < rows; i += QR_NT) {
+        const double y = yt[i];
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+          if (v < nv) q[v] += Q[i + (long long)(jb + v) * ld] * y;
+      }
+      block_sum<8, QR_NT>(q, red, out);
+      if (tid < nv) cvec[jb + tid] = out[tid];
+    }
+    __syncthreads();
+    double p[1] = {0.0};
+    for (int i = tid; i < rows; i += QR_NT) {
+      double r = yt[i];
+      for (int e = 0; e < nc; e++) r -= Q[i + (long long)e * ld] * cvec[e];
+      p[0] += r * r;
+    }
+    block_sum<1, QR_NT>(p, red, out);
+    errmax = fmax(errmax, sqrt(out[0]));
+  }
+  if (tid == 0) {
+    const double est = 10.0 * 0.7978845608028654 * errmax;
+    const double scale = (B.rule == RULE_RELATIVE) ? sigma_lb : 1.0;
+    B.err = est;
+    B.sigma0 = sigma_lb;
+    B.status = (nt == 0) || (est <= accept * B.thr * scale) ? 1 : 0;
+  }
+}
+
+// One-sided Jacobi SVD of M (nc x nc); V accumulates the rotations.
+// Outputs G = (M V)[:, perm[:k]] (U carries sigma), Z = V[:, perm[:k]],
+// sorted singular values and the rank k by the block's rule.
+__global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
+                                                const int *list, TsvdBufs bf) {
+  __shared__ double sig[MAX_ELL];
+  __shared__ int perm[MAX_ELL];
+  __shared__ int rotated;
+  TsvdBlock &B = blocks[list[blockIdx.x]];
+  const int nc = B.ell;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NW = 4;
+  double *M = bf.M + B.moff;
+  double *V = bf.Vj + B.moff;
+  for (int e = threadIdx.x; e < nc * nc; e += 256)
+    V[e] = ((e % nc) == (e / nc)) ? 1.0 : 0.0;
+  __syncthreads();
+  const int np = nc + (nc & 1);
+  const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
+  for (int sweep = 0; sweep < 60 && nc > 1; sweep++) {
+    if (threadIdx.x == 0) rotated = 0;
+    __syncthreads();
+    for (int r = 0; r < np - 1; r++) {
+      for (int pi = wave; pi < np / 2; pi += NW) {
+        const int pa = pi, pb = np - 1 - pi;
+        const int a = (pa == 0) ? 0 : ((pa - 1 + r) % (np - 1)) + 1;
+        const int b = (pb == 0) ? 0 : ((pb - 1 + r) % (np - 1)) + 1;
+        if (a >= nc || b >= nc) continue;
+        const int p = min(a, b), q = max(a, b);
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = lane; i < nc; i += 64) {
+          const double mp = M[i + (long long)p * nc], mq = M[i + (long long)q * nc];
+          al += mp * mp;
+          be += mq * mq;
+          ga += mp * mq;
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        if (ga == 0.0 || fabs(ga) <= tol * sqrt(al * be)) continue;
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+        for (int i = lane; i < nc; i += 64) {
+          const double mp = M[i + (long long)p * nc], mq = M[i + (long long)q * nc];
+          M[i + (long long)p * nc] = c * mp - s * mq;
+          M[i + (long long)q * nc] = s * mp + c * mq;
+          const double vp = V[i + (long long)p * nc], vq = V[i + (long long)q * nc];
+          V[i + (long long)p * nc] = c * vp - s * vq;
+          V[i + (long long)q * nc] = s * vp + c * vq;
+        }
+        if (lane == 0) rotated = 1;
+      }
+      __syncthreads();
+    }
+    if (!rotated) break;
+  }
+  __syncthreads();
+  for (int j = wave; j < nc; j += NW) {
+    double ss = 0.0;
+    for (int i = lane; i < nc; i += 64) {
+      const double x = M[i + (long long)j * nc];
+      ss += x * x;
+    }
+    ss = wave_sum(ss);
+    if (lane == 0) sig[j] = sqrt(ss);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nc; j += 256) {
+    int rank = 0;
+    const double sj = sig[j];
+    for (int i = 0; i < nc; i++) {
+      const double si = sig[i];
+      rank += (si > sj) || (si == sj && i < j);
+    }
+    perm[rank] = j;
+  }
+  __syncthreads();
+  __shared__ int kk;
+  if (threadIdx.x == 0) {
+    const double s0 = sig[perm[0]];
+    int cut = 1;
+    if (B.rule == RULE_RELATIVE) {
+      if (s0 > B.thr)
+        for (cut = 1; cut < nc; cut++)
+          if (sig[perm[cut]] < B.thr * s0) break;
+    } else {
+      for (cut = 1; cut < nc; cut++)
+        if (sig[perm[cut]] <= B.thr) break;
+    }
+    B.k = cut;
+    B.sigma0 = s0;
+    kk = cut;
+  }
+  __syncthreads();
+  double *S = bf.Sig + B.soff;
+  for (int j = threadIdx.x; j < nc; j += 256) S[j] = sig[perm[j]];
+  double *G = bf.G + B.moff, *Z = bf.Z + B.moff;
+  for (int e = threadIdx.x; e < nc * kk; e += 256) {
+    const int i = e % nc, j = e / nc;
+    G[i + (long long)j * nc] = M[i + (long long)perm[j] * nc];
+    Z[i + (long long)j * nc] = V[i + (long long)perm[j] * nc];
+  }
+}
+
+// U = Q G (m x k) and V = Q2 Z (n x k).  work = (block, side, row chunk).
+__global__ __launch_bounds__(256) void k_form(TsvdBlock *blocks, const int4 *work,
+                                              TsvdBufs bf) {
+  const int4 w = work[blockIdx.x];
+  const TsvdBlock B = blocks[w.x];
+  const int side = w.y;
+  const int rows = side ? B.n : B.m;
+  const int i = w.z * 256 + threadIdx.x;
+  if (i >= rows) return;
+  const int nc = B.ell, k = B.k;
+  const double *Qb = side ? bf.Q2 + B.boff : bf.Q + B.qoff;
+  const double *Gb = side ? bf.Z + B.moff : bf.G + B.moff;
+  double *O = side ? B.vout : B.uout;
+  for (int j = 0; j < k; j++) {
+    double acc = 0.0;
+    for (int e = 0; e < nc; e++) acc += Qb[i + (long long)e * rows] * Gb[e + (long long)j * nc];
+    O[i + (long long)j * rows] = acc;
+  }
+}
+
+struct CopyItem {
+  const double *src;
+  double *dst;
+  long long count;
+};
+
+__global__ __launch_bounds__(256) void k_copy_list(const CopyItem *items) {
+  const CopyItem it = items[blockIdx.x];
+  for (long long e = threadIdx.x + (long long)blockIdx.y * 256; e < it.count;
+       e += 256ll * gridDim.y)
+    it.dst[e] = it.src[e];
+}
+
+// M = R_U R_V^T for recompression: R_U is the upper triangle left in the
+// U panel (Y slot, ld m) and R_V the one left in the V panel (Bt slot,
+// ld n) by k_qr.
+__global__ __launch_bounds__(256) void k_rmul(TsvdBlock *blocks, const int *list,
+                                              TsvdBufs bf) {
+  const TsvdBlock B = blocks[list[blockIdx.x]];
+  const int s = B.ell;
+  const double *RU = bf.Y + B.yoff, *RV = bf.Bt + B.boff;
+  double *M = bf.M + B.moff;
+  for (int e = threadIdx.x; e < s * s; e += 256) {
+    const int i = e % s, j = e / s;
+    double acc = 0.0;
+    for (int t = max(i, j); t < s; t++)
+      acc += RU[i + (long long)t * B.m] * RV[j + (long long)t * B.n];
+    M[i + (long long)j * s] = acc;
+  }
+}
+
+// D = U V^T (m x n, ld m) for the dense recompression method.
+typedef MfmaTile<2, 2, 2, 2, 16> DenseTile;
+struct OuterJob {
+  const double *U, *V;
+  double *D;
+  int m, n, s, pad;
+};
+__global__ __launch_bounds__(256) void k_outer(const OuterJob *jobs, const int4 *work) {
+  __shared__ double lds[DenseTile::LDS_DOUBLES];
+  const int4 w = work[blockIdx.x];
+  const OuterJob J = jobs[w.x];
+  const int i0 = w.y * DenseTile::TM, j0 = w.z * DenseTile::TN;
+  auto la = [&](int i, int k) -> double {
+    return (i0 + i < J.m && k < J.s) ? J.U[(i0 + i) + (long long)k * J.m] : 0.0;
+  };
+  auto lb = [&](int k, int j) -> double {
+    return (k < J.s && j0 + j < J.n) ? J.V[(j0 + j) + (long long)k * J.n] : 0.0;
+  };
+  DenseTile tile;
+  tile.zero();
+  tile.run<true, true>(lds, J.s, la, lb);
+  tile.store([&](int i, int j, double v) {
+    if (i0 + i < J.m && j0 + j < J.n) J.D[(i0 + i) + (long long)(j0 + j) * J.m] = v;
+  });
+}
+
+// ------------------------------------------------------------ driver
+template <class T>
+static int to_device(T **d, const std::vector<T> &h, hipStream_t s) {
+  HM_OK(dalloc(d, h.size()));
+  if (!h.empty())
+    HM_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T),
+                          hipMemcpyHostToDevice, s));
+  return SUCCESS;
+}
+
+namespace {
+struct Scoped {
+  std::vector<void *> ptrs;
+  ~Scoped() {
+    for (void *p : ptrs) dfree(p);
+  }
+  template <class T>
+  T *keep(T *p) {
+    ptrs.push_back((void *)p);
+    return p;
+  }
+};
+}  // namespace
+
+int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
+             double **u_pool, double **v_pool, TsvdStats *stats,
+             std::vector<double> *sigma, int sigma_cap, hipStream_t s) {
+  *u_pool = nullptr;
+  *v_pool = nullptr;
+  TsvdStats st;
+  const int nb = (int)blocks.size();
+  const int cap = std::min(opt.sketch_max, MAX_ELL);
+  std::vector<int> active;
+  for (int i = 0; i < nb; i++) {
+    TsvdBlock &b = blocks[i];
+    b.k = 0;
+    b.status = 0;
+    b.uout = b.vout = nullptr;
+    const int mn = std::min(b.m, b.n);
+    if (mn <= 0) {
+      b.k = 1;  // empty block: rank 1, zero factors (handled on the host side)
+      b.ell = 0;
+      continue;
+    }
+    b.ell = std::max(1, std::min(std::min(b.ell, mn), cap));
+    b.ntest = (b.ell >= mn) ? 0 : opt.n_test;
+    active.push_back(i);
+  }
+  if (sigma) sigma->assign((size_t)nb * sigma_cap, 0.0);
+  struct RoundOut {
+    double *u, *v;
+  };
+  std::vector<RoundOut> round_pools;
+  Scoped round_mem;
+  std::vector<long long> uoff_final(nb, 0), voff_final(nb, 0);
+  while (!active.empty()) {
+    st.rounds++;
+    Scoped mem;
+    // offsets for this round
+    long long ytot = 0, otot = 0, ntot = 0, qtot = 0, btot = 0, mtot = 0, stot = 0;
+    for (int i : active) {
+      TsvdBlock &b = blocks[i];
+      const int L = b.ell + b.ntest;
+      b.yoff = ytot; ytot += (long long)b.m * L;
+      b.ooff = otot; otot += (long long)b.n * L;
+      b.noff = ntot; ntot += L;
+      b.qoff = qtot; qtot += (long long)b.m * b.ell;
+      b.boff = btot; btot += (long long)b.n * b.ell;
+      b.moff = mtot; mtot += (long long)b.ell * b.ell;
+      b.soff = stot; stot += b.ell;
+      b.status = 0;
+      st.max_sketch = std::max(st.max_sketch, b.ell);
+    }
+    TsvdBufs bf;
+    HM_OK(dalloc(&bf.Y, ytot)); mem.keep(bf.Y);
+    HM_OK(dalloc(&bf.Om, otot)); mem.keep(bf.Om);
+    HM_OK(dalloc(&bf.onorm, ntot)); mem.keep(bf.onorm);
+    HM_OK(dalloc(&bf.Q, qtot)); mem.keep(bf.Q);
+    HM_OK(dalloc(&bf.Bt, btot)); mem.keep(bf.Bt);
+    HM_OK(dalloc(&bf.Q2, btot)); mem.keep(bf.Q2);
+    HM_OK(dalloc(&bf.M, mtot)); mem.keep(bf.M);
+    HM_OK(dalloc(&bf.Vj, mtot)); mem.keep(bf.Vj);
+    HM_OK(dalloc(&bf.G, mtot)); mem.keep(bf.G);
+    HM_OK(dalloc(&bf.Z, mtot)); mem.keep(bf.Z);
+    HM_OK(dalloc(&bf.Sig, stot)); mem.keep(bf.Sig);
+    // device copy of all blocks (indices are global block numbers)
+    TsvdBlock *d_blocks = nullptr;
+    HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
+    // work lists, biggest K first for balance
+    std::vector<int> order = active;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+      return (long long)blocks[x].n > (long long)blocks[y].n;
+    });
+    std::vector<int2> wo;
+    std::vector<int4> ws;
+    for (int i : order) {
+      const TsvdBlock &b = blocks[i];
+      const int L = b.ell + b.ntest;
+      for (int j = 0; j < L; j++) wo.push_back(make_int2(i, j));
+      const int mt = (b.m + SketchTile::TM - 1) / SketchTile::TM;
+      const int ntl = (L + SketchTile::TN - 1) / SketchTile::TN;
+      for (int a = 0; a < mt; a++)
+        for (int c = 0; c < ntl; c++) ws.push_back(make_int4(i, a, c, 0));
+    }
+    int2 *d_wo;
+    int4 *d_ws;
+    int *d_list;
+    HM_OK(to_device(&d_wo, wo, s)); mem.keep(d_wo);
+    HM_OK(to_device(&d_ws, ws, s)); mem.keep(d_ws);
+    HM_OK(to_device(&d_list, order, s)); mem.keep(d_list);
+    {
+      Timed t0 = timing_begin("tsvd_omega", s);
+      hipLaunchKernelGGL(k_gen_omega, dim3((unsigned)wo.size()), dim3(256), 0, s,
+                         d_blocks, d_wo, bf, opt.seed);
+      timing_end(t0, s);
+      HM_LAUNCH_CHECK();
+      Timed t1 = timing_begin("tsvd_sketch", s);
+      hipLaunchKernelGGL(k_sketch, dim3((unsigned)ws.size()), dim3(256), 0, s,
+                         d_blocks, d_ws, bf);
+      timing_end(t1, s);
+      HM_LAUNCH_CHECK();
+      Timed t2 = timing_begin("tsvd_qr", s);
+      hipLaunchKernelGGL(k_qr, dim3((unsigned)order.size()), dim3(QR_NT), 0, s,
+                         d_blocks, d_list, 0, bf, opt.accept);
+      timing_end(t2, s);
+      HM_LAUNCH_CHECK();
+    }
+    HM_HIP(hipMemcpyAsync(blocks.data(), d_blocks, nb * sizeof(TsvdBlock),
+                          hipMemcpyDeviceToHost, s));
+    HM_HIP(hipStreamSynchronize(s));
+    std::vector<int> done, retry;
+    for (int i : order) {
+      TsvdBlock &b = blocks[i];
+      const int mn = std::min(b.m, b.n);
+      if (b.status || b.ell >= cap) {
+        if (!b.status) st.capped++;
+        if (b.ntest == 0) st.exact_blocks++;
+        done.push_back(i);
+      } else {
+        retry.push_back(i);
+      }
+      (void)mn;
+    }
+    st.retried += (int)retry.size();
+    if (!done.empty()) {
+      std::vector<int4> wp;
+      for (int i : done) {
+        const TsvdBlock &b = blocks[i];
+        const int mt = (b.n + SketchTile::TM - 1) / SketchTile::TM;
+        const int ntl = (b.ell + SketchTile::TN - 1) / SketchTile::TN;
+        for (int a = 0; a < mt; a++)
+          for (int c = 0; c < ntl; c++) wp.push_back(make_int4(i, a, c, 0));
+      }
+      int4 *d_wp;
+      int *d_done;
+      HM_OK(to_device(&d_wp, wp, s)); mem.keep(d_wp);
+      HM_OK(to_device(&d_done, done, s)); mem.keep(d_done);
+      Timed t3 = timing_begin("tsvd_project", s);
+      hipLaunchKernelGGL(k_project, dim3((unsigned)wp.size()), dim3(256), 0, s,
+                         d_blocks, d_wp, bf);
+      timing_end(t3, s);
+      HM_LAUNCH_CHECK();
+      Timed t4 = timing_begin("tsvd_qr2", s);
+      hipLaunchKernelGGL(k_qr, dim3((unsigned)done.size()), dim3(QR_NT), 0, s,
+                         d_blocks, d_done, 1, bf, opt.accept);
+      timing_end(t4, s);
+      HM_LAUNCH_CHECK();
+      Timed t5 = timing_begin("tsvd_jacobi", s);
+      hipLaunchKernelGGL(k_jacobi, dim3((unsigned)done.size()), dim3(256), 0, s,
+                         d_blocks, d_done, bf);
+      timing_end(t5, s);
+      HM_LAUNCH_CHECK();
+      std::vector<TsvdBlock> hb(nb);
+      HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
+                            hipMemcpyDeviceToHost, s));
+      HM_HIP(hipStreamSynchronize(s));
+      long long ut = 0, vt = 0;
+      for (int i : done) {
+        blocks[i].k = hb[i].k;
+        blocks[i].sigma0 = hb[i].sigma0;
+        ut += (long long)blocks[i].m * blocks[i].k;
+        vt += (long long)blocks[i].n * blocks[i].k;
+      }
+      RoundOut ro;
+      HM_OK(dalloc(&ro.u, ut)); round_mem.keep(ro.u);
+      HM_OK(dalloc(&ro.v, vt)); round_mem.keep(ro.v);
+      round_pools.push_back(ro);
+      long long uo = 0, vo = 0;
+      for (int i : done) {
+        blocks[i].uout = ro.u + uo;
+        blocks[i].vout = ro.v + vo;
+        uo += (long long)blocks[i].m * blocks[i].k;
+        vo += (long long)blocks[i].n * blocks[i].k;
+      }
+      HM_HIP(hipMemcpyAsync(d_blocks, blocks.data(), nb * sizeof(TsvdBlock),
+                            hipMemcpyHostToDevice, s));
+      std::vector<int4> wf;
+      for (int i : done) {
+        const TsvdBlock &b = blocks[i];
+        for (int side = 0; side < 2; side++) {
+          const int rows = side ? b.n : b.m;
+          for (int c = 0; c < (rows + 255) / 256; c++) wf.push_back(make_int4(i, side, c, 0));
+        }
+      }
+      int4 *d_wf;
+      HM_OK(to_device(&d_wf, wf, s)); mem.keep(d_wf);
+      Timed t6 = timing_begin("tsvd_form", s);
+      hipLaunchKernelGGL(k_form, dim3((unsigned)wf.size()), dim3(256), 0, s,
+                         d_blocks, d_wf, bf);
+      timing_end(t6, s);
+      HM_LAUNCH_CHECK();
+      if (sigma) {
+        std::vector<double> hs(stot);
+        HM_HIP(hipMemcpyAsync(hs.data(), bf.Sig, stot * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        for (int i : done) {
+          const TsvdBlock &b = blocks[i];
+          for (int j = 0; j < std::min(b.ell, sigma_cap); j++)
+            (*sigma)[(size_t)i * sigma_cap + j] = hs[b.soff + j];
+        }
+      }
+      HM_HIP(hipStreamSynchronize(s));
+    }
+    for (int i : retry) {
+      TsvdBlock &b = blocks[i];
+      const int mn = std::min(b.m, b.n);
+      b.ell = std::min(std::min(2 * b.ell, mn), cap);
+      b.ntest = (b.ell >= mn) ? 0 : opt.n_test;
+    }
+    active = retry;
+  }
+  // gather into two packed pools
+  long long ut = 0, vt = 0;
+  for (int i = 0; i < nb; i++) {
+    ut += (long long)blocks[i].m * blocks[i].k;
+    vt += (long long)blocks[i].n * blocks[i].k;
+  }
+  double *up = nullptr, *vp = nullptr;
+  HM_OK(dalloc(&up, ut));
+  int rc = dalloc(&vp, vt);
+  if (rc != SUCCESS) {
+    dfree(up);
+    return rc;
+  }
+  HM_HIP(hipMemsetAsync(up, 0, (ut + 1) * sizeof(double), s));
+  HM_HIP(hipMemsetAsync(vp, 0, (vt + 1) * sizeof(double), s));
+  std::vector<CopyItem> items;
+  long long uo = 0, vo = 0;
+  for (int i = 0; i < nb; i++) {
+    TsvdBlock &b = blocks[i];
+    if (b.uout) {
+      items.push_back({b.uout, up + uo, (long long)b.m * b.k});
+      items.push_back({b.vout, vp + vo, (long long)b.n * b.k});
+    } else if (b.n > 0 && b.k > 0) {
+      // empty block: V = e_0
+      const double one = 1.0;
+      HM_HIP(hipMemcpyAsync(vp + vo, &one, sizeof(double), hipMemcpyHostToDevice, s));
+      HM_HIP(hipStreamSynchronize(s));
+    }
+    b.uout = up + uo;
+    b.vout = vp + vo;
+    uo += (long long)b.m * b.k;
+    vo += (long long)b.n * b.k;
+  }
+  if (!items.empty()) {
+    CopyItem *d_items = nullptr;
+    rc = to_device(&d_items, items, s);
+    if (rc == SUCCESS) {
+      hipLaunchKernelGGL(k_copy_list, dim3((unsigned)items.size(), 8), dim3(256),
+                         0, s, d_items);
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(s);
+    }
+    dfree(d_items);
+  }
+  HM_HIP(hipStreamSynchronize(s));
+  *u_pool = up;
+  *v_pool = vp;
+  if (stats) *stats = st;
+  return SUCCESS;
+}
+
+// QR-method recompression of factored nodes (s < min(m, n)).
+static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
+                         double *const *d_u, double *const *d_v, double thr,
+                         int *k_out, hipStream_t s) {
+  std::vector<TsvdBlock> blocks(nn);
+  long long ytot = 0, btot = 0, mtot = 0, stot = 0;
+  for (int i = 0; i < nn; i++) {
+    TsvdBlock &b = blocks[i];
+    memset(&b, 0, sizeof(b));
+    b.m = m[i];
+    b.n = n[i];
+    b.ell = sr[i];
+    b.ntest = 0;
+    b.rule = RULE_ABSOLUTE;
+    b.thr = thr;
+    b.yoff = b.qoff = ytot;
+    ytot += (long long)b.m * b.ell;
+    b.boff = btot;
+    btot += (long long)b.n * b.ell;
+    b.moff = mtot;
+    mtot += (long long)b.ell * b.ell;
+    b.soff = stot;
+    stot += b.ell;
+  }
+  Scoped mem;
+  TsvdBufs bf{};
+  HM_OK(dalloc(&bf.Y, ytot)); mem.keep(bf.Y);
+  HM_OK(dalloc(&bf.Q, ytot)); mem.keep(bf.Q);
+  HM_OK(dalloc(&bf.Bt, btot)); mem.keep(bf.Bt);
+  HM_OK(dalloc(&bf.Q2, btot)); mem.keep(bf.Q2);
+  HM_OK(dalloc(&bf.M, mtot)); mem.keep(bf.M);
+  HM_OK(dalloc(&bf.Vj, mtot)); mem.keep(bf.Vj);
+  HM_OK(dalloc(&bf.G, mtot)); mem.keep(bf.G);
+  HM_OK(dalloc(&bf.Z, mtot)); mem.keep(bf.Z);
+  HM_OK(dalloc(&bf.Sig, stot)); mem.keep(bf.Sig);
+  bf.onorm = nullptr;
+  bf.Om = nullptr;
+  std::vector<CopyItem> in;
+  for (int i = 0; i < nn; i++) {
+    in.push_back({d_u[i], bf.Y + blocks[i].yoff, (long long)m[i] * sr[i]});
+    in.push_back({d_v[i], bf.Bt + blocks[i].boff, (long long)n[i] * sr[i]});
+  }
+  CopyItem *d_in;
+  HM_OK(to_device(&d_in, in, s)); mem.keep(d_in);
+  hipLaunchKernelGGL(k_copy_list, dim3((unsigned)in.size(), 8), dim3(256), 0, s, d_in);
+  HM_LAUNCH_CHECK();
+  std::vector<int> list(nn);
+  for (int i = 0; i < nn; i++) list[i] = i;
+  TsvdBlock *d_blocks;
+  int *d_list;
+  HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
+  HM_OK(to_device(&d_list, list, s)); mem.keep(d_list);
+  Timed t0 = timing_begin("recompress_core", s);
+  hipLaunchKernelGGL(k_qr, dim3(nn), dim3(QR_NT), 0, s, d_blocks, d_list, 0, bf, 0.0);
+  hipLaunchKernelGGL(k_qr, dim3(nn), dim3(QR_NT), 0, s, d_blocks, d_list, 1, bf, 0.0);
+  hipLaunchKernelGGL(k_rmul, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf);
+  hipLaunchKernelGGL(k_jacobi, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf);
+  timing_end(t0, s);
+  HM_LAUNCH_CHECK();
+  HM_HIP(hipMemcpyAsync(blocks.data(), d_blocks, nn * sizeof(TsvdBlock),
+                        hipMemcpyDeviceToHost, s));
+  HM_HIP(hipStreamSynchronize(s));
+  long long ut = 0, vt = 0;
+  for (auto &b : blocks) {
+    ut += (long long)b.m * b.k;
+    vt += (long long)b.n * b.k;
+  }
+  double *uo, *vo;
+  HM_OK(dalloc(&uo, ut)); mem.keep(uo);
+  HM_OK(dalloc(&vo, vt)); mem.keep(vo);
+  ut = vt = 0;
+  std::vector<int4> wf;
+  std::vector<CopyItem> outc;
+  for (int i = 0; i < nn; i++) {
+    TsvdBlock &b = blocks[i];
+    b.uout = uo + ut;
+    b.vout = vo + vt;
+    outc.push_back({b.uout, d_u[i], (long long)b.m * b.k});
+    outc.push_back({b.vout, d_v[i], (long long)b.n * b.k});
+    ut += (long long)b.m * b.k;
+    vt += (long long)b.n * b.k;
+    for (int side = 0; side < 2; side++) {
+      const int rows = side ? b.n : b.m;
+      for (int c = 0; c < (rows + 255) / 256; c++) wf.push_back(make_int4(i, side, c, 0));
+    }
+    k_out[i] = b.k;
+  }
+  HM_HIP(hipMemcpyAsync(d_blocks, blocks.data(), nn * sizeof(TsvdBlock),
+                        hipMemcpyHostToDevice, s));
+  int4 *d_wf;
+  CopyItem *d_out;
+  HM_OK(to_device(&d_wf, wf, s)); mem.keep(d_wf);
+  HM_OK(to_device(&d_out, outc, s)); mem.keep(d_out);
+  hipLaunchKernelGGL(k_form, dim3((unsigned)wf.size()), dim3(256), 0, s, d_blocks, d_wf, bf);
+  HM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_copy_list, dim3((unsigned)outc.size(), 8), dim3(256), 0, s, d_out);
+  HM_LAUNCH_CHECK();
+  HM_HIP(hipStreamSynchronize(s));
+  return SUCCESS;
+}
+
+// Dense-method recompression: D = U V^T, exact SVD, absolute rule.
+static int recompress_dense(int nn, const int *m, const int *n, const int *sr,
+                            double *const *d_u, double *const *d_v, double thr,
+                            int *k_out, hipStream_t s) {
+  Scoped mem;
+  std::vector<OuterJob> jobs(nn);
+  long long dtot = 0;
+  std::vector<long long> doff(nn);
+  for (int i = 0; i < nn; i++) {
+    doff[i] = dtot;
+    dtot += (long long)m[i] * n[i];
+  }
+  double *D;
+  HM_OK(dalloc(&D, dtot)); mem.keep(D);
+  std::vector<int4> work;
+  for (int i = 0; i < nn; i++) {
+    jobs[i] = {d_u[i], d_v[i], D + doff[i], m[i], n[i], sr[i], 0};
+    for (int a = 0; a < (m[i] + DenseTile::TM - 1) / DenseTile::TM; a++)
+      for (int c = 0; c < (n[i] + DenseTile::TN - 1) / DenseTile::TN; c++)
+        work.push_back(make_int4(i, a, c, 0));
+  }
+  OuterJob *d_jobs;
+  int4 *d_work;
+  HM_OK(to_device(&d_jobs, jobs, s)); mem.keep(d_jobs);
+  HM_OK(to_device(&d_work, work, s)); mem.keep(d_work);
+  if (!work.empty()) {
+    hipLaunchKernelGGL(k_outer, dim3((unsigned)work.size()), dim3(256), 0, s, d_jobs, d_work);
+    HM_LAUNCH_CHECK();
+  }
+  std::vector<TsvdBlock> blocks(nn);
+  for (int i = 0; i < nn; i++) {
+    TsvdBlock &b = blocks[i];
+    memset(&b, 0, sizeof(b));
+    b.A = D + doff[i];
+    b.lda = m[i];
+    b.m = m[i];
+    b.n = n[i];
+    b.ell = std::min(m[i], n[i]);
+    b.rule = RULE_ABSOLUTE;
+    b.thr = thr;
+    b.uid = 0x5eed0000ull + (unsigned long long)i;
+  }
+  TsvdOptions opt;
+  double *up = nullptr, *vp = nullptr;
+  HM_OK(tsvd_run(blocks, opt, &up, &vp, nullptr, nullptr, 0, s));
+  mem.keep(up);
+  mem.keep(vp);
+  std::vector<CopyItem> outc;
+  for (int i = 0; i < nn; i++) {
+    outc.push_back({blocks[i].uout, d_u[i], (long long)m[i] * blocks[i].k});
+    outc.push_back({blocks[i].vout, d_v[i], (long long)n[i] * blocks[i].k});
+    k_out[i] = blocks[i].k;
+  }
+  CopyItem *d_out;
+  HM_OK(to_device(&d_out, outc, s)); mem.keep(d_out);
+  hipLaunchKernelGGL(k_copy_list, dim3((unsigned)outc.size(), 8), dim3(256), 0, s, d_out);
+  HM_LAUNCH_CHECK();
+  HM_HIP(hipStreamSynchronize(s));
+  return SUCCESS;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
+                                       const int *s, double *const *d_u,
+                                       double *const *d_v, double thr,
+                                       int method, int *k_out, void *stream) {
+  HM_OK(ensure_init());
+  if (nnodes <= 0) return SUCCESS;
+  if (!m || !n || !s || !d_u || !d_v || !k_out) {
+    set_error("hm_hip_recompress_batch: NULL argument");
+    return INPUT_ERROR;
+  }
+  std::vector<int> qr_idx, dn_idx;
+  for (int i = 0; i < nnodes; i++) {
+    if (s[i] <= 0 || m[i] <= 0 || n[i] <= 0 || s[i] > MAX_ELL) {
+      set_error("hm_hip_recompress_batch: node %d has m=%d n=%d s=%d", i, m[i], n[i], s[i]);
+      return INPUT_ERROR;
+    }
+    const int mn = std::min(m[i], n[i]);
+    int meth = method;
+    if (meth < 0) meth = (s[i] < mn) ? 0 : 1;
+    if (meth == 0 && s[i] >= mn) {
+      set_error("hm_hip_recompress_batch: QR method needs s < min(m, n) (node %d)", i);
+      return INPUT_ERROR;
+    }
+    (meth == 0 ? qr_idx : dn_idx).push_back(i);
+  }
+  hipStream_t st = stream_of(stream);
+  for (int pass = 0; pass < 2; pass++) {
+    const std::vector<int> &idx = pass ? dn_idx : qr_idx;
+    if (idx.empty()) continue;
+    std::vector<int> mm, nn_, ss, kk(idx.size());
+    std::vector<double *> uu, vv;
+    for (int i : idx) {
+      mm.push_back(m[i]);
+      nn_.push_back(n[i]);
+      ss.push_back(s[i]);
+      uu.push_back(d_u[i]);
+      vv.push_back(d_v[i]);
+    }
+    int rc = pass ? recompress_dense((int)idx.size(), mm.data(), nn_.data(), ss.data(),
+                                     uu.data(), vv.data(), thr, kk.data(), st)
+                  : recompress_qr((int)idx.size(), mm.data(), nn_.data(), ss.data(),
+                                  uu.data(), vv.data(), thr, kk.data(), st);
+    if (rc != SUCCESS) return rc;
+    for (size_t t = 0; t < idx.size(); t++) k_out[idx[t]] = kk[t];
+  }
+  return SUCCESS;
+}

@@ -1,0 +1,169 @@
+/*
+ * hm_hip: the thin C-ABI HIP layer under libhmat_lib (gfx950 / MI355X).
+ *
+ * Every call takes plain pointers and sizes; device pointers are marked d_.
+ * Streams are hipStream_t passed as void* (NULL = the library's own
+ * stream).  Every int-returning call returns an enum ErrorCode value
+ * (0 = SUCCESS); hm_hip_last_error() gives the message of the last
+ * failure on the calling thread.  There is no host fallback: without a
+ * usable GPU every compute call fails with ALLOCATION_FAILURE and a message.
+ *
+ * In the reference all of this arithmetic is BLAS/LAPACK
+ * (include/internal/blas_wrapper.h:1-23, include/internal/lapack_wrapper.h:
+ * 1-14); these entry points are what replaces that boundary.
+ *
+ * A device tree (hm_dtree) stores a HODLR matrix level-major and packed:
+ *   - all 2^h diagonal leaves back to back, each m_i x m_i column-major;
+ *   - per level l = 1..h, blocks j = 0..2^l-1 (j even: top-right child [1]
+ *     of parent j/2, j odd: bottom-left child [2]); all U panels of the
+ *     tree in one array and all V panels in another, each m x s / n x s
+ *     column-major at a per-block offset;
+ *   - int metadata (leaf offsets, block dims, ranks, offsets) on device.
+ * One tree must not be used by two concurrent calls (its product scratch
+ * is owned by the tree).
+ */
+#ifndef HM_HIP_H
+#define HM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef HM_HIP_API
+#define HM_HIP_API __attribute__((visibility("default")))
+#endif
+
+struct TreeHODLR;
+struct HODLRInternalNode;
+typedef struct hm_dtree hm_dtree;
+
+/* ---- runtime ---------------------------------------------------------- */
+HM_HIP_API int hm_hip_init(int device);
+HM_HIP_API int hm_hip_device_count(int *count);
+HM_HIP_API const char *hm_hip_last_error(void);
+HM_HIP_API int hm_hip_synchronize(void *stream);
+
+/* Raw device memory helpers (for callers that bring no allocator). */
+HM_HIP_API int hm_hip_malloc(void **d_ptr, size_t bytes);
+HM_HIP_API int hm_hip_free(void *d_ptr);
+HM_HIP_API int hm_hip_memcpy_h2d(void *d_dst, const void *src, size_t bytes, void *stream);
+HM_HIP_API int hm_hip_memcpy_d2h(void *dst, const void *d_src, size_t bytes, void *stream);
+HM_HIP_API int hm_hip_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
+/* 2-D strided copies of an m x n column-major block (leading dims in
+ * elements of 8 bytes). */
+HM_HIP_API int hm_hip_copy2d_h2d(double *d_dst, long ld_dst, const double *src, long ld_src,
+                      int m, int n, void *stream);
+HM_HIP_API int hm_hip_copy2d_d2h(double *dst, long ld_dst, const double *d_src, long ld_src,
+                      int m, int n, void *stream);
+
+/* ---- device trees ----------------------------------------------------- */
+/* Build a device copy of a host (sub)tree.  `node` with `height` layers of
+ * internal nodes below and including it; for a whole tree pass
+ * tree->root and tree->height. */
+HM_HIP_API int hm_hip_tree_upload(const struct HODLRInternalNode *node, int height,
+                       hm_dtree **out, void *stream);
+/* Copy a device tree into a host tree of the same height: sets every m, n,
+ * s, and allocates (with `alloc`, or malloc when NULL) and fills every
+ * diagonal `data`, `u` and `v` array. */
+HM_HIP_API int hm_hip_tree_download(const hm_dtree *tree, struct TreeHODLR *host,
+                         void *(*alloc)(size_t), void *stream);
+HM_HIP_API int hm_hip_tree_destroy(hm_dtree *tree);
+
+typedef struct hm_tree_info {
+  int height;        /* layers of internal nodes */
+  int n;             /* matrix order */
+  int nleaf;         /* 2^height */
+  int nblock;        /* off-diagonal blocks, 2^(height+1) - 2 */
+  int max_rank;
+  long long sum_rank_dims; /* sum over blocks of (m + n) * s */
+  long long diag_elems;    /* sum over leaves of m_i^2 */
+} hm_tree_info;
+HM_HIP_API int hm_hip_tree_info(const hm_dtree *tree, hm_tree_info *info);
+/* ranks[g] for block g in level-major order (level 1 first). */
+HM_HIP_API int hm_hip_tree_ranks(const hm_dtree *tree, int *ranks);
+
+/* ---- products ----------------------------------------------------------
+ * y = H x (FP64).  Replaces the dgemv loops of src/vector_algebra.c:52-197.
+ */
+HM_HIP_API int hm_hip_matvec(hm_dtree *tree, const double *d_x, double *d_y, void *stream);
+
+/* Y = op(H) X with op = identity (trans = 0) or transpose (trans = 1).
+ * X is N x R with X(i, j) at d_X[i * x_rs + j * x_cs]; Y likewise.
+ * X * H is Y^T = H^T X^T (trans = 1 with row/column strides swapped).
+ * Replaces the dgemm loops of src/dense_algebra.c:82-1151. */
+HM_HIP_API int hm_hip_matmat(hm_dtree *tree, int trans, int R, const double *d_X,
+                  long x_rs, long x_cs, double *d_Y, long y_rs, long y_cs,
+                  void *stream);
+
+/* ---- compression -------------------------------------------------------
+ * Dense -> HODLR on the device.  d_A is m x m column-major with leading
+ * dimension lda and is not modified.  `ms` (host, 2^height ints) gives
+ * custom leaf sizes or is NULL for halving (top-left gets ceil).
+ * Rank rule of the reference (src/constructors.c:329-336): keep sigma_i
+ * while sigma_i >= thr * sigma_0; rank 1 when sigma_0 <= thr.
+ * Replaces src/constructors.c:212-546 and src/lapack_wrapper.c:8-59. */
+typedef struct hm_compress_opts {
+  int sketch_init;      /* initial sketch width (columns), default 16 */
+  int sketch_max;       /* largest sketch width before giving up, default 512 */
+  int n_test;           /* posterior test vectors per block, default 6 */
+  unsigned long long seed;
+  const int *rank_hint; /* per block (level-major) predicted rank, or NULL */
+  int oversample;       /* added to rank_hint, default 8 */
+  double accept;        /* residual acceptance factor vs thr*sigma0, default 0.1 */
+} hm_compress_opts;
+
+typedef struct hm_compress_stats {
+  int sketch_rounds;    /* total sketch passes summed over levels */
+  int blocks_retried;   /* block-rounds that failed the residual test */
+  int max_sketch;       /* widest sketch used */
+  int fallback_blocks;  /* blocks that hit sketch_max (then exact path) */
+  double ms_total;
+} hm_compress_stats;
+
+HM_HIP_API void hm_hip_compress_opts_default(hm_compress_opts *opts);
+HM_HIP_API int hm_hip_compress_dense(const double *d_A, long lda, int m, int height,
+                          const int *ms, double thr,
+                          const hm_compress_opts *opts, hm_dtree **out,
+                          hm_compress_stats *stats, void *stream);
+
+/* ---- synthetic kernel matrices (device generator) -----------------------
+ * A(i, j) = k(|p_i - p_j|) + shift * delta_ij for points d_pts (n x dim,
+ * row-major).  kind 0: exp(-r / ell); kind 1: 1 / (r + ell);
+ * kind 2: Matern-3/2 (1 + sqrt(3) r / ell) exp(-sqrt(3) r / ell). */
+HM_HIP_API int hm_hip_fill_kernel_matrix(int kind, int n, int dim, const double *d_pts,
+                              double ell, double shift, double *d_A, long lda,
+                              void *stream);
+
+/* ---- recompression ------------------------------------------------------
+ * Batched recompression of low-rank nodes, all on the device.  Node i has
+ * U_i (m_i x s_i, ld m_i) at d_u[i] and V_i (n_i x s_i, ld n_i) at d_v[i]
+ * (arrays of device pointers, host-resident).  On return the first k_i
+ * columns of U_i / V_i hold the recompressed factors (U carries sigma),
+ * k_out[i] = k_i (host).  Absolute rule: keep sigma_j > thr, at least 1.
+ * method 0: QR method (needs s_i < min(m_i, n_i)), 1: dense U V^T SVD,
+ * -1: choose per node as the reference's callers do.
+ * Replaces src/hodlr_algebra.c:97-343. */
+HM_HIP_API int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
+                            const int *s, double *const *d_u,
+                            double *const *d_v, double thr, int method,
+                            int *k_out, void *stream);
+
+/* ---- diagnostics -------------------------------------------------------
+ * One v_mfma_f64_16x16x4_f64 with A(i,k) = 1 + i + 16 k and
+ * B(k,j) = 1 + 2 j - k, read back through the lane layout the kernels
+ * assume; out (host, 256 doubles) receives D row-major. */
+HM_HIP_API int hm_hip_selftest_mfma(double *out);
+
+/* ---- kernel timing (HIP events around tracked kernels) ----------------- */
+HM_HIP_API int hm_hip_kernel_timing(int enable);
+HM_HIP_API int hm_hip_kernel_timing_read(const char *name, double *total_ms, long *count);
+HM_HIP_API int hm_hip_kernel_timing_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,41 @@
+/*
+ * Recompression of a low-rank off-diagonal node.
+ *
+ * The reference keeps its two recompression routines static
+ * (src/hodlr_algebra.c:97-211 `recompress`, QR based, used when
+ * s < min(m, n); src/hodlr_algebra.c:280-343 `recompress_large_s`, dense
+ * SVD of U*V^T, used otherwise) and reaches them only through
+ * multiply_hodlr_hodlr (src/hodlr_algebra.c:835-845, 1116-1126).  This
+ * header exports them directly so that callers, and the H x H product when
+ * it is added, can recompress on the GPU.  Both replace u, v and s of the
+ * node in place (u and v are freed and re-malloc'd) and use the
+ * reference's absolute truncation rule: keep sigma_i while
+ * sigma_i > svd_threshold, at least one column.
+ *
+ * Return value: 0, or a negative LAPACK-style argument index on bad input.
+ */
+#ifndef HMAT_LIB_HODLR_ALGEBRA_H
+#define HMAT_LIB_HODLR_ALGEBRA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct NodeOffDiagonal;
+
+/* QR method: U = Q_U R_U, V = Q_V R_V, SVD of R_U R_V^T (s x s).
+ * Requires node->s < min(node->m, node->n).
+ * Replaces src/hodlr_algebra.c:97-211. */
+int recompress(struct NodeOffDiagonal *node, const int m_smaller,
+               const double svd_threshold, int *ierr);
+
+/* Dense method: SVD of the m x n product U V^T.
+ * Replaces src/hodlr_algebra.c:280-343. */
+int recompress_large_s(struct NodeOffDiagonal *node, const int m_smaller,
+                       const double svd_threshold, int *ierr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

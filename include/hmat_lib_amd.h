@@ -1,0 +1,45 @@
+/*
+ * Extensions of libhmat_lib (MI355X build) beyond the reference API.
+ *
+ * The reference has no device: every call of its API hands over host
+ * arrays.  This build keeps that API unchanged (the hmat_lib headers) and
+ * keeps, per tree, a device copy in a side table keyed by the node pointer
+ * and height.  An entry is reused while the tree's structure fingerprint
+ * (node sizes, ranks and the u/v/data pointer values) is unchanged and is
+ * dropped by free_tree_data / free_tree_hodlr.  Callers that overwrite
+ * u/v/data contents in place must call hmat_amd_invalidate.
+ *
+ * No counterpart in the reference; these are for callers who want the
+ * device-resident path (inputs already in HBM).
+ */
+#ifndef HMAT_LIB_AMD_H
+#define HMAT_LIB_AMD_H
+
+#include "hm_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct TreeHODLR;
+
+/* Drop the cached device copy of a tree (after in-place edits). */
+void hmat_amd_invalidate(const struct TreeHODLR *tree);
+
+/* The device copy of a host tree (uploaded on first use, then cached and
+ * owned by the side table). */
+hm_dtree *hmat_amd_device_tree(const struct TreeHODLR *tree, int *ierr);
+
+/* Fill an empty host tree of the same height from a device tree (malloc'd
+ * u/v/data as dense_to_tree_hodlr would) and cache the device tree for it;
+ * ownership of `dtree` passes to the side table. */
+void hmat_amd_attach(struct TreeHODLR *tree, hm_dtree *dtree, int *ierr);
+
+/* Number of cached device trees (diagnostics). */
+int hmat_amd_cache_size(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

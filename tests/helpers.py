@@ -1,0 +1,131 @@
+"""Shared fixtures for the tests: the reference's test matrices, oracle
+wrappers and comparisons.  Matrices follow the reference's fixture
+definitions (tests/utils/common_data.c of hmat_lib)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from hmat_lib_amd import capi
+from hmat_lib_amd import tree as T
+from oracle import oracle as O
+
+
+# -------------------------------------------------- reference fixtures
+def laplacian(m):
+    """2 on the diagonal, -1 next to it (common_data.c:27-41)."""
+    a = 2.0 * np.eye(m)
+    i = np.arange(m - 1)
+    a[i, i + 1] = -1.0
+    a[i + 1, i] = -1.0
+    return np.asfortranarray(a)
+
+
+def laplacian_corners(m, symmetric=True):
+    """Laplacian with 0.5 in the bottom-left (and top-right) corner:
+    matrix[m-1] = 0.5 is row m-1, column 0 (column-major)."""
+    a = laplacian(m)
+    a[m - 1, 0] = 0.5
+    if symmetric:
+        a[0, m - 1] = 0.5
+    return a
+
+
+def glibc_decay_matrices(m, scale, sorted_first, count_sorted, count_unsorted):
+    """The decay matrices of test_constructors.c:1047-1105: srand(42), then
+    `count_sorted` sorted and `count_unsorted` unsorted draws of
+    x_i = 1 - 2 rand()/RAND_MAX, A_ij = exp(-scale |x_i - x_j|)."""
+    libc = O.libc()
+    libc.srand(42)
+    RAND_MAX = 2147483647
+    out = []
+    for kind, cnt in (("sorted", count_sorted), ("unsorted", count_unsorted)):
+        for _ in range(cnt):
+            x = np.array([1.0 - 2.0 * (libc.rand() / RAND_MAX) for _ in range(m)])
+            if kind == "sorted":
+                x = np.sort(x)
+            a = np.exp(-scale * np.abs(x[:, None] - x[None, :]))
+            out.append((kind, np.asfortranarray(a)))
+    return out
+
+
+# -------------------------------------------------- oracle helpers
+def oracle_compress(a, height, thr, ms=None):
+    """Oracle tree of a dense matrix (a is copied; the oracle overwrites)."""
+    lib = O.oracle()
+    ierr = C.c_int(0)
+    t = lib.orc_allocate_tree(height, C.byref(ierr))
+    assert ierr.value == 0
+    work = np.asfortranarray(a.copy())
+    ms_arr = (C.c_int * len(ms))(*ms) if ms is not None else None
+    lib.orc_dense_to_tree(t, a.shape[0], ms_arr, capi.dptr(work), thr, C.byref(ierr))
+    assert ierr.value == 0, ierr.value
+    return t
+
+
+def oracle_free(t):
+    O.oracle().orc_free_tree(t)
+
+
+def oracle_matmat(t, x, mode="hd"):
+    """mode hd: H X, htd: H^T X, dh: X H (oracle restatement)."""
+    lib = O.oracle()
+    x = np.asfortranarray(x, dtype=np.float64)
+    ierr = C.c_int(0)
+    n = t.contents.root.contents.m
+    if mode == "dh":
+        rows = x.shape[0]
+        out = np.zeros((rows, n), order="F")
+        lib.orc_multiply_dense_hodlr(t, capi.dptr(x), rows, rows, capi.dptr(out), rows,
+                                     C.byref(ierr))
+    else:
+        r = x.shape[1]
+        out = np.zeros((n, r), order="F")
+        f = lib.orc_multiply_hodlr_dense if mode == "hd" else lib.orc_multiply_hodlr_transpose_dense
+        f(t, capi.dptr(x), r, n, capi.dptr(out), n, C.byref(ierr))
+    assert ierr.value == 0
+    return out
+
+
+def oracle_matvec(t, x):
+    lib = O.oracle()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = t.contents.root.contents.m
+    out = np.zeros(n)
+    ierr = C.c_int(0)
+    lib.orc_multiply_vector(t, capi.dptr(x), capi.dptr(out), C.byref(ierr))
+    assert ierr.value == 0
+    return out
+
+
+def rel_err(a, b):
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def sign_normalize(u, v):
+    """Flip column pairs so that the largest-magnitude entry of each V
+    column is positive (U V^T is invariant)."""
+    u = u.copy()
+    v = v.copy()
+    for j in range(u.shape[1]):
+        k = np.argmax(np.abs(v[:, j]))
+        if v[k, j] < 0:
+            u[:, j] *= -1
+            v[:, j] *= -1
+    return u, v
+
+
+def random_flat_tree(rng, leaf_sizes, ranks_fn):
+    """A random FlatTree with given leaf sizes and a rank per (level, j)."""
+    h = int(np.log2(len(leaf_sizes)))
+    diag = [rng.standard_normal((m, m)) for m in leaf_sizes]
+    ft = T.FlatTree(h, list(leaf_sizes), diag)
+    for l in range(1, h + 1):
+        for j in range(1 << l):
+            (r0, r1), (c0, c1) = ft.block_ranges(l, j)
+            s = ranks_fn(l, j)
+            ft.blocks[(l, j)] = (rng.standard_normal((r1 - r0, s)),
+                                 rng.standard_normal((c1 - c0, s)))
+    return ft

@@ -1,0 +1,141 @@
+"""HODLR x vector / HODLR x dense on the GPU vs the oracle restatement and
+vs exact dense algebra, through both the device C-ABI and the drop-in API.
+
+Tolerance: 1e-12 relative (FP64 throughout; only the summation order
+differs from the reference's BLAS loops)."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from hmat_lib_amd import capi
+from hmat_lib_amd import tree as T
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_mfma_lane_layout(gpu):
+    _say("case: mfma selftest")
+    out = (C.c_double * 256)()
+    capi.check(capi.hip().hm_hip_selftest_mfma(out), "selftest")
+    got = np.array(out[:]).reshape(16, 16)
+    i = np.arange(16)
+    A = 1.0 + i[:, None] + 16.0 * np.arange(4)[None, :]
+    B = 1.0 + 2.0 * i[None, :] - np.arange(4)[:, None]
+    np.testing.assert_array_equal(got, A @ B)
+
+
+CASES = [
+    ("uniform h3 s2", [32] * 8, lambda l, j: 2),
+    ("uniform h4 mixed", [64] * 16, lambda l, j: 1 + (l + j) % 5),
+    ("ragged h3", [10, 11, 1, 9, 5, 20, 7, 6], lambda l, j: 1 + (j % 3)),
+    ("ragged zeros h2", [20, 1, 1, 0], lambda l, j: 1),
+    ("big leaves h2", [300, 129, 77, 256], lambda l, j: 17 + l),
+    ("rank 40 h5", [96] * 32, lambda l, j: 40 - 5 * l),
+]
+
+
+@pytest.mark.parametrize("name,leaves,ranks", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("R", [1, 3, 64, 100])
+def test_products_vs_dense(gpu, name, leaves, ranks, R):
+    from hmat_lib_amd import device as D
+    _say(f"case: products {name} R={R}")
+    rng = np.random.default_rng(len(leaves) * 131 + R)
+    ft = H.random_flat_tree(rng, leaves, ranks)
+    lib = capi.hmat()
+    keep = T.Keep()
+    t = T.fill_tree(lib, ft, keep)
+    try:
+        dense = ft.to_dense()
+        n = ft.n
+        x = np.asfortranarray(rng.standard_normal((n, R)))
+        dt = D.DeviceTree.upload(t.contents.root, ft.height)
+        dx = D.DeviceArray.from_numpy(x)
+        dy = D.DeviceArray((n, R))
+        if R == 1:
+            dt.matvec(dx, dy)
+        else:
+            dt.matmat(dx, dy)
+        assert H.rel_err(dy.numpy(), dense @ x) < TOL
+        dt.matmat(dx, dy, trans=1)
+        assert H.rel_err(dy.numpy(), dense.T @ x) < TOL
+        # X H through the strided transposed form
+        xr = np.asfortranarray(rng.standard_normal((R, n)))
+        dxr = D.DeviceArray.from_numpy(xr)
+        dyr = D.DeviceArray((R, n))
+        dt.matmat(dxr, dyr, trans=1, R=R, x_strides=(R, 1), y_strides=(R, 1))
+        assert H.rel_err(dyr.numpy(), xr @ dense) < TOL
+        dt.free()
+    finally:
+        T.detach_and_free(lib, t)
+
+
+def test_dropin_api_vs_oracle(gpu):
+    """multiply_* of libhmat_lib vs the oracle's restatement, same tree."""
+    _say("case: drop-in API vs oracle")
+    rng = np.random.default_rng(7)
+    ft = H.random_flat_tree(rng, [50, 40, 33, 60, 45, 41, 39, 52], lambda l, j: 1 + (l * 3 + j) % 4)
+    lib = capi.hmat()
+    keep = T.Keep()
+    t = T.fill_tree(lib, ft, keep)
+    try:
+        n = ft.n
+        ierr = C.c_int(-1)
+        x = rng.standard_normal(n)
+        y = np.zeros(n)
+        lib.multiply_vector(t, capi.dptr(x), capi.dptr(y), C.byref(ierr))
+        assert ierr.value == 0
+        assert H.rel_err(y, H.oracle_matvec(t, x)) < TOL
+        X = np.asfortranarray(rng.standard_normal((n, 16)))
+        for name, mode in (("multiply_hodlr_dense", "hd"),
+                           ("multiply_hodlr_transpose_dense", "htd")):
+            out = np.zeros((n, 16), order="F")
+            getattr(lib, name)(t, capi.dptr(X), 16, n, capi.dptr(out), n, C.byref(ierr))
+            assert ierr.value == 0
+            assert H.rel_err(out, H.oracle_matmat(t, X, mode)) < TOL
+        Xr = np.asfortranarray(rng.standard_normal((9, n)))
+        out = np.zeros((9, n), order="F")
+        lib.multiply_dense_hodlr(t, capi.dptr(Xr), 9, 9, capi.dptr(out), 9, C.byref(ierr))
+        assert ierr.value == 0
+        assert H.rel_err(out, H.oracle_matmat(t, Xr, "dh")) < TOL
+        # internal-node forms on the root and on a subtree
+        out = np.zeros((n, 16), order="F")
+        lib.multiply_internal_node_dense(t.contents.root, ft.height, capi.dptr(X), 16, n,
+                                         None, None, capi.dptr(out), n)
+        assert H.rel_err(out, ft.to_dense() @ X) < TOL
+        sub = t.contents.root.contents.children[3].internal
+        m3 = sub.contents.m
+        Xs = np.asfortranarray(rng.standard_normal((m3, 5)))
+        outs = np.zeros((m3, 5), order="F")
+        lib.multiply_internal_node_transpose_dense(sub, ft.height - 1, capi.dptr(Xs), 5, m3,
+                                                   None, None, capi.dptr(outs), m3)
+        off = ft.n - m3
+        dsub = ft.to_dense()[off:, off:]
+        assert H.rel_err(outs, dsub.T @ Xs) < TOL
+        outd = np.zeros((5, m3), order="F")
+        Xd = np.asfortranarray(rng.standard_normal((5, m3)))
+        lib.multiply_dense_internal_node(sub, ft.height - 1, capi.dptr(Xd), 5, 5,
+                                         None, None, capi.dptr(outd), 5)
+        assert H.rel_err(outd, Xd @ dsub) < TOL
+    finally:
+        T.detach_and_free(lib, t)
+
+
+def test_dropin_errors(gpu):
+    lib = capi.hmat()
+    ierr = C.c_int(0)
+    t = lib.allocate_tree_monolithic(2, C.byref(ierr))
+    x = np.zeros(4)
+    assert not lib.multiply_vector(None, capi.dptr(x), None, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    assert not lib.multiply_vector(t, capi.dptr(x), capi.dptr(x), C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    lib.free_tree_hodlr(C.byref(t))

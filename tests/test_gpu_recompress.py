@@ -1,0 +1,114 @@
+"""Recompression on the GPU (QR method and dense method) vs the reference's
+12-digit golden values (sign-normalized), its zero cases, and the oracle
+restatement on random factor pairs.  Absolute rule, thr 1e-8."""
+import ctypes as C
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from hmat_lib_amd import capi
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _sparse(shape, entries):
+    out = np.zeros(shape)
+    for k, v in entries.items():
+        i, j = (int(x) for x in k.split(","))
+        out[i, j] = v
+    return out
+
+
+def _node(u, v):
+    nd = capi.NodeOffDiagonal()
+    uu = np.asfortranarray(u.copy())
+    vv = np.asfortranarray(v.copy())
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.malloc.restype = ctypes.c_void_p
+    pu = libc.malloc(uu.size * 8)
+    pv = libc.malloc(vv.size * 8)
+    ctypes.memmove(pu, uu.ctypes.data, uu.size * 8)
+    ctypes.memmove(pv, vv.ctypes.data, vv.size * 8)
+    nd.u = ctypes.cast(pu, capi.PDouble)
+    nd.v = ctypes.cast(pv, capi.PDouble)
+    nd.m, nd.n, nd.s = u.shape[0], v.shape[0], u.shape[1]
+    return nd
+
+
+def _read(nd):
+    u = np.ctypeslib.as_array(nd.u, shape=(nd.m * nd.s,)).reshape(nd.m, nd.s, order="F").copy()
+    v = np.ctypeslib.as_array(nd.v, shape=(nd.n * nd.s,)).reshape(nd.n, nd.s, order="F").copy()
+    return u, v
+
+
+@pytest.mark.parametrize("method", ["qr", "large_s"])
+def test_golden(gpu, method):
+    data = json.load(open(os.path.join(GOLD, "recompress.json")))
+    lib = capi.hmat()
+    fn = lib.recompress if method == "qr" else lib.recompress_large_s
+    for i, case in enumerate(data[method]):
+        print(f"case: recompress {method} {i}", flush=True)
+        m, n, s = case["m"], case["n"], case["s"]
+        u = _sparse((m, s), case["u"])
+        v = _sparse((n, s), case["v"])
+        nd = _node(u, v)
+        ierr = C.c_int(0)
+        rc = fn(C.byref(nd), min(m, n), 1e-8, C.byref(ierr))
+        assert rc == 0 and ierr.value == 0, capi.last_error()
+        assert nd.s == case["s_exp"]
+        uu, vv = _read(nd)
+        ue = _sparse((m, case["s_exp"]), case["u_exp"])
+        ve = _sparse((n, case["s_exp"]), case["v_exp"])
+        np.testing.assert_allclose(uu @ vv.T, ue @ ve.T, atol=1e-10)
+        if case["s_exp"] == 3:
+            # distinct singular values: factors unique up to column signs
+            a, b = H.sign_normalize(uu, vv)
+            c, d = H.sign_normalize(ue, ve)
+            np.testing.assert_allclose(a, c, atol=1e-10)
+            np.testing.assert_allclose(b, d, atol=1e-10)
+        sys.stdout.flush()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_vs_oracle(gpu, seed):
+    """Random U V^T with a known spectrum: rank and factors vs the oracle."""
+    from hmat_lib_amd import device as D
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    us, vs, exp = [], [], []
+    for _ in range(6):
+        m, n = rng.integers(40, 300), rng.integers(40, 300)
+        s = int(rng.integers(4, 30))
+        r = int(rng.integers(1, s))
+        qa, _ = np.linalg.qr(rng.standard_normal((m, r)))
+        qb, _ = np.linalg.qr(rng.standard_normal((n, r)))
+        sig = np.logspace(0, -12, r)
+        core = qa * sig @ qb.T
+        # spread the rank-r product over s redundant columns
+        mix = rng.standard_normal((r, s))
+        u = qa * sig @ mix
+        v = qb @ np.linalg.pinv(mix).T
+        assert np.allclose(u @ v.T, core, atol=1e-12)
+        us.append(np.asfortranarray(u))
+        vs.append(np.asfortranarray(v))
+        nd = capi.NodeOffDiagonal()
+        exp.append((core, int((sig > 1e-8).sum())))
+    nu, nv = D.recompress_batch(us, vs, 1e-8, method=-1)
+    for i in range(len(us)):
+        core, k = exp[i]
+        assert nu[i].shape[1] == k, (i, nu[i].shape, k)
+        assert np.linalg.norm(nu[i] @ nv[i].T - core, 2) < 2e-8
+        # oracle on the same node
+        nd = _node(us[i], vs[i])
+        ierr = C.c_int(0)
+        O.oracle().orc_recompress(C.byref(nd), min(us[i].shape[0], vs[i].shape[0]), 1e-8,
+                                  C.byref(ierr))
+        assert nd.s == k
+        uo, vo = _read(nd)
+        np.testing.assert_allclose(nu[i] @ nv[i].T, uo @ vo.T, atol=1e-9)

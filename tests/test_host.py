@@ -1,0 +1,221 @@
+"""Host-side logic of the drop-in library (no GPU compute): symbol exports,
+tree allocators, size formulas, rank/workspace queries, input errors.
+CPU only."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from hmat_lib_amd import capi
+from hmat_lib_amd import tree as T
+from tests import helpers as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = set(re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", txt))
+    return {n for n in names if not n.startswith(("if", "defined", "sizeof", "__attribute__",
+                                                  "visibility", "void", "int", "double",
+                                                  "struct", "return"))}
+
+
+def _exports(path):
+    out = os.popen(f"nm -D --defined-only {path}").read()
+    return {l.split()[-1] for l in out.splitlines() if " T " in l}
+
+
+def test_every_declared_symbol_is_exported():
+    hip_syms = _exports(os.path.join(capi.LIBDIR, "libhmat_hip.so"))
+    lib_syms = _exports(os.path.join(capi.LIBDIR, "libhmat_lib.so"))
+    test_syms = _exports(os.path.join(capi.LIBDIR, "libhmat_lib_test.so"))
+    for h in glob.glob(os.path.join(ROOT, "include", "hmat_lib", "*.h")) + \
+            [os.path.join(ROOT, "include", "hmat_lib_amd.h")]:
+        for name in _declared(h):
+            if name in ("malloc", "free", "alloc", "permanent_allocator"):
+                continue
+            assert name in lib_syms, (h, name)
+            assert name in test_syms, (h, name)
+    for name in _declared(os.path.join(ROOT, "include", "hm_hip.h")):
+        if name.startswith("hm_hip_"):
+            assert name in hip_syms, name
+
+
+def test_libraries_load():
+    capi.hip()
+    capi.hmat()
+    capi.hmat(test_abi=True)
+
+
+def _consistent(t, height):
+    tc = t.contents
+    assert tc.height == height and tc.len_work_queue == 1 << (height - 1)
+    assert not tc.root.contents.parent
+    lev = T.levels(tc.root, height)
+    for d in range(height):
+        for p in lev[d]:
+            for c in (1, 2):
+                leaf = p.contents.children[c].leaf.contents
+                assert leaf.type == capi.OFFDIAGONAL
+                assert C.addressof(leaf.parent.contents) == C.addressof(p.contents)
+                od = leaf.data.off_diagonal
+                assert not od.u and not od.v and od.m == od.s == od.n == 0
+            if d + 1 < height:
+                for c in (0, 3):
+                    ch = p.contents.children[c].internal.contents
+                    assert C.addressof(ch.parent.contents) == C.addressof(p.contents)
+                    assert ch.m == 0
+    for q, p in enumerate(lev[-1]):
+        assert C.addressof(tc.work_queue[q].contents) == C.addressof(p.contents)
+        for k, c in enumerate((0, 3)):
+            leaf = p.contents.children[c].leaf
+            assert leaf.contents.type == capi.DIAGONAL
+            assert not leaf.contents.data.diagonal.data
+            assert C.addressof(tc.innermost_leaves[2 * q + k].contents) == \
+                C.addressof(leaf.contents)
+
+
+@pytest.mark.parametrize("height", range(1, 7))
+def test_allocators_structure(height):
+    lib = capi.hmat()
+    ierr = C.c_int(-1)
+    for fn in (lib.allocate_tree, lib.allocate_tree_monolithic):
+        t = fn(height, C.byref(ierr))
+        assert ierr.value == capi.SUCCESS
+        _consistent(t, height)
+        lib.free_tree_hodlr(C.byref(t))
+        assert not t
+
+
+def test_array_sizes():
+    """test_allocators.c:82-99."""
+    lib = capi.hmat()
+    exp = [(1, 1, 2, 4), (2, 3, 4, 10), (4, 7, 8, 22), (8, 15, 16, 46), (16, 31, 32, 94),
+           (32, 63, 64, 190)]
+    for h, (wq, ni, il, nl) in enumerate(exp, start=1):
+        s = [C.c_size_t() for _ in range(4)]
+        lib.compute_construct_tree_array_sizes(h, *[C.byref(x) for x in s])
+        si, sl, sw, sil = (x.value for x in s)
+        assert sw == wq * 8 and sil == il * 8
+        assert si == ni * C.sizeof(capi.HODLRInternalNode)
+        assert sl == nl * C.sizeof(capi.HODLRLeafNode)
+
+
+def test_struct_layout_matches_reference_abi():
+    # sizes on LP64 of the reference structs (pointer 8, int 4, padding)
+    assert C.sizeof(capi.NodeOffDiagonal) == 32
+    assert C.sizeof(capi.HODLRLeafNode) == 48
+    assert C.sizeof(capi.HODLRInternalNode) == 48
+    assert C.sizeof(capi.TreeHODLR) == 56
+
+
+def test_construct_tree_from_arrays():
+    lib = capi.hmat()
+    h = 3
+    s = [C.c_size_t() for _ in range(4)]
+    lib.compute_construct_tree_array_sizes(h, *[C.byref(x) for x in s])
+    tree = capi.TreeHODLR()
+    internal = (C.c_char * s[0].value)()
+    leaves = (C.c_char * s[1].value)()
+    wq = (C.c_char * s[2].value)()
+    il = (C.c_char * s[3].value)()
+    ierr = C.c_int(-1)
+    lib.construct_tree(h, C.byref(tree), C.cast(internal, capi.PNode),
+                       C.cast(leaves, C.POINTER(capi.HODLRLeafNode)),
+                       C.cast(wq, C.POINTER(capi.PNode)),
+                       C.cast(il, C.POINTER(C.POINTER(capi.HODLRLeafNode))), C.byref(ierr))
+    assert ierr.value == capi.SUCCESS
+    _consistent(C.pointer(tree), h)
+    assert C.addressof(tree.root.contents) == C.addressof(internal)
+
+
+def test_input_errors():
+    lib = capi.hmat()
+    ierr = C.c_int(-1)
+    assert not lib.allocate_tree(0, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    assert not lib.allocate_tree_monolithic(0, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    t = lib.allocate_tree_monolithic(2, C.byref(ierr))
+    a = np.zeros((8, 8), order="F")
+    lib.dense_to_tree_hodlr(None, 8, None, capi.dptr(a), 1e-8, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    lib.dense_to_tree_hodlr(t, 8, None, None, 1e-8, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    ms = (C.c_int * 4)(2, 2, 2, 1)  # sums to 7, not 8
+    lib.dense_to_tree_hodlr(t, 8, ms, capi.dptr(a), 1e-8, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    x = np.zeros(8)
+    assert not lib.multiply_vector(t, capi.dptr(x), capi.dptr(x), C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    assert not lib.multiply_hodlr_dense(t, None, 1, 8, None, 8, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    lib.free_tree_hodlr(C.byref(t))
+
+
+def test_highest_rank_and_workspace():
+    """test_hodlr_dense_algebra.c:51-113: workspace = max rank x matrix_a."""
+    lib = capi.hmat()
+    rng = np.random.default_rng(0)
+    for h in range(1, 5):
+        ft = H.random_flat_tree(rng, [8] * (1 << h), lambda l, j: h)
+        keep = T.Keep()
+        t = T.fill_tree(lib, ft, keep)
+        assert lib.get_highest_s(t) == h
+        assert lib.compute_multiply_hodlr_dense_workspace(t, 2048) == h * 2048
+        T.detach_and_free(lib, t)
+        ft = H.random_flat_tree(rng, [8] * (1 << h),
+                                lambda l, j, h=h: (4 + h) if (l == h and j == 1) else 1)
+        t = T.fill_tree(lib, ft, keep)
+        assert lib.compute_multiply_hodlr_dense_workspace(t, 42) == (4 + h) * 42
+        T.detach_and_free(lib, t)
+
+
+def test_test_abi_uses_injected_allocator():
+    lib = capi.hmat(test_abi=True)
+    libc = C.CDLL("libc.so.6")
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    libc.free.argtypes = [C.c_void_p]
+    counts = {"m": 0, "f": 0}
+    MAL = C.CFUNCTYPE(C.c_void_p, C.c_size_t)
+    FRE = C.CFUNCTYPE(None, C.c_void_p)
+
+    @MAL
+    def mal(n):
+        counts["m"] += 1
+        return libc.malloc(n)
+
+    @FRE
+    def fre(p):
+        counts["f"] += 1
+        libc.free(p)
+
+    ierr = C.c_int(-1)
+    t = lib.allocate_tree_monolithic(3, C.byref(ierr), mal, fre)
+    assert ierr.value == 0 and counts["m"] == 5
+    _consistent(t, 3)
+    lib.free_tree_hodlr(C.byref(t), fre)
+    assert counts["f"] == 5
+
+
+@pytest.mark.skipif(os.environ.get("HM_FORCE_GPU_ABSENT_TEST") is None and
+                    __import__("hmat_lib_amd.device", fromlist=["x"]).device_count() > 0,
+                    reason="a GPU is present")
+def test_no_device_fails_loudly(capfd):
+    """Without a HIP device the compute path reports ALLOCATION_FAILURE with
+    a message; there is no host fallback."""
+    lib = capi.hmat()
+    ierr = C.c_int(-1)
+    t = lib.allocate_tree_monolithic(1, C.byref(ierr))
+    a = np.asfortranarray(np.eye(4))
+    lib.dense_to_tree_hodlr(t, 4, None, capi.dptr(a), 1e-8, C.byref(ierr))
+    assert ierr.value == capi.ALLOCATION_FAILURE
+    err = capfd.readouterr().err
+    assert "hmat_lib (MI355X)" in err and "failed" in err
+    lib.free_tree_hodlr(C.byref(t))
