@@ -171,6 +171,7 @@ def _declare_hip_api(lib):
                                      c_double, c_int, PInt, V], c_int),
         "hm_hip_selftest_mfma": ([PDouble], c_int),
         "hm_hip_kernel_timing": ([c_int], c_int),
+        "hm_hip_kernel_timing_select": ([C.c_char_p], c_int),
         "hm_hip_kernel_timing_read": ([C.c_char_p, PDouble, POINTER(c_long)], c_int),
         "hm_hip_kernel_timing_reset": ([], c_int),
     }

@@ -384,7 +384,7 @@ __global__ void k_selftest_mfma(double *out) {
   d4 c = {0.0, 0.0, 0.0, 0.0};
   c = mfma_f64(a, b, c);
   for (int v = 0; v < 4; v++) {
-    const int i = 4 * (lane >> 4) + v, j = lane & 15;
+    const int i = (lane >> 4) + 4 * v, j = lane & 15;
     out[i * 16 + j] = c[v];
   }
 }

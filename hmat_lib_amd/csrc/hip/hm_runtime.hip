@@ -63,6 +63,7 @@ struct TimingAcc {
   long count = 0;
 };
 static bool g_timing = false;
+static std::string g_timing_only;  // empty: time every named kernel
 static std::vector<TimingSlot> g_slots;
 static std::map<std::string, TimingAcc> g_acc;
 
@@ -84,6 +85,7 @@ Timed timing_begin(const char *name, hipStream_t s) {
   Timed t;
   if (!g_timing) return t;
   std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_timing_only.empty() && g_timing_only != name) return t;
   int slot = -1;
   for (size_t i = 0; i < g_slots.size(); i++)
     if (!g_slots[i].pending) { slot = (int)i; break; }
@@ -223,6 +225,12 @@ int hm_hip_copy2d_d2h(double *dst, long ld_dst, const double *d_src,
 int hm_hip_kernel_timing(int enable) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_timing = enable != 0;
+  return SUCCESS;
+}
+
+int hm_hip_kernel_timing_select(const char *name) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_timing_only = name ? name : "";
   return SUCCESS;
 }
 

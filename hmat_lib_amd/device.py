@@ -182,7 +182,8 @@ def recompress_batch(us, vs, thr, method=-1):
     return outu, outv
 
 
-def kernel_timing(enable: bool):
+def kernel_timing(enable: bool, only: str | None = None):
+    capi.hip().hm_hip_kernel_timing_select(only.encode() if only else None)
     capi.hip().hm_hip_kernel_timing(1 if enable else 0)
 
 

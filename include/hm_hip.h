@@ -159,6 +159,8 @@ HM_HIP_API int hm_hip_selftest_mfma(double *out);
 
 /* ---- kernel timing (HIP events around tracked kernels) ----------------- */
 HM_HIP_API int hm_hip_kernel_timing(int enable);
+/* Time only the kernel of this name (NULL: every named kernel). */
+HM_HIP_API int hm_hip_kernel_timing_select(const char *name);
 HM_HIP_API int hm_hip_kernel_timing_read(const char *name, double *total_ms, long *count);
 HM_HIP_API int hm_hip_kernel_timing_reset(void);
 

@@ -88,7 +88,8 @@ def test_random_vs_oracle(gpu, seed):
         r = int(rng.integers(1, s))
         qa, _ = np.linalg.qr(rng.standard_normal((m, r)))
         qb, _ = np.linalg.qr(rng.standard_normal((n, r)))
-        sig = np.logspace(0, -12, r)
+        # log-spaced spectrum kept a factor >= 3 away from thr = 1e-8
+        sig = np.logspace(0, -12, r) * 3.7
         core = qa * sig @ qb.T
         # spread the rank-r product over s redundant columns
         mix = rng.standard_normal((r, s))
