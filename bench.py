@@ -156,6 +156,7 @@ def main():
     roof["algorithmic_bytes"] = bytes_alg
     roof["algorithmic_flops"] = flops_alg
     roof["hbm_frac"] = bytes_alg / k_avg / PEAK_HBM
+    roof["tile_variant"] = int(capi.hip().hm_hip_matmat_variant(dt.h, 0, R))
 
     # CPU baseline: reference multiply_hodlr_dense on the same tree
     cpu = None

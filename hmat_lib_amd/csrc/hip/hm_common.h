@@ -63,20 +63,39 @@ struct BlockMeta {
   long long uoff, voff;
 };
 
-// Product scratch layout.  For leaf c and level l (1..h) let
-// q = c >> (h - l).  In Y = H X the leaf's rows are multiplied into block
+// Product scratch layout.  Leaves are cut into row chunks of at most
+// CHUNK_ROWS rows.  For chunk k of leaf c and level l (1..h) let
+// q = c >> (h - l).  In Y = H X the chunk's rows are multiplied into block
 // (l, q^1) [input side] and receive block (l, q) [output side]; in
-// Y = H^T X it is (l, q) and (l, q^1).  P_c holds, level after level, the
-// input-side partial products F_in^T X_c of leaf c (S_in(c) x R).
+// Y = H^T X it is (l, q) and (l, q^1).  P(l, k) = F_in rows(k)^T X_k is an
+// s x R column-major block; blocks are stored level by level and chunk by
+// chunk, so the partials of one off-diagonal block are contiguous.
+static constexpr int CHUNK_ROWS = 64;
+static constexpr int REDUCE_SEG = 16;
+
 struct ProductPlan {
-  std::vector<long long> pofs[2];  // per leaf prefix of S_in (rows), x R
-  std::vector<int> proff[2];       // [c * h + (l-1)] row offset inside P_c
-  std::vector<long long> wofs;     // per block prefix of s, x R
+  int nchunk = 0;
+  std::vector<int> ch_leaf, ch_r0, ch_rows;  // per chunk
+  std::vector<int> leaf_chunk0;              // nleaf + 1
+  std::vector<long long> pofs[2];            // [(l-1)*nchunk + k], x R
+  std::vector<long long> wofs;               // per block prefix of s, x R
   int s_in_max[2] = {0, 0};
   int s_out_max[2] = {0, 0};
+  // two-stage reduction: block g's chunks are cut into segments of
+  // REDUCE_SEG chunks; seg_list holds (block, segment) pairs of blocks
+  // with more than one segment, multi_list those blocks, part_ofs[g] the
+  // offset (x s_g x R) of block g's segment sums in the partial buffer.
+  std::vector<int2> seg_list;
+  std::vector<int> multi_list;
+  std::vector<long long> part_ofs;
+  int2 *d_seg_list = nullptr;
+  int *d_multi_list = nullptr;
+  long long *d_part_ofs = nullptr;
+  int4 *d_ch = nullptr;                      // (leaf, r0, rows, 0)
+  int *d_leaf_chunk0 = nullptr;
   long long *d_pofs[2] = {nullptr, nullptr};
-  int *d_proff[2] = {nullptr, nullptr};
   long long *d_wofs = nullptr;
+  int apply_variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};  // autotuned, by R bucket
 };
 
 }  // namespace hm
@@ -226,6 +245,49 @@ struct MfmaTile {
       stash<A_MFAST, B_NFAST>(lds);
       __syncthreads();
       if (k0 + KC < K) fetch<A_MFAST, B_NFAST>(k0 + KC, la, lb);
+      compute(lds);
+    }
+    __syncthreads();
+  }
+
+  // Interior form of run(): A is column-major with leading dimension lda
+  // (rows contiguous), B(k, j) sits at B[k * brs + j * bcs]; the tile lies
+  // fully inside both operands and K is a multiple of KC, so each thread
+  // walks fixed pointers with no index arithmetic or bounds checks.
+  template <bool B_NFAST>
+  __device__ void run_interior(double *lds, int K, const double *A, long long lda,
+                               const double *B, long long brs, long long bcs) {
+    static_assert(NT % TM == 0 && NT % KC == 0 && NT % TN == 0, "tile shape");
+    if (K <= 0) return;
+    const int t = threadIdx.x;
+    const double *pa = A + (t % TM) + (long long)(t / TM) * lda;
+    const long long aes = (long long)(NT / TM) * lda, astep = (long long)KC * lda;
+    const double *pb;
+    long long bes;
+    if (B_NFAST) {
+      pb = B + (long long)(t % TN) * bcs + (long long)(t / TN) * brs;
+      bes = (long long)(NT / TN) * brs;
+    } else {
+      pb = B + (long long)(t % KC) * brs + (long long)(t / KC) * bcs;
+      bes = (long long)(NT / KC) * bcs;
+    }
+    const long long bstep = (long long)KC * brs;
+#pragma unroll
+    for (int e = 0; e < AREG; e++) ra[e] = pa[e * aes];
+#pragma unroll
+    for (int e = 0; e < BREG; e++) rb[e] = pb[e * bes];
+    for (int k0 = 0; k0 < K; k0 += KC) {
+      __syncthreads();
+      stash<true, B_NFAST>(lds);
+      __syncthreads();
+      if (k0 + KC < K) {
+        pa += astep;
+        pb += bstep;
+#pragma unroll
+        for (int e = 0; e < AREG; e++) ra[e] = pa[e * aes];
+#pragma unroll
+        for (int e = 0; e < BREG; e++) rb[e] = pb[e * bes];
+      }
       compute(lds);
     }
     __syncthreads();

@@ -4,67 +4,111 @@
 // (src/vector_algebra.c:52-197, src/dense_algebra.c:82-1151).  Here a
 // product is three launches over the whole tree:
 //
-//   partials  P_c = F_in(c)^T X_c for every leaf c and every level at once
-//             (X_c read once; F_in = V for H X, U for H^T X)
-//   reduce    W_b = sum of the partials of the leaves under the block's
-//             column node (fixed order, deterministic)
+//   partials  P(l, k) = F_in(k)^T X_k for every row chunk k (<= 64 rows of
+//             one leaf) and every level l at once (X read once; F_in = V
+//             for H X, U for H^T X)
+//   reduce    W_b = sum of the partials of the chunks under the block's
+//             column node: one contiguous region per block, summed in a
+//             fixed order (deterministic)
 //   apply     Y_c = D_c X_c (or D_c^T X_c) + sum_l F_out rows(c) W_b
 //             as one K-concatenated GEMM per leaf row tile
 //
 // so every diagonal and U/V element is read once from HBM.  R = 1 with no
 // transpose runs wavefront-reduction kernels; everything else runs FP64
-// MFMA tiles (v_mfma_f64_16x16x4_f64) staged through LDS.
+// MFMA tiles (v_mfma_f64_16x16x4_f64) staged through LDS.  The apply tile
+// shape is chosen per tree and RHS width by a small timing autotuner.
+#include <cstdlib>
+
 #include "hm_common.h"
 
 namespace hm {
 
 struct ProdArgs {
-  int h, nleaf, R, trans;
+  int h, nleaf, R, trans, nchunk, pad;
   const int *leaf_off;
   const long long *diag_off;
   const BlockMeta *blk;
   const double *diag, *U, *V;
+  const int4 *ch;
+  const int *leaf_chunk0;
   const long long *pofs;
-  const int *proff;
   const long long *wofs;
-  double *P, *W;
+  const int2 *seg_list;
+  const int *multi_list;
+  const long long *part_ofs;
+  double *P, *W, *Wp;
   const double *X;
   long long x_rs, x_cs;
   double *Y;
   long long y_rs, y_cs;
 };
 
+// Per-level view of one leaf, preloaded into LDS by threads 0..h-1 so that
+// no thread walks the levels through dependent global loads.
+struct LevelInfo {
+  const double *F;  // factor panel at the leaf's first row (input or output side)
+  double *P;        // partial block of this chunk (input side) or W_b (output side)
+  int ld, s, g, pad;
+};
+static constexpr int MAX_LEVELS = 24;
+
+// side 0: input side (V for H X, U for H^T X), P = partial of chunk k;
+// side 1: output side (U for H X, V for H^T X), P = W_b.
+__device__ __forceinline__ void load_levels(const ProdArgs &a, int c, int k, int side,
+                                            LevelInfo *li) {
+  const int l = threadIdx.x + 1;
+  if (l <= a.h) {
+    const int q = c >> (a.h - l);
+    const bool want_q = (side == 0) == (a.trans != 0);  // block (l, q) or (l, q^1)
+    const int g = blk_index(l, want_q ? q : (q ^ 1));
+    const BlockMeta b = a.blk[g];
+    const int off = a.leaf_off[c] - a.leaf_off[q << (a.h - l)];
+    const bool use_u = want_q;  // rows of block (l, q) are the leaf's rows: its U
+    LevelInfo x;
+    x.F = use_u ? a.U + b.uoff + off : a.V + b.voff + off;
+    x.ld = use_u ? b.m : b.n;
+    x.s = b.s;
+    x.g = g;
+    x.pad = 0;
+    x.P = side == 0 ? a.P + a.pofs[(long long)(l - 1) * a.nchunk + k] * a.R
+                    : a.W + a.wofs[g] * a.R;
+    li[l - 1] = x;
+  }
+}
+
 // ----------------------------------------------------------- R = 1 path
 __global__ __launch_bounds__(256) void k_mv_partials(ProdArgs a) {
-  const int c = blockIdx.x;
+  __shared__ LevelInfo li[MAX_LEVELS];
+  const int k = blockIdx.x;
+  const int4 chk = a.ch[k];
+  const int c = chk.x, r0 = chk.y, rows = chk.z;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lo = a.leaf_off[c], mc = a.leaf_off[c + 1] - lo;
-  double *Pc = a.P + a.pofs[c];
+  load_levels(a, c, k, 0, li);
+  const int lo = a.leaf_off[c] + r0;
+  const double xi = lane < rows ? a.X[(long long)(lo + lane) * a.x_rs] : 0.0;
+  __syncthreads();
   int o = 0;
-  for (int l = 1; l <= a.h; l++) {
-    const int q = c >> (a.h - l);
-    const BlockMeta b = a.blk[blk_index(l, q ^ 1)];
-    const int off = lo - a.leaf_off[q << (a.h - l)];
-    const double *F = a.V + b.voff + off;
-    for (int k = 0; k < b.s; k++) {
-      if (((o + k) & 3) != wave) continue;
-      double acc = 0.0;
-      for (int i = lane; i < mc; i += 64)
-        acc += F[i + (long long)k * b.n] * a.X[(long long)(lo + i) * a.x_rs];
+  for (int l = 0; l < a.h; l++) {
+    const LevelInfo x = li[l];
+    for (int kk = 0; kk < x.s; kk++) {
+      if (((o + kk) & 3) != wave) continue;
+      double acc = lane < rows ? x.F[r0 + lane + (long long)kk * x.ld] * xi : 0.0;
       acc = wave_sum(acc);
-      if (lane == 0) Pc[o + k] = acc;
+      if (lane == 0) x.P[kk] = acc;
     }
-    o += b.s;
+    o += x.s;
   }
 }
 
 __global__ __launch_bounds__(256) void k_mv_apply(ProdArgs a, int rtiles) {
   __shared__ double red[4][64];
+  __shared__ LevelInfo li[MAX_LEVELS];
   const int c = blockIdx.x / rtiles, rt = blockIdx.x % rtiles;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lo = a.leaf_off[c], mc = a.leaf_off[c + 1] - lo;
   const int r0 = rt * 64;
   if (r0 >= mc) return;
+  load_levels(a, c, 0, 1, li);
   const int i = r0 + lane;
   const bool valid = i < mc;
   const int ii = valid ? i : 0;
@@ -79,14 +123,10 @@ __global__ __launch_bounds__(256) void k_mv_apply(ProdArgs a, int rtiles) {
   }
   if (j < j1) acc0 += D[ii + (long long)j * mc] * a.X[(long long)(lo + j) * a.x_rs];
   double acc = acc0 + acc1;
-  for (int l = 1 + wave; l <= a.h; l += 4) {
-    const int q = c >> (a.h - l);
-    const int g = blk_index(l, q);
-    const BlockMeta b = a.blk[g];
-    const int off = lo - a.leaf_off[q << (a.h - l)];
-    const double *F = a.U + b.uoff + off;
-    const double *Wb = a.W + a.wofs[g];
-    for (int k = 0; k < b.s; k++) acc += F[ii + (long long)k * b.m] * Wb[k];
+  __syncthreads();
+  for (int l = wave; l < a.h; l += 4) {
+    const LevelInfo x = li[l];
+    for (int k = 0; k < x.s; k++) acc += x.F[ii + (long long)k * x.ld] * x.P[k];
   }
   red[wave][lane] = acc;
   __syncthreads();
@@ -97,116 +137,207 @@ __global__ __launch_bounds__(256) void k_mv_apply(ProdArgs a, int rtiles) {
 }
 
 // ------------------------------------------------------------ reduction
+// out[e] = sum over p < nparts of base[p * stride + e], e < E, by one
+// workgroup of 256 threads; when E < 256 the parts are split over 256 / E
+// thread groups and combined through LDS in a fixed order.
+__device__ __forceinline__ void group_sum(const double *base, long long stride, int nparts,
+                                          long long E, double *out, double *red) {
+  if (E >= 256) {
+    for (long long e = threadIdx.x; e < E; e += 256) {
+      double s0 = 0.0, s1 = 0.0;
+      int p = 0;
+      for (; p + 1 < nparts; p += 2) {
+        s0 += base[p * stride + e];
+        s1 += base[(p + 1) * stride + e];
+      }
+      if (p < nparts) s0 += base[p * stride + e];
+      out[e] = s0 + s1;
+    }
+    return;
+  }
+  const int G = 256 / (int)E;
+  const int g = threadIdx.x / (int)E, e = threadIdx.x % (int)E;
+  double s0 = 0.0, s1 = 0.0;
+  if (g < G) {
+    int p = g;
+    for (; p + G < nparts; p += 2 * G) {
+      s0 += base[p * stride + e];
+      s1 += base[(p + G) * stride + e];
+    }
+    if (p < nparts) s0 += base[p * stride + e];
+  }
+  red[threadIdx.x] = s0 + s1;
+  __syncthreads();
+  if (threadIdx.x < E) {
+    double t = 0.0;
+    for (int gg = 0; gg < G; gg++) t += red[gg * E + threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+// chunk range of the source node of block g (the node its columns span in
+// H X; its rows in H^T X)
+__device__ __forceinline__ void src_chunks(const ProdArgs &a, const BlockMeta &b, int &cb,
+                                           int &ce) {
+  const int span = 1 << (a.h - b.level);
+  const int src = a.trans ? b.j : (b.j ^ 1);
+  cb = a.leaf_chunk0[src * span];
+  ce = a.leaf_chunk0[(src + 1) * span];
+}
+
+// stage 1 (blocks with more than one segment): segment sums
+__global__ __launch_bounds__(256) void k_reduce_seg(ProdArgs a) {
+  __shared__ double red[256];
+  const int2 it = a.seg_list[blockIdx.x];
+  const BlockMeta b = a.blk[it.x];
+  int cb, ce;
+  src_chunks(a, b, cb, ce);
+  const int c0 = cb + it.y * REDUCE_SEG;
+  const int nparts = max(0, min(ce, c0 + REDUCE_SEG) - c0);
+  const long long E = (long long)b.s * a.R;
+  const double *base = a.P + a.pofs[(long long)(b.level - 1) * a.nchunk + c0] * a.R;
+  double *out = a.Wp + (a.part_ofs[it.x] + (long long)it.y * b.s) * a.R;
+  group_sum(base, E, nparts, E, out, red);
+}
+
+// stage 2: W_b = sum of the segment sums (or of the chunk partials when the
+// block has a single segment)
 __global__ __launch_bounds__(256) void k_reduce_w(ProdArgs a) {
+  __shared__ double red[256];
   const int g = blockIdx.x;
   const BlockMeta b = a.blk[g];
-  const int s = b.s;
-  if (s == 0) return;
-  const int l = b.level;
-  const int src = a.trans ? b.j : (b.j ^ 1);
-  const int span = 1 << (a.h - l);
-  const int c_lo = src * span;
-  double *Wb = a.W + a.wofs[g] * a.R;
-  const long long tot = (long long)s * a.R;
-  for (long long e = threadIdx.x + (long long)blockIdx.y * blockDim.x; e < tot;
-       e += (long long)blockDim.x * gridDim.y) {
-    const int k = (int)(e % s), r = (int)(e / s);
-    double sum = 0.0;
-    for (int c = c_lo; c < c_lo + span; c++) {
-      const long long p0 = a.pofs[c];
-      const long long sin_c = a.pofs[c + 1] - p0;
-      sum += a.P[p0 * a.R + a.proff[(long long)c * a.h + (l - 1)] + k + r * sin_c];
-    }
-    Wb[k + (long long)r * s] = sum;
+  if (b.s == 0) return;
+  int cb, ce;
+  src_chunks(a, b, cb, ce);
+  const long long E = (long long)b.s * a.R;
+  const int nch = ce - cb;
+  const int nseg = (nch + REDUCE_SEG - 1) / REDUCE_SEG;
+  if (nseg > 1) {
+    group_sum(a.Wp + a.part_ofs[g] * a.R, E, nseg, E, a.W + a.wofs[g] * a.R, red);
+  } else {
+    const double *base = a.P + a.pofs[(long long)(b.level - 1) * a.nchunk + cb] * a.R;
+    group_sum(base, E, nch, E, a.W + a.wofs[g] * a.R, red);
   }
 }
 
 // ------------------------------------------------------------ MFMA path
 typedef MfmaTile<1, 4, 1, 1, 16> PartTile;  // 16 x 64
-typedef MfmaTile<2, 2, 2, 2, 16> ApplyTile; // 64 x 64
 
 template <bool B_NFAST>
 __global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
                                                      int ntiles) {
   __shared__ double lds[PartTile::LDS_DOUBLES];
   __shared__ const double *rowptr[16];
+  __shared__ double *outptr[16];
+  __shared__ int outld[16];
+  __shared__ LevelInfo li[MAX_LEVELS];
   int bid = blockIdx.x;
   const int nt = bid % ntiles;
   bid /= ntiles;
   const int mt = bid % mtiles;
-  const int c = bid / mtiles;
-  const int lo = a.leaf_off[c], mc = a.leaf_off[c + 1] - lo;
-  const long long p0 = a.pofs[c];
-  const int sin_c = (int)(a.pofs[c + 1] - p0);
+  const int k = bid / mtiles;
+  const int4 chk = a.ch[k];
+  const int c = chk.x, r0 = chk.y, rows = chk.z;
+  const int lo = a.leaf_off[c] + r0;
   const int rho0 = mt * 16;
-  if (rho0 >= sin_c) return;
+  load_levels(a, c, k, 0, li);
+  __syncthreads();
   if (threadIdx.x < 16) {
     const int rho = rho0 + threadIdx.x;
     const double *ptr = nullptr;
-    if (rho < sin_c) {
-      for (int l = 1; l <= a.h; l++) {
-        const int q = c >> (a.h - l);
-        const int bi = a.trans ? q : (q ^ 1);
-        const BlockMeta b = a.blk[blk_index(l, bi)];
-        const int base = a.proff[(long long)c * a.h + (l - 1)];
-        if (rho >= base && rho < base + b.s) {
-          const int off = lo - a.leaf_off[q << (a.h - l)];
-          const int k = rho - base;
-          ptr = a.trans ? a.U + b.uoff + off + (long long)k * b.m
-                        : a.V + b.voff + off + (long long)k * b.n;
-          break;
-        }
+    double *optr = nullptr;
+    int old = 1;
+    int base = 0;
+    for (int l = 0; l < a.h; l++) {
+      const LevelInfo x = li[l];
+      if (rho >= base && rho < base + x.s) {
+        const int kk = rho - base;
+        ptr = x.F + r0 + (long long)kk * x.ld;
+        optr = x.P + kk;
+        old = x.s;
+        break;
       }
+      base += x.s;
     }
     rowptr[threadIdx.x] = ptr;
+    outptr[threadIdx.x] = optr;
+    outld[threadIdx.x] = old;
   }
   __syncthreads();
+  bool any = false;
+  for (int r = 0; r < 16; r++) any |= (rowptr[r] != nullptr);
+  if (!any) return;
   const int n0 = nt * 64;
-  auto la = [&](int i, int k) -> double {
+  auto la = [&](int i, int kx) -> double {
     const double *p = rowptr[i];
-    return (p && k < mc) ? p[k] : 0.0;
+    return (p && kx < rows) ? p[kx] : 0.0;
   };
-  auto lb = [&](int k, int j) -> double {
-    return (k < mc && n0 + j < a.R)
-               ? a.X[(long long)(lo + k) * a.x_rs + (long long)(n0 + j) * a.x_cs]
+  auto lb = [&](int kx, int j) -> double {
+    return (kx < rows && n0 + j < a.R)
+               ? a.X[(long long)(lo + kx) * a.x_rs + (long long)(n0 + j) * a.x_cs]
                : 0.0;
   };
   PartTile tile;
   tile.zero();
-  tile.template run<false, B_NFAST>(lds, mc, la, lb);
-  double *Pc = a.P + p0 * a.R;
+  tile.template run<false, B_NFAST>(lds, rows, la, lb);
   tile.store([&](int i, int j, double v) {
-    const int rho = rho0 + i;
-    if (rho < sin_c && n0 + j < a.R) Pc[rho + (long long)(n0 + j) * sin_c] = v;
+    double *o = outptr[i];
+    if (o && n0 + j < a.R) o[(long long)(n0 + j) * outld[i]] = v;
   });
 }
 
-template <bool B_NFAST>
-__global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles,
-                                                  int ntiles) {
-  __shared__ double lds[ApplyTile::LDS_DOUBLES];
-  __shared__ int lp[33];
-  __shared__ const double *fb[32];
-  __shared__ const double *wb[32];
-  __shared__ int fld[32], ws[32];
-  int bid = blockIdx.x;
-  const int nt = bid % ntiles;
-  bid /= ntiles;
-  const int rt = bid % rtiles;
-  const int c = bid / rtiles;
+// Work order of the apply kernel: leaves are dealt to the 8 XCDs round
+// robin (workgroup b runs on XCD b % 8), and all row/column tiles of one
+// leaf stay on its XCD so that X_c and the W_b are shared through one L2.
+__device__ __forceinline__ bool apply_tile_of(int bid, int nleaf, int rtiles, int ntiles,
+                                              int &c, int &rt, int &nt) {
+  const int xcd = bid & 7, slot = bid >> 3;
+  const int per = rtiles * ntiles;
+  c = (slot / per) * 8 + xcd;
+  const int w = slot % per;
+  rt = w / ntiles;
+  nt = w % ntiles;
+  return c < nleaf;
+}
+
+// FAST: every leaf is a whole number of TM x KC tiles, R is a multiple of
+// TN and there is no transpose (checked on the host); only the
+// interior form of the diagonal GEMM is compiled in.
+template <bool B_NFAST, int WM, int FM, int FN, int KC, bool FAST>
+__global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles, int ntiles) {
+  typedef MfmaTile<WM, 4 / WM, FM, FN, KC> Tile;
+  __shared__ double lds[Tile::LDS_DOUBLES];
+  __shared__ int lp[MAX_LEVELS + 1];
+  __shared__ LevelInfo li[MAX_LEVELS];
+  __shared__ const double *tabA[KC];
+  __shared__ const double *tabB[KC];
+  __shared__ int tabS[KC];
+  int c, rt, nt;
+  if (!apply_tile_of(blockIdx.x, a.nleaf, rtiles, ntiles, c, rt, nt)) return;
   const int lo = a.leaf_off[c], mc = a.leaf_off[c + 1] - lo;
-  const int r0 = rt * 64;
+  const int r0 = rt * Tile::TM;
   if (r0 >= mc) return;
-  const int n0 = nt * 64;
+  const int n0 = nt * Tile::TN;
   const double *D = a.diag + a.diag_off[c];
-  ApplyTile tile;
+  Tile tile;
   tile.zero();
   auto lb = [&](int k, int j) -> double {
     return (k < mc && n0 + j < a.R)
                ? a.X[(long long)(lo + k) * a.x_rs + (long long)(n0 + j) * a.x_cs]
                : 0.0;
   };
-  if (!a.trans) {
+  load_levels(a, c, 0, 1, li);
+  const bool interior = FAST || ((r0 + Tile::TM <= mc) && (mc % KC == 0) &&
+                                 (n0 + Tile::TN <= a.R) && !a.trans);
+  if (FAST) {
+    tile.template run_interior<B_NFAST>(lds, mc, D + r0, mc,
+                                        a.X + (long long)lo * a.x_rs + (long long)n0 * a.x_cs,
+                                        a.x_rs, a.x_cs);
+  } else if (interior) {
+    tile.template run_interior<B_NFAST>(lds, mc, D + r0, mc,
+                                        a.X + (long long)lo * a.x_rs + (long long)n0 * a.x_cs,
+                                        a.x_rs, a.x_cs);
+  } else if (!a.trans) {
     auto la = [&](int i, int k) -> double {
       return (r0 + i < mc && k < mc) ? D[(r0 + i) + (long long)k * mc] : 0.0;
     };
@@ -220,44 +351,51 @@ __global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles,
   // low-rank segment: K runs over (level, rank column) pairs
   if (threadIdx.x == 0) {
     int run = 0;
-    for (int l = 1; l <= a.h; l++) {
-      const int q = c >> (a.h - l);
-      const int bo = a.trans ? (q ^ 1) : q;
-      const int g = blk_index(l, bo);
-      const BlockMeta b = a.blk[g];
-      const int off = lo - a.leaf_off[q << (a.h - l)];
-      lp[l - 1] = run;
-      fb[l - 1] = a.trans ? a.V + b.voff + off : a.U + b.uoff + off;
-      fld[l - 1] = a.trans ? b.n : b.m;
-      wb[l - 1] = a.W + a.wofs[g] * a.R;
-      ws[l - 1] = b.s;
-      run += b.s;
+    for (int l = 0; l < a.h; l++) {
+      lp[l] = run;
+      run += li[l].s;
     }
     lp[a.h] = run;
   }
   __syncthreads();
   const int K2 = lp[a.h];
-  if (K2 > 0) {
-    const int h = a.h;
-    auto lvl = [&](int kap) -> int {
-      int l = 0;
-      while (l + 1 < h && kap >= lp[l + 1]) l++;
-      return l;
+  // K runs over (level, rank column) pairs; each K-chunk first resolves its
+  // KC pairs into LDS pointers (factor column at this tile's first row, W
+  // row and stride), then streams through the MFMA tile.
+  for (int k0 = 0; k0 < K2; k0 += KC) {
+    if (threadIdx.x < KC) {
+      const int kap = k0 + threadIdx.x;
+      const double *pa = nullptr, *pb = nullptr;
+      int st = 0;
+      if (kap < K2) {
+        int l = 0;
+        while (l + 1 < a.h && kap >= lp[l + 1]) l++;
+        const int kk = kap - lp[l];
+        pa = li[l].F + r0 + (long long)kk * li[l].ld;
+        pb = li[l].P + kk;
+        st = li[l].s;
+      }
+      tabA[threadIdx.x] = pa;
+      tabB[threadIdx.x] = pb;
+      tabS[threadIdx.x] = st;
+    }
+    __syncthreads();
+    auto la2 = [&](int i, int k) -> double {
+      const double *p = tabA[k];
+      return (p && (FAST || r0 + i < mc)) ? p[i] : 0.0;
     };
-    auto la2 = [&](int i, int kap) -> double {
-      if (r0 + i >= mc || kap >= K2) return 0.0;
-      const int l = lvl(kap);
-      return fb[l][(r0 + i) + (long long)(kap - lp[l]) * fld[l]];
+    auto lb2 = [&](int k, int j) -> double {
+      const double *p = tabB[k];
+      return (p && (FAST || n0 + j < a.R)) ? p[(long long)(n0 + j) * tabS[k]] : 0.0;
     };
-    auto lb2 = [&](int kap, int j) -> double {
-      if (kap >= K2 || n0 + j >= a.R) return 0.0;
-      const int l = lvl(kap);
-      return wb[l][(kap - lp[l]) + (long long)(n0 + j) * ws[l]];
-    };
-    tile.template run<true, false>(lds, K2, la2, lb2);
+    tile.template fetch<true, false>(0, la2, lb2);
+    tile.template stash<true, false>(lds);
+    __syncthreads();
+    tile.compute(lds);
+    __syncthreads();
   }
   tile.store([&](int i, int j, double v) {
-    if (r0 + i < mc && n0 + j < a.R)
+    if (FAST || (r0 + i < mc && n0 + j < a.R))
       a.Y[(long long)(lo + r0 + i) * a.y_rs + (long long)(n0 + j) * a.y_cs] = v;
   });
 }
@@ -270,18 +408,25 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   a.nleaf = t->nleaf;
   a.R = R;
   a.trans = trans;
+  a.nchunk = t->plan.nchunk;
+  a.pad = 0;
   a.leaf_off = t->d_leaf_off;
   a.diag_off = t->d_diag_off;
   a.blk = t->d_blk;
   a.diag = t->d_diag;
   a.U = t->d_U;
   a.V = t->d_V;
+  a.ch = t->plan.d_ch;
+  a.leaf_chunk0 = t->plan.d_leaf_chunk0;
   a.pofs = t->plan.d_pofs[trans];
-  a.proff = t->plan.d_proff[trans];
   a.wofs = t->plan.d_wofs;
-  long long pmax = std::max(t->plan.pofs[0][t->nleaf], t->plan.pofs[1][t->nleaf]);
+  a.seg_list = t->plan.d_seg_list;
+  a.multi_list = t->plan.d_multi_list;
+  a.part_ofs = t->plan.d_part_ofs;
+  long long pmax = std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back());
   a.P = t->d_scratch;
   a.W = t->d_scratch + pmax * R;
+  a.Wp = a.W + t->plan.wofs[t->nblock] * R;
   a.X = X;
   a.x_rs = x_rs;
   a.x_cs = x_cs;
@@ -289,6 +434,110 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   a.y_rs = y_rs;
   a.y_cs = y_cs;
   return a;
+}
+
+// Apply-kernel variants: (WM, FM, FN, KC) -> tile TM = WM*FM*16 rows x
+// TN = (4/WM)*FN*16 columns, K-chunk KC.
+struct ApplyVariant {
+  int tm, tn, kc;
+  void (*launch)(const ProdArgs &, bool nfast, bool fast, int rtiles, int ntiles,
+                 int grid, hipStream_t s);
+};
+
+template <int WM, int FM, int FN, int KC>
+static void launch_apply(const ProdArgs &a, bool nfast, bool fast, int rtiles, int ntiles,
+                         int grid, hipStream_t s) {
+  if (fast) {
+    if (nfast)
+      hipLaunchKernelGGL((k_mm_apply<true, WM, FM, FN, KC, true>), dim3(grid), dim3(256), 0,
+                         s, a, rtiles, ntiles);
+    else
+      hipLaunchKernelGGL((k_mm_apply<false, WM, FM, FN, KC, true>), dim3(grid), dim3(256), 0,
+                         s, a, rtiles, ntiles);
+  } else {
+    if (nfast)
+      hipLaunchKernelGGL((k_mm_apply<true, WM, FM, FN, KC, false>), dim3(grid), dim3(256), 0,
+                         s, a, rtiles, ntiles);
+    else
+      hipLaunchKernelGGL((k_mm_apply<false, WM, FM, FN, KC, false>), dim3(grid), dim3(256),
+                         0, s, a, rtiles, ntiles);
+  }
+}
+
+static const ApplyVariant kVariants[] = {
+    {64, 64, 16, launch_apply<2, 2, 2, 16>},   // 64 x 64, KC 16
+    {64, 64, 32, launch_apply<2, 2, 2, 32>},   // 64 x 64, KC 32
+    {128, 64, 32, launch_apply<2, 4, 2, 32>},  // 128 x 64, KC 32
+    {128, 32, 32, launch_apply<4, 2, 2, 32>},  // 128 x 32, KC 32
+};
+static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+static void run_apply(hm_dtree *t, const ProdArgs &a, int v, bool nfast, hipStream_t s) {
+  const ApplyVariant &av = kVariants[v];
+  const int rtiles = (t->max_leaf + av.tm - 1) / av.tm;
+  const int ntiles = (a.R + av.tn - 1) / av.tn;
+  const int grid = ((t->nleaf + 7) / 8) * 8 * rtiles * ntiles;
+  bool fast = !a.trans && (a.R % av.tn == 0);
+  for (int c = 0; fast && c < t->nleaf; c++) {
+    const int m = t->leaf_off[c + 1] - t->leaf_off[c];
+    fast = (m % av.tm == 0) && (m % av.kc == 0);
+  }
+  av.launch(a, nfast, fast, rtiles, ntiles, grid, s);
+}
+
+static int r_bucket(int R) { return R <= 16 ? 0 : (R <= 128 ? 1 : 2); }
+
+// Pick the fastest apply variant for this tree, transpose mode and RHS
+// bucket by timing each once (HIP events) on the live operands.  HM_TUNE=0
+// disables tuning (variant 1), HM_APPLY_VARIANT=v forces one.
+static int choose_variant(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_t s) {
+  int &slot = t->plan.apply_variant[a.trans][r_bucket(a.R)];
+  if (slot >= 0) return slot;
+  const char *force = getenv("HM_APPLY_VARIANT");
+  if (force) {
+    slot = std::max(0, std::min(kNumVariants - 1, atoi(force)));
+    return slot;
+  }
+  const char *tune = getenv("HM_TUNE");
+  if (tune && tune[0] == '0') {
+    slot = 1;
+    return slot;
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    slot = 1;
+    return slot;
+  }
+  float best = 1e30f;
+  int bestv = 1;
+  for (int v = 0; v < kNumVariants; v++) {
+    run_apply(t, a, v, nfast, s);  // warm
+    (void)hipEventRecord(e0, s);
+    for (int rep = 0; rep < 3; rep++) run_apply(t, a, v, nfast, s);
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) {
+      best = ms;
+      bestv = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  slot = bestv;
+  return slot;
+}
+
+static int run_reduce(hm_dtree *t, const ProdArgs &a, hipStream_t s) {
+  Timed tr = timing_begin("reduce_w", s);
+  if (!t->plan.seg_list.empty())
+    hipLaunchKernelGGL(k_reduce_seg, dim3((unsigned)t->plan.seg_list.size()), dim3(256), 0,
+                       s, a);
+  hipLaunchKernelGGL(k_reduce_w, dim3(t->nblock), dim3(256), 0, s, a);
+  timing_end(tr, s);
+  HM_LAUNCH_CHECK();
+  return SUCCESS;
 }
 
 int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
@@ -304,30 +553,28 @@ int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
   }
   HM_OK(ensure_scratch(t, R));
   ProdArgs a = make_args(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs);
-  const int nb = t->nblock;
+  const int nchunk = t->plan.nchunk;
   if (R == 1 && !trans) {
-    Timed tp = timing_begin("mv_partials", s);
-    hipLaunchKernelGGL(k_mv_partials, dim3(t->nleaf), dim3(256), 0, s, a);
-    timing_end(tp, s);
-    HM_LAUNCH_CHECK();
-    Timed tr = timing_begin("reduce_w", s);
-    hipLaunchKernelGGL(k_reduce_w, dim3(nb, 1), dim3(256), 0, s, a);
-    timing_end(tr, s);
-    HM_LAUNCH_CHECK();
+    if (nchunk > 0) {
+      Timed tp = timing_begin("mv_partials", s);
+      hipLaunchKernelGGL(k_mv_partials, dim3(nchunk), dim3(256), 0, s, a);
+      timing_end(tp, s);
+      HM_LAUNCH_CHECK();
+    }
+    HM_OK(run_reduce(t, a, s));
     const int rtiles = (t->max_leaf + 63) / 64;
     Timed ta = timing_begin("mv_apply", s);
-    hipLaunchKernelGGL(k_mv_apply, dim3(t->nleaf * rtiles), dim3(256), 0, s, a,
-                       rtiles);
+    hipLaunchKernelGGL(k_mv_apply, dim3(t->nleaf * rtiles), dim3(256), 0, s, a, rtiles);
     timing_end(ta, s);
     HM_LAUNCH_CHECK();
     return SUCCESS;
   }
   const bool nfast = (x_rs != 1);
-  const int ntiles = (R + 63) / 64;
   const int smax = t->plan.s_in_max[trans];
-  if (smax > 0) {
+  if (smax > 0 && nchunk > 0) {
+    const int ntiles = (R + 63) / 64;
     const int mtiles = (smax + 15) / 16;
-    dim3 grid((unsigned)(t->nleaf * mtiles * ntiles));
+    dim3 grid((unsigned)(nchunk * mtiles * ntiles));
     Timed tp = timing_begin("mm_partials", s);
     if (nfast)
       hipLaunchKernelGGL(k_mm_partials<true>, grid, dim3(256), 0, s, a, mtiles, ntiles);
@@ -335,21 +582,11 @@ int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
       hipLaunchKernelGGL(k_mm_partials<false>, grid, dim3(256), 0, s, a, mtiles, ntiles);
     timing_end(tp, s);
     HM_LAUNCH_CHECK();
-    long long maxwork = (long long)t->max_rank * R;
-    int ysplit = (int)std::min<long long>(64, (maxwork + 255) / 256);
-    if (ysplit < 1) ysplit = 1;
-    Timed tr = timing_begin("reduce_w", s);
-    hipLaunchKernelGGL(k_reduce_w, dim3(nb, ysplit), dim3(256), 0, s, a);
-    timing_end(tr, s);
-    HM_LAUNCH_CHECK();
+    HM_OK(run_reduce(t, a, s));
   }
-  const int rtiles = (t->max_leaf + 63) / 64;
-  dim3 grid((unsigned)(t->nleaf * rtiles * ntiles));
+  const int v = choose_variant(t, a, nfast, s);
   Timed ta = timing_begin("mm_apply", s);
-  if (nfast)
-    hipLaunchKernelGGL(k_mm_apply<true>, grid, dim3(256), 0, s, a, rtiles, ntiles);
-  else
-    hipLaunchKernelGGL(k_mm_apply<false>, grid, dim3(256), 0, s, a, rtiles, ntiles);
+  run_apply(t, a, v, nfast, s);
   timing_end(ta, s);
   HM_LAUNCH_CHECK();
   return SUCCESS;
@@ -371,6 +608,11 @@ int hm_hip_matmat(hm_dtree *t, int trans, int R, const double *d_X, long x_rs,
   HM_OK(ensure_init());
   return products_run(t, trans ? 1 : 0, R, d_X, x_rs, x_cs, d_Y, y_rs, y_cs,
                       stream_of(stream));
+}
+
+int hm_hip_matmat_variant(const hm_dtree *t, int trans, int R) {
+  if (!t) return -1;
+  return t->plan.apply_variant[trans ? 1 : 0][r_bucket(R)];
 }
 
 }  // extern "C"

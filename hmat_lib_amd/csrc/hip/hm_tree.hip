@@ -17,10 +17,12 @@ void tree_free(hm_dtree *t) {
   dfree(t->d_leaf_off);
   dfree(t->d_diag_off);
   dfree(t->d_blk);
-  for (int k = 0; k < 2; k++) {
-    dfree(t->plan.d_pofs[k]);
-    dfree(t->plan.d_proff[k]);
-  }
+  for (int k = 0; k < 2; k++) dfree(t->plan.d_pofs[k]);
+  dfree(t->plan.d_ch);
+  dfree(t->plan.d_seg_list);
+  dfree(t->plan.d_multi_list);
+  dfree(t->plan.d_part_ofs);
+  dfree(t->plan.d_leaf_chunk0);
   dfree(t->plan.d_wofs);
   dfree(t->d_scratch);
   delete t;
@@ -56,36 +58,79 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     t->max_rank = std::max(t->max_rank, b.s);
   }
   ProductPlan &p = t->plan;
+  p.ch_leaf.clear();
+  p.ch_r0.clear();
+  p.ch_rows.clear();
+  p.leaf_chunk0.assign(nl + 1, 0);
+  for (int c = 0; c < nl; c++) {
+    p.leaf_chunk0[c] = (int)p.ch_leaf.size();
+    const int m = t->leaf_off[c + 1] - t->leaf_off[c];
+    for (int r = 0; r < m; r += CHUNK_ROWS) {
+      p.ch_leaf.push_back(c);
+      p.ch_r0.push_back(r);
+      p.ch_rows.push_back(std::min(CHUNK_ROWS, m - r));
+    }
+  }
+  p.leaf_chunk0[nl] = (int)p.ch_leaf.size();
+  p.nchunk = (int)p.ch_leaf.size();
   for (int tr = 0; tr < 2; tr++) {
-    p.pofs[tr].assign(nl + 1, 0);
-    p.proff[tr].assign((size_t)nl * h, 0);
+    p.pofs[tr].assign((size_t)h * p.nchunk + 1, 0);
     p.s_in_max[tr] = 0;
     p.s_out_max[tr] = 0;
-    for (int c = 0; c < nl; c++) {
-      int run_in = 0, run_out = 0;
-      for (int l = 1; l <= h; l++) {
-        int q = c >> (h - l);
-        int bin = tr ? q : (q ^ 1);
-        int bout = tr ? (q ^ 1) : q;
-        p.proff[tr][(size_t)c * h + (l - 1)] = run_in;
-        run_in += t->blk[blk_index(l, bin)].s;
-        run_out += t->blk[blk_index(l, bout)].s;
+    long long run = 0;
+    for (int l = 1; l <= h; l++) {
+      for (int k = 0; k < p.nchunk; k++) {
+        const int q = p.ch_leaf[k] >> (h - l);
+        const int bin = tr ? q : (q ^ 1);
+        p.pofs[tr][(size_t)(l - 1) * p.nchunk + k] = run;
+        run += t->blk[blk_index(l, bin)].s;
       }
-      p.pofs[tr][c + 1] = p.pofs[tr][c] + run_in;
-      p.s_in_max[tr] = std::max(p.s_in_max[tr], run_in);
-      p.s_out_max[tr] = std::max(p.s_out_max[tr], run_out);
+    }
+    p.pofs[tr][(size_t)h * p.nchunk] = run;
+    for (int c = 0; c < nl; c++) {
+      int sin = 0, sout = 0;
+      for (int l = 1; l <= h; l++) {
+        const int q = c >> (h - l);
+        sin += t->blk[blk_index(l, tr ? q : (q ^ 1))].s;
+        sout += t->blk[blk_index(l, tr ? (q ^ 1) : q)].s;
+      }
+      p.s_in_max[tr] = std::max(p.s_in_max[tr], sin);
+      p.s_out_max[tr] = std::max(p.s_out_max[tr], sout);
     }
   }
   p.wofs.assign(t->nblock + 1, 0);
   for (int g = 0; g < t->nblock; g++) p.wofs[g + 1] = p.wofs[g] + t->blk[g].s;
+  p.seg_list.clear();
+  p.multi_list.clear();
+  p.part_ofs.assign(t->nblock + 1, 0);
+  long long pr = 0;
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    const int span = 1 << (h - b.level);
+    const int q0 = (b.j ^ 1) * span, q1 = b.j * span;  // source nodes (either mode)
+    const int nch = std::max(p.leaf_chunk0[q0 + span] - p.leaf_chunk0[q0],
+                             p.leaf_chunk0[q1 + span] - p.leaf_chunk0[q1]);
+    const int nseg = (nch + REDUCE_SEG - 1) / REDUCE_SEG;
+    p.part_ofs[g] = pr;
+    if (nseg > 1 && b.s > 0) {
+      p.multi_list.push_back(g);
+      for (int sgi = 0; sgi < nseg; sgi++) p.seg_list.push_back(make_int2(g, sgi));
+      pr += (long long)nseg * b.s;
+    }
+  }
+  p.part_ofs[t->nblock] = pr;
+  std::vector<int4> ch(p.nchunk);
+  for (int k = 0; k < p.nchunk; k++) ch[k] = make_int4(p.ch_leaf[k], p.ch_r0[k], p.ch_rows[k], 0);
 
   HM_OK(upload_vec(&t->d_leaf_off, t->leaf_off, s));
   HM_OK(upload_vec(&t->d_diag_off, t->diag_off, s));
   HM_OK(upload_vec(&t->d_blk, t->blk, s));
-  for (int tr = 0; tr < 2; tr++) {
-    HM_OK(upload_vec(&p.d_pofs[tr], p.pofs[tr], s));
-    HM_OK(upload_vec(&p.d_proff[tr], p.proff[tr], s));
-  }
+  HM_OK(upload_vec(&p.d_ch, ch, s));
+  HM_OK(upload_vec(&p.d_seg_list, p.seg_list, s));
+  HM_OK(upload_vec(&p.d_multi_list, p.multi_list, s));
+  HM_OK(upload_vec(&p.d_part_ofs, p.part_ofs, s));
+  HM_OK(upload_vec(&p.d_leaf_chunk0, p.leaf_chunk0, s));
+  for (int tr = 0; tr < 2; tr++) HM_OK(upload_vec(&p.d_pofs[tr], p.pofs[tr], s));
   HM_OK(upload_vec(&p.d_wofs, p.wofs, s));
   HM_HIP(hipStreamSynchronize(s));
   return SUCCESS;
@@ -105,8 +150,8 @@ int tree_alloc_pools(hm_dtree *t) {
 
 int ensure_scratch(hm_dtree *t, int R) {
   const ProductPlan &p = t->plan;
-  long long need_p = std::max(p.pofs[0][t->nleaf], p.pofs[1][t->nleaf]);
-  long long need = (need_p + p.wofs[t->nblock]) * (long long)R + 64;
+  long long need_p = std::max(p.pofs[0].back(), p.pofs[1].back());
+  long long need = (need_p + p.wofs[t->nblock] + p.part_ofs[t->nblock]) * (long long)R + 64;
   if ((size_t)need <= t->scratch_elems) return SUCCESS;
   dfree(t->d_scratch);
   t->d_scratch = nullptr;

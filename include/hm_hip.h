@@ -98,6 +98,10 @@ HM_HIP_API int hm_hip_matmat(hm_dtree *tree, int trans, int R, const double *d_X
                   long x_rs, long x_cs, double *d_Y, long y_rs, long y_cs,
                   void *stream);
 
+/* Apply-kernel tile variant the autotuner picked for this tree, transpose
+ * mode and RHS width (-1 before the first product of that kind). */
+HM_HIP_API int hm_hip_matmat_variant(const hm_dtree *tree, int trans, int R);
+
 /* ---- compression -------------------------------------------------------
  * Dense -> HODLR on the device.  d_A is m x m column-major with leading
  * dimension lda and is not modified.  `ms` (host, 2^height ints) gives
