@@ -7,12 +7,14 @@
 // (Halko, Martinsson & Tropp 2011, Alg. 4.2 and Lemma 4.1):
 //
 //   1. Y = A [Omega | Omega_t]        Gaussian sketch, ell + ntest columns
-//   2. Y[:, :ell] = Q R               Householder QR (LAPACK dlarfg signs)
-//   3. err = 10 sqrt(2/pi) max |(I - Q Q^T) A w_t|   for the test columns;
-//      accept when err <= accept * thr * sigma0 (relative rule) or
-//      <= accept * thr (absolute rule); otherwise ell doubles
-//   4. B^T = A^T Q = Q2 R2            second pass over A, Householder QR
-//   5. R2^T = W S Z^T                 one-sided Jacobi SVD (ell x ell)
+//   2. Y = Q R                        TSQR (hm_tsqr.hip); Q keeps the first
+//                                     ell columns, and the trailing block of
+//                                     R gives |(I - Q Q^T) A w_t| exactly
+//   3. B^T = A^T Q = Q2 R2            second pass over A, TSQR
+//   4. R2^T = W S Z^T                 one-sided Jacobi SVD (ell x ell)
+//   5. err = 10 sqrt(2/pi) max_t |(I - Q Q^T) A w_t|; accept when
+//      err <= accept * thr * sigma_0 (relative rule; sigma_0 from step 4)
+//      or <= accept * thr (absolute rule); otherwise ell doubles
 //   6. U = Q (R2^T Z)[:, :k] = Q W S,  V = Q2 Z[:, :k]
 //
 // When ell reaches min(m, n) the sketch spans the whole range of A and the
@@ -22,13 +24,14 @@
 // value lies within 0.5% above the threshold.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
+#include "hm_tsqr.h"
 #include "hm_tsvd.h"
 
 namespace hm {
 
-static constexpr int QR_NT = 512;
 static constexpr int MAX_ELL = 1024;
 
 // ------------------------------------------------------------ RNG
@@ -77,28 +80,20 @@ __device__ __forceinline__ int own_from(int tid, int r, int NT) {
 
 // --------------------------------------------------- kernels
 struct TsvdBufs {
-  double *Y, *Om, *onorm, *Q, *Bt, *Q2, *M, *Vj, *G, *Z, *Sig;
+  double *Y, *Om, *Q, *Bt, *Q2, *M, *Vj, *G, *Z, *Sig;
 };
 
 __global__ __launch_bounds__(256) void k_gen_omega(TsvdBlock *blocks,
                                                    const int2 *work,
                                                    TsvdBufs bf,
                                                    unsigned long long seed) {
-  __shared__ double red[4], out[1];
   const int2 w = work[blockIdx.x];
   const TsvdBlock &B = blocks[w.x];
   const int j = w.y;
   const int n = B.n;
   const unsigned long long key = mix64(seed ^ mix64(B.uid * 0x100000001b3ull + (unsigned long long)j));
   double *col = bf.Om + B.ooff + (long long)j * n;
-  double p[1] = {0.0};
-  for (int i = threadIdx.x; i < n; i += 256) {
-    double g = gauss(key, i);
-    col[i] = g;
-    p[0] += g * g;
-  }
-  block_sum<1, 256>(p, red, out);
-  if (threadIdx.x == 0) bf.onorm[B.noff + j] = sqrt(out[0]);
+  for (int i = threadIdx.x; i < n; i += 256) col[i] = gauss(key, i);
 }
 
 // Y = A Omega, tiles of 128 rows x 32 sketch columns.
@@ -153,177 +148,12 @@ __global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
   });
 }
 
-// Householder QR of a rows x nc panel (in place), explicit Q, and either
-// the posterior residual test (mode 0, sketch) or M = R^T (mode 1).
-__global__ __launch_bounds__(QR_NT) void k_qr(TsvdBlock *blocks,
-                                              const int *list, int mode,
-                                              TsvdBufs bf, double accept) {
-  __shared__ double red[8 * (QR_NT / 64)], out[8];
-  __shared__ double tau_s[MAX_ELL];
-  __shared__ double cvec[MAX_ELL];
-  TsvdBlock &B = blocks[list[blockIdx.x]];
-  const int tid = threadIdx.x;
-  const int nc = B.ell;
-  int rows, nt;
-  double *Y, *Q;
-  if (mode == 0) {
-    rows = B.m;
-    nt = B.ntest;
-    Y = bf.Y + B.yoff;
-    Q = bf.Q + B.qoff;
-  } else {
-    rows = B.n;
-    nt = 0;
-    Y = bf.Bt + B.boff;
-    Q = bf.Q2 + B.boff;
-  }
-  const long long ld = rows;
-  double sigma_lb = 0.0;
-  if (mode == 0 && nt > 0) {
-    const int L = nc + nt;
-    for (int jb = 0; jb < L; jb += 8) {
-      const int nv = min(8, L - jb);
-      double p[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) p[v] = 0.0;
-      for (int i = tid; i < rows; i += QR_NT) {
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) {
-            double y = Y[i + (long long)(jb + v) * ld];
-            p[v] += y * y;
-          }
-      }
-      block_sum<8, QR_NT>(p, red, out);
-      for (int v = 0; v < nv; v++) {
-        double on = bf.onorm[B.noff + jb + v];
-        if (on > 0.0) sigma_lb = fmax(sigma_lb, sqrt(out[v]) / on);
-      }
-    }
-  }
-  // factorization
-  for (int k = 0; k < nc; k++) {
-    double p[1] = {0.0};
-    for (int i = own_from(tid, k + 1, QR_NT); i < rows; i += QR_NT) {
-      double x = Y[i + (long long)k * ld];
-      p[0] += x * x;
-    }
-    block_sum<1, QR_NT>(p, red, out);
-    const double xn2 = out[0];
-    const double alpha = Y[k + (long long)k * ld];
-    double tau = 0.0, beta = alpha, scal = 1.0;
-    if (xn2 > 0.0) {
-      double h = hypot(alpha, sqrt(xn2));
-      beta = (alpha >= 0.0) ? -h : h;
-      tau = (beta - alpha) / beta;
-      scal = 1.0 / (alpha - beta);
-    }
-    __syncthreads();
-    for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
-      if (i == k) Y[i + (long long)k * ld] = beta;
-      else if (tau != 0.0) Y[i + (long long)k * ld] *= scal;
-    }
-    if (tid == 0) tau_s[k] = tau;
-    if (tau == 0.0) continue;
-    for (int jb = k + 1; jb < nc; jb += 8) {
-      const int nv = min(8, nc - jb);
-      double q[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) q[v] = 0.0;
-      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) q[v] += vi * Y[i + (long long)(jb + v) * ld];
-      }
-      block_sum<8, QR_NT>(q, red, out);
-      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) Y[i + (long long)(jb + v) * ld] -= tau * out[v] * vi;
-      }
-    }
-  }
-  __syncthreads();
-  if (mode == 1) {
-    double *M = bf.M + B.moff;
-    for (int e = tid; e < nc * nc; e += QR_NT) {
-      const int j = e % nc, i = e / nc;  // M[j + i*nc] = R[i][j]
-      M[j + (long long)i * nc] = (i <= j) ? Y[i + (long long)j * ld] : 0.0;
-    }
-  }
-  // explicit Q = H_0 ... H_{nc-1} [I; 0]
-  for (int i = tid; i < rows; i += QR_NT)
-    for (int j = 0; j < nc; j++) Q[i + (long long)j * ld] = (i == j) ? 1.0 : 0.0;
-  __syncthreads();
-  for (int k = nc - 1; k >= 0; k--) {
-    const double tau = tau_s[k];
-    if (tau == 0.0) continue;
-    for (int jb = k; jb < nc; jb += 8) {
-      const int nv = min(8, nc - jb);
-      double q[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) q[v] = 0.0;
-      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) q[v] += vi * Q[i + (long long)(jb + v) * ld];
-      }
-      block_sum<8, QR_NT>(q, red, out);
-      for (int i = own_from(tid, k, QR_NT); i < rows; i += QR_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + (long long)k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) Q[i + (long long)(jb + v) * ld] -= tau * out[v] * vi;
-      }
-    }
-  }
-  __syncthreads();
-  if (mode == 1) return;
-  // posterior test on the untouched test columns
-  double errmax = 0.0;
-  for (int t = 0; t < nt; t++) {
-    const double *yt = Y + (long long)(nc + t) * ld;
-    for (int jb = 0; jb < nc; jb += 8) {
-      const int nv = min(8, nc - jb);
-      double q[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) q[v] = 0.0;
-      for (int i = tid; i < rows; i += QR_NT) {
-        const double y = yt[i];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) q[v] += Q[i + (long long)(jb + v) * ld] * y;
-      }
-      block_sum<8, QR_NT>(q, red, out);
-      if (tid < nv) cvec[jb + tid] = out[tid];
-    }
-    __syncthreads();
-    double p[1] = {0.0};
-    for (int i = tid; i < rows; i += QR_NT) {
-      double r = yt[i];
-      for (int e = 0; e < nc; e++) r -= Q[i + (long long)e * ld] * cvec[e];
-      p[0] += r * r;
-    }
-    block_sum<1, QR_NT>(p, red, out);
-    errmax = fmax(errmax, sqrt(out[0]));
-  }
-  if (tid == 0) {
-    const double est = 10.0 * 0.7978845608028654 * errmax;
-    const double scale = (B.rule == RULE_RELATIVE) ? sigma_lb : 1.0;
-    B.err = est;
-    B.sigma0 = sigma_lb;
-    B.status = (nt == 0) || (est <= accept * B.thr * scale) ? 1 : 0;
-  }
-}
-
 // One-sided Jacobi SVD of M (nc x nc); V accumulates the rotations.
 // Outputs G = (M V)[:, perm[:k]] (U carries sigma), Z = V[:, perm[:k]],
 // sorted singular values and the rank k by the block's rule.
 __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
-                                                const int *list, TsvdBufs bf) {
+                                                const int *list, TsvdBufs bf,
+                                                double accept) {
   __shared__ double sig[MAX_ELL];
   __shared__ int perm[MAX_ELL];
   __shared__ int rotated;
@@ -412,6 +242,17 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
     B.k = cut;
     B.sigma0 = s0;
     kk = cut;
+    // posterior test (B.err was set by the sketch's QR)
+    bool ok = true;
+    if (B.ntest > 0) {
+      const double est = 10.0 * 0.7978845608028654 * B.err;
+      const double bound = accept * B.thr * (B.rule == RULE_RELATIVE ? s0 : 1.0);
+      const bool tiny = (B.rule == RULE_RELATIVE) && !(s0 > B.thr);
+      ok = tiny || est <= bound;
+      if (ok && !tiny && cut == nc) ok = false;  // sketch saturated: widen
+      B.err = est;
+    }
+    B.status = ok ? 1 : 0;
   }
   __syncthreads();
   double *S = bf.Sig + B.soff;
@@ -457,20 +298,19 @@ __global__ __launch_bounds__(256) void k_copy_list(const CopyItem *items) {
     it.dst[e] = it.src[e];
 }
 
-// M = R_U R_V^T for recompression: R_U is the upper triangle left in the
-// U panel (Y slot, ld m) and R_V the one left in the V panel (Bt slot,
-// ld n) by k_qr.
+// M = R_U R_V^T for recompression; R_U at RU + moff, R_V at RV + moff
+// (both s x s upper triangular, ld s) from the two QRs.
 __global__ __launch_bounds__(256) void k_rmul(TsvdBlock *blocks, const int *list,
-                                              TsvdBufs bf) {
+                                              TsvdBufs bf, const double *RU,
+                                              const double *RV) {
   const TsvdBlock B = blocks[list[blockIdx.x]];
   const int s = B.ell;
-  const double *RU = bf.Y + B.yoff, *RV = bf.Bt + B.boff;
+  const double *ru = RU + B.moff, *rv = RV + B.moff;
   double *M = bf.M + B.moff;
   for (int e = threadIdx.x; e < s * s; e += 256) {
     const int i = e % s, j = e / s;
     double acc = 0.0;
-    for (int t = max(i, j); t < s; t++)
-      acc += RU[i + (long long)t * B.m] * RV[j + (long long)t * B.n];
+    for (int t = max(i, j); t < s; t++) acc += ru[i + (long long)t * s] * rv[j + (long long)t * s];
     M[i + (long long)j * s] = acc;
   }
 }
@@ -546,7 +386,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       continue;
     }
     b.ell = std::max(1, std::min(std::min(b.ell, mn), cap));
-    b.ntest = (b.ell >= mn) ? 0 : opt.n_test;
+    b.ntest = (b.ell >= mn) ? 0 : std::min(opt.n_test, b.m - b.ell);
     active.push_back(i);
   }
   if (sigma) sigma->assign((size_t)nb * sigma_cap, 0.0);
@@ -560,24 +400,23 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     st.rounds++;
     Scoped mem;
     // offsets for this round
-    long long ytot = 0, otot = 0, ntot = 0, qtot = 0, btot = 0, mtot = 0, stot = 0;
+    long long ytot = 0, otot = 0, qtot = 0, btot = 0, mtot = 0, stot = 0;
     for (int i : active) {
       TsvdBlock &b = blocks[i];
       const int L = b.ell + b.ntest;
       b.yoff = ytot; ytot += (long long)b.m * L;
       b.ooff = otot; otot += (long long)b.n * L;
-      b.noff = ntot; ntot += L;
       b.qoff = qtot; qtot += (long long)b.m * b.ell;
       b.boff = btot; btot += (long long)b.n * b.ell;
       b.moff = mtot; mtot += (long long)b.ell * b.ell;
       b.soff = stot; stot += b.ell;
       b.status = 0;
+      b.err = 0.0;
       st.max_sketch = std::max(st.max_sketch, b.ell);
     }
     TsvdBufs bf;
     HM_OK(dalloc(&bf.Y, ytot)); mem.keep(bf.Y);
     HM_OK(dalloc(&bf.Om, otot)); mem.keep(bf.Om);
-    HM_OK(dalloc(&bf.onorm, ntot)); mem.keep(bf.onorm);
     HM_OK(dalloc(&bf.Q, qtot)); mem.keep(bf.Q);
     HM_OK(dalloc(&bf.Bt, btot)); mem.keep(bf.Bt);
     HM_OK(dalloc(&bf.Q2, btot)); mem.keep(bf.Q2);
@@ -586,7 +425,6 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     HM_OK(dalloc(&bf.G, mtot)); mem.keep(bf.G);
     HM_OK(dalloc(&bf.Z, mtot)); mem.keep(bf.Z);
     HM_OK(dalloc(&bf.Sig, stot)); mem.keep(bf.Sig);
-    // device copy of all blocks (indices are global block numbers)
     TsvdBlock *d_blocks = nullptr;
     HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
     // work lists, biggest K first for balance
@@ -595,7 +433,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       return (long long)blocks[x].n > (long long)blocks[y].n;
     });
     std::vector<int2> wo;
-    std::vector<int4> ws;
+    std::vector<int4> ws, wp;
     for (int i : order) {
       const TsvdBlock &b = blocks[i];
       const int L = b.ell + b.ntest;
@@ -605,36 +443,93 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       for (int a = 0; a < mt; a++)
         for (int c = 0; c < ntl; c++) ws.push_back(make_int4(i, a, c, 0));
     }
+    std::vector<int> order_m = order;
+    std::sort(order_m.begin(), order_m.end(), [&](int x, int y) {
+      return (long long)blocks[x].m > (long long)blocks[y].m;
+    });
+    for (int i : order_m) {
+      const TsvdBlock &b = blocks[i];
+      const int mt = (b.n + SketchTile::TM - 1) / SketchTile::TM;
+      const int ntl = (b.ell + SketchTile::TN - 1) / SketchTile::TN;
+      for (int a = 0; a < mt; a++)
+        for (int c = 0; c < ntl; c++) wp.push_back(make_int4(i, a, c, 0));
+    }
     int2 *d_wo;
-    int4 *d_ws;
+    int4 *d_ws, *d_wp;
     int *d_list;
     HM_OK(to_device(&d_wo, wo, s)); mem.keep(d_wo);
     HM_OK(to_device(&d_ws, ws, s)); mem.keep(d_ws);
+    HM_OK(to_device(&d_wp, wp, s)); mem.keep(d_wp);
     HM_OK(to_device(&d_list, order, s)); mem.keep(d_list);
+    Timed t0 = timing_begin("tsvd_omega", s);
+    hipLaunchKernelGGL(k_gen_omega, dim3((unsigned)wo.size()), dim3(256), 0, s, d_blocks,
+                       d_wo, bf, opt.seed);
+    timing_end(t0, s);
+    HM_LAUNCH_CHECK();
+    Timed t1 = timing_begin("tsvd_sketch", s);
+    hipLaunchKernelGGL(k_sketch, dim3((unsigned)ws.size()), dim3(256), 0, s, d_blocks, d_ws,
+                       bf);
+    timing_end(t1, s);
+    HM_LAUNCH_CHECK();
     {
-      Timed t0 = timing_begin("tsvd_omega", s);
-      hipLaunchKernelGGL(k_gen_omega, dim3((unsigned)wo.size()), dim3(256), 0, s,
-                         d_blocks, d_wo, bf, opt.seed);
-      timing_end(t0, s);
-      HM_LAUNCH_CHECK();
-      Timed t1 = timing_begin("tsvd_sketch", s);
-      hipLaunchKernelGGL(k_sketch, dim3((unsigned)ws.size()), dim3(256), 0, s,
-                         d_blocks, d_ws, bf);
-      timing_end(t1, s);
-      HM_LAUNCH_CHECK();
-      Timed t2 = timing_begin("tsvd_qr", s);
-      hipLaunchKernelGGL(k_qr, dim3((unsigned)order.size()), dim3(QR_NT), 0, s,
-                         d_blocks, d_list, 0, bf, opt.accept);
-      timing_end(t2, s);
-      HM_LAUNCH_CHECK();
+      std::vector<TsqrPanel> pan;
+      for (int i : order) {
+        const TsvdBlock &b = blocks[i];
+        TsqrPanel P;
+        memset(&P, 0, sizeof(P));
+        P.A = bf.Y + b.yoff;
+        P.lda = b.m;
+        P.m = b.m;
+        P.ncol = b.ell + b.ntest;
+        P.nq = b.ell;
+        P.Q = bf.Q + b.qoff;
+        P.ldq = b.m;
+        P.est = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
+                                           offsetof(TsvdBlock, err));
+        P.ell = b.ell;
+        pan.push_back(P);
+      }
+      HM_OK(tsqr_run(pan, s));
     }
-    HM_HIP(hipMemcpyAsync(blocks.data(), d_blocks, nb * sizeof(TsvdBlock),
+    Timed t3 = timing_begin("tsvd_project", s);
+    hipLaunchKernelGGL(k_project, dim3((unsigned)wp.size()), dim3(256), 0, s, d_blocks, d_wp,
+                       bf);
+    timing_end(t3, s);
+    HM_LAUNCH_CHECK();
+    {
+      std::vector<TsqrPanel> pan;
+      for (int i : order) {
+        const TsvdBlock &b = blocks[i];
+        TsqrPanel P;
+        memset(&P, 0, sizeof(P));
+        P.A = bf.Bt + b.boff;
+        P.lda = b.n;
+        P.m = b.n;
+        P.ncol = b.ell;
+        P.nq = b.ell;
+        P.Q = bf.Q2 + b.boff;
+        P.ldq = b.n;
+        P.Mt = bf.M + b.moff;
+        pan.push_back(P);
+      }
+      HM_OK(tsqr_run(pan, s));
+    }
+    Timed t5 = timing_begin("tsvd_jacobi", s);
+    hipLaunchKernelGGL(k_jacobi, dim3((unsigned)order.size()), dim3(256), 0, s, d_blocks,
+                       d_list, bf, opt.accept);
+    timing_end(t5, s);
+    HM_LAUNCH_CHECK();
+    std::vector<TsvdBlock> hb(nb);
+    HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
                           hipMemcpyDeviceToHost, s));
     HM_HIP(hipStreamSynchronize(s));
     std::vector<int> done, retry;
     for (int i : order) {
       TsvdBlock &b = blocks[i];
-      const int mn = std::min(b.m, b.n);
+      b.k = hb[i].k;
+      b.sigma0 = hb[i].sigma0;
+      b.err = hb[i].err;
+      b.status = hb[i].status;
       if (b.status || b.ell >= cap) {
         if (!b.status) st.capped++;
         if (b.ntest == 0) st.exact_blocks++;
@@ -642,45 +537,11 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       } else {
         retry.push_back(i);
       }
-      (void)mn;
     }
     st.retried += (int)retry.size();
     if (!done.empty()) {
-      std::vector<int4> wp;
-      for (int i : done) {
-        const TsvdBlock &b = blocks[i];
-        const int mt = (b.n + SketchTile::TM - 1) / SketchTile::TM;
-        const int ntl = (b.ell + SketchTile::TN - 1) / SketchTile::TN;
-        for (int a = 0; a < mt; a++)
-          for (int c = 0; c < ntl; c++) wp.push_back(make_int4(i, a, c, 0));
-      }
-      int4 *d_wp;
-      int *d_done;
-      HM_OK(to_device(&d_wp, wp, s)); mem.keep(d_wp);
-      HM_OK(to_device(&d_done, done, s)); mem.keep(d_done);
-      Timed t3 = timing_begin("tsvd_project", s);
-      hipLaunchKernelGGL(k_project, dim3((unsigned)wp.size()), dim3(256), 0, s,
-                         d_blocks, d_wp, bf);
-      timing_end(t3, s);
-      HM_LAUNCH_CHECK();
-      Timed t4 = timing_begin("tsvd_qr2", s);
-      hipLaunchKernelGGL(k_qr, dim3((unsigned)done.size()), dim3(QR_NT), 0, s,
-                         d_blocks, d_done, 1, bf, opt.accept);
-      timing_end(t4, s);
-      HM_LAUNCH_CHECK();
-      Timed t5 = timing_begin("tsvd_jacobi", s);
-      hipLaunchKernelGGL(k_jacobi, dim3((unsigned)done.size()), dim3(256), 0, s,
-                         d_blocks, d_done, bf);
-      timing_end(t5, s);
-      HM_LAUNCH_CHECK();
-      std::vector<TsvdBlock> hb(nb);
-      HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
-                            hipMemcpyDeviceToHost, s));
-      HM_HIP(hipStreamSynchronize(s));
       long long ut = 0, vt = 0;
       for (int i : done) {
-        blocks[i].k = hb[i].k;
-        blocks[i].sigma0 = hb[i].sigma0;
         ut += (long long)blocks[i].m * blocks[i].k;
         vt += (long long)blocks[i].n * blocks[i].k;
       }
@@ -729,7 +590,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       TsvdBlock &b = blocks[i];
       const int mn = std::min(b.m, b.n);
       b.ell = std::min(std::min(2 * b.ell, mn), cap);
-      b.ntest = (b.ell >= mn) ? 0 : opt.n_test;
+      b.ntest = (b.ell >= mn) ? 0 : std::min(opt.n_test, b.m - b.ell);
     }
     active = retry;
   }
@@ -819,7 +680,9 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
   HM_OK(dalloc(&bf.G, mtot)); mem.keep(bf.G);
   HM_OK(dalloc(&bf.Z, mtot)); mem.keep(bf.Z);
   HM_OK(dalloc(&bf.Sig, stot)); mem.keep(bf.Sig);
-  bf.onorm = nullptr;
+  double *RU, *RV;
+  HM_OK(dalloc(&RU, mtot)); mem.keep(RU);
+  HM_OK(dalloc(&RV, mtot)); mem.keep(RV);
   bf.Om = nullptr;
   std::vector<CopyItem> in;
   for (int i = 0; i < nn; i++) {
@@ -837,10 +700,23 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
   HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
   HM_OK(to_device(&d_list, list, s)); mem.keep(d_list);
   Timed t0 = timing_begin("recompress_core", s);
-  hipLaunchKernelGGL(k_qr, dim3(nn), dim3(QR_NT), 0, s, d_blocks, d_list, 0, bf, 0.0);
-  hipLaunchKernelGGL(k_qr, dim3(nn), dim3(QR_NT), 0, s, d_blocks, d_list, 1, bf, 0.0);
-  hipLaunchKernelGGL(k_rmul, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf);
-  hipLaunchKernelGGL(k_jacobi, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf);
+  {
+    std::vector<TsqrPanel> pan;
+    for (int i = 0; i < nn; i++) {
+      const TsvdBlock &b = blocks[i];
+      TsqrPanel P;
+      memset(&P, 0, sizeof(P));
+      P.A = bf.Y + b.yoff;  P.lda = b.m;  P.m = b.m;  P.ncol = b.ell;  P.nq = b.ell;
+      P.Q = bf.Q + b.qoff;  P.ldq = b.m;  P.R = RU + b.moff;  P.ldr = b.ell;
+      pan.push_back(P);
+      P.A = bf.Bt + b.boff; P.lda = b.n;  P.m = b.n;
+      P.Q = bf.Q2 + b.boff; P.ldq = b.n;  P.R = RV + b.moff;
+      pan.push_back(P);
+    }
+    HM_OK(tsqr_run(pan, s));
+  }
+  hipLaunchKernelGGL(k_rmul, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf, RU, RV);
+  hipLaunchKernelGGL(k_jacobi, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf, 0.0);
   timing_end(t0, s);
   HM_LAUNCH_CHECK();
   HM_HIP(hipMemcpyAsync(blocks.data(), d_blocks, nn * sizeof(TsvdBlock),
