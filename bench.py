@@ -26,6 +26,12 @@ sys.path.insert(0, ROOT)
 PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 PEAK_FP64 = 78.6e12        # MI355X dense FP64 (matrix), flop/s
 
+# HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by the counts of a
+# 256 MiB device copy (fetch x2.000, write x1.000); profiles/ holds the
+# commands and the per-kernel table.
+MEASURED_TRAFFIC = {("exp1d_64k", "k_mm_apply"): 206633862}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -47,6 +53,9 @@ def main():
     ap.add_argument("--workload", default="exp1d_64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 10 device-to-device copies of this many MiB "
+                         "(the known-size kernel that calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -61,6 +70,16 @@ def main():
     import numpy as np
 
     from hmat_lib_amd import capi, device as D, workloads as WL
+
+    if args.copy_probe_mib:
+        D.init(0)
+        nbytes = args.copy_probe_mib << 20
+        src, dst = D.DeviceArray((nbytes // 8,), zero=True), D.DeviceArray((nbytes // 8,))
+        for _ in range(10):
+            capi.check(capi.hip().hm_hip_memcpy_d2d(dst.ptr, src.ptr, nbytes, None), "copy")
+        D.synchronize()
+        log(f"copy probe: 10 x {nbytes} bytes device to device")
+        return
 
     w = WL.WORKLOADS[args.workload]
     D.init(0)
@@ -150,7 +169,7 @@ def main():
         roof = {"bound": "hbm", "achieved": bytes_alg / k_avg / 1e9,
                 "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = MEASURED_TRAFFIC.get((w.name, "k_mm_apply"))
     roof["kernel"] = "k_mm_apply"
     roof["kernel_avg_us"] = k_avg * 1e6
     roof["algorithmic_bytes"] = bytes_alg

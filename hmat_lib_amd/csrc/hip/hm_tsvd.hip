@@ -169,6 +169,9 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
   const int np = nc + (nc & 1);
   const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
   for (int sweep = 0; sweep < 60 && nc > 1; sweep++) {
+    // every thread has read the previous sweep's flag before it is cleared,
+    // so all waves take the same exit decision
+    __syncthreads();
     if (threadIdx.x == 0) rotated = 0;
     __syncthreads();
     for (int r = 0; r < np - 1; r++) {
@@ -217,11 +220,13 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
     if (lane == 0) sig[j] = sqrt(ss);
   }
   __syncthreads();
+  // descending order, ties by index; a NaN sorts last, so perm is always a
+  // permutation of 0..nc-1
   for (int j = threadIdx.x; j < nc; j += 256) {
     int rank = 0;
-    const double sj = sig[j];
+    const double sj = sig[j] == sig[j] ? sig[j] : -1.0;
     for (int i = 0; i < nc; i++) {
-      const double si = sig[i];
+      const double si = sig[i] == sig[i] ? sig[i] : -1.0;
       rank += (si > sj) || (si == sj && i < j);
     }
     perm[rank] = j;
