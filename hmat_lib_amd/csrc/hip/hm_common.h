@@ -73,6 +73,13 @@ struct BlockMeta {
 static constexpr int CHUNK_ROWS = 64;
 static constexpr int REDUCE_SEG = 16;
 
+// Block path (trees whose per-chunk rank sum is large): W_b = F_in^T X_src
+// is computed per block as a split-K GEMM over K segments of WSEG_ROWS
+// rows; blocks with more than one segment write s x R partials that a
+// second kernel sums in segment order.
+static constexpr int WSEG_ROWS = 4096;
+static constexpr int WTILE = 64;  // rank rows per work item
+
 struct ProductPlan {
   int nchunk = 0;
   std::vector<int> ch_leaf, ch_r0, ch_rows;  // per chunk
@@ -96,6 +103,15 @@ struct ProductPlan {
   long long *d_pofs[2] = {nullptr, nullptr};
   long long *d_wofs = nullptr;
   int apply_variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};  // autotuned, by R bucket
+  // block path, per transpose mode: items (block, rank tile, K segment, rows
+  // in segment), partial offsets per block (x R), blocks with > 1 segment
+  std::vector<int4> witems[2];
+  std::vector<long long> wpofs[2];
+  std::vector<int> wmulti[2];
+  int wmulti_smax[2] = {0, 0};
+  int4 *d_witems[2] = {nullptr, nullptr};
+  long long *d_wpofs[2] = {nullptr, nullptr};
+  int *d_wmulti[2] = {nullptr, nullptr};
 };
 
 }  // namespace hm

@@ -286,6 +286,64 @@ __global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
   });
 }
 
+// ------------------------------------------------------ block W path
+// One work item = (block g, rank tile st, K segment sg, rows in segment):
+// C(st rows x 64 RHS) = F_in[seg rows, st cols]^T X[src rows of seg, :].
+// F_in is V (n x s) for H X and U (m x s) for H^T X, so the A operand is
+// K-fast; X may be column- or row-major (B_NFAST).
+typedef MfmaTile<2, 2, 2, 2, 16> WTileT;  // 64 x 64
+static_assert(WTileT::TM == WTILE, "rank tile");
+
+template <bool B_NFAST>
+__global__ __launch_bounds__(256) void k_w_gemm(ProdArgs a, const int4 *items,
+                                                const long long *wpofs) {
+  __shared__ double lds[WTileT::LDS_DOUBLES];
+  const int4 it = items[blockIdx.x];
+  const BlockMeta b = a.blk[it.x];
+  const int s = b.s;
+  const int K = a.trans ? b.m : b.n;
+  const int src = a.trans ? b.r0 : b.c0;
+  const double *F = a.trans ? a.U + b.uoff : a.V + b.voff;
+  const int k0 = it.z * WSEG_ROWS, kl = it.w;
+  const int i0 = it.y * WTileT::TM, n0 = blockIdx.y * WTileT::TN;
+  const double *Fk = F + k0;
+  const double *Xk = a.X + (long long)(src + k0) * a.x_rs;
+  auto la = [&](int i, int k) -> double {
+    return (i0 + i < s && k < kl) ? Fk[k + (long long)(i0 + i) * K] : 0.0;
+  };
+  auto lb = [&](int k, int j) -> double {
+    return (k < kl && n0 + j < a.R) ? Xk[(long long)k * a.x_rs + (long long)(n0 + j) * a.x_cs]
+                                    : 0.0;
+  };
+  WTileT tile;
+  tile.zero();
+  tile.template run<false, B_NFAST>(lds, kl, la, lb);
+  const int nseg = (K + WSEG_ROWS - 1) / WSEG_ROWS;
+  double *out = nseg > 1 ? a.P + (wpofs[it.x] + (long long)it.z * s) * a.R
+                         : a.W + a.wofs[it.x] * a.R;
+  tile.store([&](int i, int j, double v) {
+    if (i0 + i < s && n0 + j < a.R) out[(i0 + i) + (long long)(n0 + j) * s] = v;
+  });
+}
+
+// W_b = sum over K segments of the partials (fixed order).
+__global__ __launch_bounds__(256) void k_w_reduce(ProdArgs a, const int *multi,
+                                                  const long long *wpofs) {
+  const int g = multi[blockIdx.x];
+  const BlockMeta b = a.blk[g];
+  const int K = a.trans ? b.m : b.n;
+  const int nseg = (K + WSEG_ROWS - 1) / WSEG_ROWS;
+  const long long E = (long long)b.s * a.R;
+  const double *base = a.P + wpofs[g] * a.R;
+  double *out = a.W + a.wofs[g] * a.R;
+  for (long long e = (long long)blockIdx.y * 256 + threadIdx.x; e < E;
+       e += (long long)gridDim.y * 256) {
+    double s0 = 0.0;
+    for (int sg = 0; sg < nseg; sg++) s0 += base[(long long)sg * E + e];
+    out[e] = s0;
+  }
+}
+
 // Work order of the apply kernel: leaves are dealt to the 8 XCDs round
 // robin (workgroup b runs on XCD b % 8), and all row/column tiles of one
 // leaf stay on its XCD so that X_c and the W_b are shared through one L2.
@@ -423,7 +481,8 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   a.seg_list = t->plan.d_seg_list;
   a.multi_list = t->plan.d_multi_list;
   a.part_ofs = t->plan.d_part_ofs;
-  long long pmax = std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back());
+  long long pmax = std::max(std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back()),
+                            std::max(t->plan.wpofs[0].back(), t->plan.wpofs[1].back()));
   a.P = t->d_scratch;
   a.W = t->d_scratch + pmax * R;
   a.Wp = a.W + t->plan.wofs[t->nblock] * R;
@@ -529,6 +588,42 @@ static int choose_variant(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_
   return slot;
 }
 
+// W path per transpose mode: the chunk path (partials of every level per
+// 64-row chunk, X read once) suits small rank sums; the block path (split-K
+// GEMM per block) keeps the partial buffer small when ranks are large.
+// HM_WPATH=0/1 forces one.
+static int choose_wpath(const hm_dtree *t, int trans) {
+  const char *force = getenv("HM_WPATH");
+  if (force && (force[0] == '0' || force[0] == '1')) return force[0] - '0';
+  return t->plan.s_in_max[trans] > 64 ? 1 : 0;
+}
+
+static int run_block_w(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_t s) {
+  const ProductPlan &p = t->plan;
+  const int tr = a.trans;
+  if (p.witems[tr].empty()) return SUCCESS;
+  dim3 grid((unsigned)p.witems[tr].size(), (unsigned)((a.R + WTileT::TN - 1) / WTileT::TN));
+  Timed tp = timing_begin("w_gemm", s);
+  if (nfast)
+    hipLaunchKernelGGL(k_w_gemm<true>, grid, dim3(256), 0, s, a, p.d_witems[tr],
+                       p.d_wpofs[tr]);
+  else
+    hipLaunchKernelGGL(k_w_gemm<false>, grid, dim3(256), 0, s, a, p.d_witems[tr],
+                       p.d_wpofs[tr]);
+  timing_end(tp, s);
+  HM_LAUNCH_CHECK();
+  if (!p.wmulti[tr].empty()) {
+    const long long E = (long long)p.wmulti_smax[tr] * a.R;
+    const int gy = (int)std::min<long long>(64, (E + 255) / 256);
+    Timed tr_ = timing_begin("w_reduce", s);
+    hipLaunchKernelGGL(k_w_reduce, dim3((unsigned)p.wmulti[tr].size(), gy), dim3(256), 0, s,
+                       a, p.d_wmulti[tr], p.d_wpofs[tr]);
+    timing_end(tr_, s);
+    HM_LAUNCH_CHECK();
+  }
+  return SUCCESS;
+}
+
 static int run_reduce(hm_dtree *t, const ProdArgs &a, hipStream_t s) {
   Timed tr = timing_begin("reduce_w", s);
   if (!t->plan.seg_list.empty())
@@ -571,7 +666,9 @@ int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
   }
   const bool nfast = (x_rs != 1);
   const int smax = t->plan.s_in_max[trans];
-  if (smax > 0 && nchunk > 0) {
+  if (smax > 0 && nchunk > 0 && choose_wpath(t, trans) == 1) {
+    HM_OK(run_block_w(t, a, nfast, s));
+  } else if (smax > 0 && nchunk > 0) {
     const int ntiles = (R + 63) / 64;
     const int mtiles = (smax + 15) / 16;
     dim3 grid((unsigned)(nchunk * mtiles * ntiles));

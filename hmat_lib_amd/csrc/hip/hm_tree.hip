@@ -24,6 +24,11 @@ void tree_free(hm_dtree *t) {
   dfree(t->plan.d_part_ofs);
   dfree(t->plan.d_leaf_chunk0);
   dfree(t->plan.d_wofs);
+  for (int k = 0; k < 2; k++) {
+    dfree(t->plan.d_witems[k]);
+    dfree(t->plan.d_wpofs[k]);
+    dfree(t->plan.d_wmulti[k]);
+  }
   dfree(t->d_scratch);
   delete t;
 }
@@ -119,6 +124,39 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     }
   }
   p.part_ofs[t->nblock] = pr;
+  // block path plan
+  for (int tr = 0; tr < 2; tr++) {
+    p.witems[tr].clear();
+    p.wmulti[tr].clear();
+    p.wmulti_smax[tr] = 0;
+    p.wpofs[tr].assign(t->nblock + 1, 0);
+    long long run = 0;
+    for (int g = 0; g < t->nblock; g++) {
+      const BlockMeta &b = t->blk[g];
+      const int K = tr ? b.m : b.n;
+      const int nseg = (K + WSEG_ROWS - 1) / WSEG_ROWS;
+      p.wpofs[tr][g] = run;
+      if (b.s <= 0) continue;
+      if (K <= 0) {  // empty source node: W_b = 0 (no segments to sum)
+        p.wmulti[tr].push_back(g);
+        p.wmulti_smax[tr] = std::max(p.wmulti_smax[tr], b.s);
+        continue;
+      }
+      for (int sg = 0; sg < nseg; sg++)
+        for (int st = 0; st * WTILE < b.s; st++)
+          p.witems[tr].push_back(
+              make_int4(g, st, sg, std::min(WSEG_ROWS, K - sg * WSEG_ROWS)));
+      if (nseg > 1) {
+        p.wmulti[tr].push_back(g);
+        p.wmulti_smax[tr] = std::max(p.wmulti_smax[tr], b.s);
+        run += (long long)nseg * b.s;
+      }
+    }
+    p.wpofs[tr][t->nblock] = run;
+    // longest K first (load balance)
+    std::stable_sort(p.witems[tr].begin(), p.witems[tr].end(),
+                     [](const int4 &x, const int4 &y) { return x.w > y.w; });
+  }
   std::vector<int4> ch(p.nchunk);
   for (int k = 0; k < p.nchunk; k++) ch[k] = make_int4(p.ch_leaf[k], p.ch_r0[k], p.ch_rows[k], 0);
 
@@ -132,6 +170,11 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
   HM_OK(upload_vec(&p.d_leaf_chunk0, p.leaf_chunk0, s));
   for (int tr = 0; tr < 2; tr++) HM_OK(upload_vec(&p.d_pofs[tr], p.pofs[tr], s));
   HM_OK(upload_vec(&p.d_wofs, p.wofs, s));
+  for (int tr = 0; tr < 2; tr++) {
+    HM_OK(upload_vec(&p.d_witems[tr], p.witems[tr], s));
+    HM_OK(upload_vec(&p.d_wpofs[tr], p.wpofs[tr], s));
+    HM_OK(upload_vec(&p.d_wmulti[tr], p.wmulti[tr], s));
+  }
   HM_HIP(hipStreamSynchronize(s));
   return SUCCESS;
 }
@@ -150,7 +193,8 @@ int tree_alloc_pools(hm_dtree *t) {
 
 int ensure_scratch(hm_dtree *t, int R) {
   const ProductPlan &p = t->plan;
-  long long need_p = std::max(p.pofs[0].back(), p.pofs[1].back());
+  long long need_p = std::max(std::max(p.pofs[0].back(), p.pofs[1].back()),
+                              std::max(p.wpofs[0].back(), p.wpofs[1].back()));
   long long need = (need_p + p.wofs[t->nblock] + p.part_ofs[t->nblock]) * (long long)R + 64;
   if ((size_t)need <= t->scratch_elems) return SUCCESS;
   dfree(t->d_scratch);

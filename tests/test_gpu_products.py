@@ -40,14 +40,25 @@ CASES = [
     ("ragged zeros h2", [20, 1, 1, 0], lambda l, j: 1),
     ("big leaves h2", [300, 129, 77, 256], lambda l, j: 17 + l),
     ("rank 40 h5", [96] * 32, lambda l, j: 40 - 5 * l),
+    # level-1 blocks span 5200 rows: two K segments in the block path
+    ("two segments h2", [2600] * 4, lambda l, j: 70 if l == 1 else 30 + j),
 ]
 
+# W = F_in^T X strategy: chosen by the library ("auto"), or forced to the
+# per-chunk partials ("chunk") or the per-block split-K GEMM ("block").
+WPATHS = {"auto": None, "chunk": "0", "block": "1"}
 
+
+@pytest.mark.parametrize("wpath", list(WPATHS))
 @pytest.mark.parametrize("name,leaves,ranks", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("R", [1, 3, 64, 100])
-def test_products_vs_dense(gpu, name, leaves, ranks, R):
+def test_products_vs_dense(gpu, monkeypatch, name, leaves, ranks, R, wpath):
     from hmat_lib_amd import device as D
-    _say(f"case: products {name} R={R}")
+    if WPATHS[wpath] is None:
+        monkeypatch.delenv("HM_WPATH", raising=False)
+    else:
+        monkeypatch.setenv("HM_WPATH", WPATHS[wpath])
+    _say(f"case: products {name} R={R} wpath={wpath}")
     rng = np.random.default_rng(len(leaves) * 131 + R)
     ft = H.random_flat_tree(rng, leaves, ranks)
     lib = capi.hmat()
