@@ -170,6 +170,8 @@ def _declare_hip_api(lib):
         "hm_hip_recompress_batch": ([c_int, PInt, PInt, PInt, POINTER(V), POINTER(V),
                                      c_double, c_int, PInt, V], c_int),
         "hm_hip_selftest_mfma": ([PDouble], c_int),
+        "hm_hip_audit_qr": ([c_int, POINTER(c_int), POINTER(c_int), c_int, C.c_char_p,
+                             c_int], c_int),
         "hm_hip_matmat_variant": ([V, c_int, c_int], c_int),
         "hm_hip_kernel_timing": ([c_int], c_int),
         "hm_hip_kernel_timing_select": ([C.c_char_p], c_int),

@@ -1,0 +1,112 @@
+// Launch programs: a host-built sequence of kernel launches whose work-item
+// tables live in one device blob, uploaded once and then launched with no
+// host synchronisation in between (internal interface).
+//
+// Builders run twice: first on a counting Arena (device pointers are fake
+// offsets, nothing is launched) to size the scratch, then on a real Arena.
+// Every item that touches memory records the address ranges it reads and
+// writes, so a program can be checked on the host against the buffers it
+// was given before anything runs on the device (Program::audit).
+#pragma once
+
+#include <cstring>
+#include <functional>
+#include <string>
+
+#include "hm_common.h"
+
+namespace hm {
+
+// Scratch carving.  In counting mode take() returns fake, never dereferenced
+// pointers (offsets from a fixed fake base) and only the size is tracked.
+struct Arena {
+  char *base = nullptr;
+  size_t off = 0, cap = 0;
+  bool counting = true;
+  template <class T>
+  T *take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    char *p = (counting ? fake_base() : base) + off;
+    off += count * sizeof(T) + 16;
+    return reinterpret_cast<T *>(p);
+  }
+  static char *fake_base() { return reinterpret_cast<char *>((uintptr_t)1 << 44); }
+};
+
+struct MemRange {
+  const void *lo;
+  size_t bytes;
+};
+
+struct Program {
+  std::vector<unsigned char> blob;
+  std::vector<std::function<void(hipStream_t, const unsigned char *)>> ops;
+  std::vector<std::string> names;
+  unsigned char *d_blob = nullptr;
+  // audit data: address ranges touched by items / buffers they may touch
+  bool record = false;
+  std::vector<std::pair<MemRange, std::string>> touched;
+  std::vector<MemRange> allowed;
+
+  ~Program() { dfree(d_blob); }
+  template <class T>
+  size_t push(const std::vector<T> &v) {
+    size_t off = (blob.size() + 63) & ~(size_t)63;
+    blob.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+  void add(const char *name, std::function<void(hipStream_t, const unsigned char *)> f) {
+    names.push_back(name);
+    ops.push_back(std::move(f));
+  }
+  void touch(const void *lo, size_t bytes, const char *what) {
+    if (record && bytes) touched.push_back({{lo, bytes}, what});
+  }
+  void allow(const void *lo, size_t bytes) { allowed.push_back({lo, bytes}); }
+  // Every touched range inside one allowed range; message names the first
+  // violation.  Returns SUCCESS or INPUT_ERROR.
+  int audit(std::string *msg) const;
+  int upload(hipStream_t s);
+  int run(hipStream_t s);  // launches every op in order (no sync)
+};
+
+// ---------------------------------------------------------- batched GEMM
+// C = alpha op(A) op(B) + beta C, op(A): M x K (A^T when ta), op(B): K x N
+// (B^T when tb), column-major.  beta == 0 never reads C.  Long K with few
+// output tiles is split into segments whose partial products go to arena
+// scratch and are summed in segment order (deterministic).
+struct GemmJob {
+  const double *A, *B;
+  double *C;
+  double *part;
+  long long lda, ldb, ldc;
+  int M, N, K, ksplit;
+  int kseg, pad;
+  double alpha, beta;
+};
+
+int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
+             const char *name);
+
+// dst(rows x cols, ldd) = src(rows x cols, lds), or src^T when trans
+// (then dst is cols x rows); src == NULL fills dst with zeros.
+struct Copy2D {
+  const double *src;
+  double *dst;
+  long long lds, ldd;
+  int rows, cols;
+  int trans, pad;
+};
+int copy2d_add(Program &p, const std::vector<Copy2D> &items);
+
+// *out = max over columns of the 2-norm of a (rows x cols, ld) panel
+struct ColNorm {
+  const double *a;
+  long long ld;
+  int rows, cols;
+  double *out;
+};
+int colnorm_add(Program &p, const std::vector<ColNorm> &items);
+
+}  // namespace hm

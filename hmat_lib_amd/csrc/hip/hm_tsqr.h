@@ -1,7 +1,7 @@
 // Batched tall-skinny QR with explicit Q (internal interface).
 #pragma once
 
-#include "hm_common.h"
+#include "hm_prog.h"
 
 namespace hm {
 
@@ -25,8 +25,30 @@ struct TsqrPanel {
   int ell;
 };
 
-// Factors all panels (one launch per tree level for all panels at once);
-// synchronizes the stream before returning.
-int tsqr_run(const std::vector<TsqrPanel> &panels, hipStream_t s);
+// Appends the factorisation of all panels to a program (one launch per
+// reduction level for all panels at once, then the top-down Q build).
+int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels);
+
+// Builds a program twice (counting, then real), audits the counting pass
+// against `inputs` plus the scratch, runs it and synchronizes the stream.
+int program_exec(const std::function<int(Program &, Arena &)> &build,
+                 const std::vector<MemRange> &inputs, hipStream_t s);
+
+// Householder QR of wide panels: block Gram-Schmidt run twice over panels
+// of `bw` columns (BCGS2), each panel orthonormalised by TSQR.  Y (m x L)
+// is destroyed; Q (m x L) receives the orthonormal factor and, when R is
+// set, R (L x L, upper triangular, ld ldr) the triangular one, Y = Q R.
+// Requires L <= m.
+struct QrJob {
+  double *Y;
+  long long ldy;
+  int m, L;
+  double *Q;
+  long long ldq;
+  double *R;
+  long long ldr;
+};
+int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw = 64);
+
 
 }  // namespace hm

@@ -184,22 +184,16 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down(const QrItem *items) {
   }
 }
 
-int tsqr_run(const std::vector<TsqrPanel> &panels, hipStream_t s) {
+int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
   if (panels.empty()) return SUCCESS;
-  // plan: per panel, levels of chunks
-  struct Level {
-    std::vector<QrItem> up, down;
-  };
-  std::vector<Level> levels;
-  long long scratch = 0;
-  struct Pending {
-    int panel, level, kind;  // kind 0 tau, 1 storage S, 2 storage QS, 3 top R
-    long long off;
-  };
-  // first pass: sizes
+  // per panel: rows of each level (level 0 = the panel itself)
   std::vector<std::vector<long long>> rows_per_level(panels.size());
   for (size_t p = 0; p < panels.size(); p++) {
     const TsqrPanel &P = panels[p];
+    if (P.m < P.ncol || P.ncol < 1 || P.nq > P.ncol) {
+      set_error("tsqr: panel %zu has m=%d ncol=%d nq=%d", p, P.m, P.ncol, P.nq);
+      return INPUT_ERROR;
+    }
     const int CR = std::max(128, 2 * P.ncol);
     long long rows = P.m;
     for (;;) {
@@ -209,47 +203,47 @@ int tsqr_run(const std::vector<TsqrPanel> &panels, hipStream_t s) {
       rows = nch * P.ncol;
     }
   }
-  // scratch layout
-  std::vector<std::vector<long long>> s_off(panels.size()), q_off(panels.size()),
-      t_off(panels.size());
-  std::vector<long long> rtop_off(panels.size());
-  for (size_t p = 0; p < panels.size(); p++) {
-    const TsqrPanel &P = panels[p];
-    const int CR = std::max(128, 2 * P.ncol);
-    for (size_t L = 0; L < rows_per_level[p].size(); L++) {
-      const long long rows = rows_per_level[p][L];
-      const long long nch = std::max<long long>(1, rows / CR);
-      t_off[p].push_back(scratch);
-      scratch += nch * P.ncol;
-      if (L > 0) {
-        s_off[p].push_back(scratch);
-        scratch += rows * P.ncol;
-        q_off[p].push_back(scratch);
-        scratch += rows * P.nq;
-      } else {
-        s_off[p].push_back(-1);
-        q_off[p].push_back(-1);
-      }
-    }
-    rtop_off[p] = scratch;
-    scratch += (long long)P.ncol * P.ncol;
-  }
-  double *sc = nullptr;
-  HM_OK(dalloc(&sc, (size_t)scratch + 1));
+  struct Level {
+    std::vector<QrItem> up, down;
+  };
   size_t maxL = 0;
   for (auto &r : rows_per_level) maxL = std::max(maxL, r.size());
-  levels.resize(maxL);
+  std::vector<Level> levels(maxL);
   for (size_t p = 0; p < panels.size(); p++) {
     const TsqrPanel &P = panels[p];
     const int CR = std::max(128, 2 * P.ncol);
     const size_t nl = rows_per_level[p].size();
+    // scratch of this panel: tau per level, stacked R and level Q for
+    // levels >= 1, the top R when the caller wants none
+    std::vector<double *> tau(nl), st(nl, nullptr), qs(nl, nullptr);
+    for (size_t L = 0; L < nl; L++) {
+      const long long rows = rows_per_level[p][L];
+      const long long nch = std::max<long long>(1, rows / CR);
+      tau[L] = ar.take<double>((size_t)nch * P.ncol);
+      prog.touch(tau[L], (size_t)nch * P.ncol * sizeof(double), "tsqr tau");
+      if (L > 0) {
+        st[L] = ar.take<double>((size_t)rows * P.ncol);
+        qs[L] = ar.take<double>((size_t)rows * std::max(P.nq, 1));
+        prog.touch(st[L], (size_t)rows * P.ncol * sizeof(double), "tsqr stack");
+        prog.touch(qs[L], (size_t)rows * std::max(P.nq, 1) * sizeof(double), "tsqr q");
+      }
+    }
+    double *rtop = P.R ? P.R : ar.take<double>((size_t)P.ncol * P.ncol);
+    const long long ldr_top = P.R ? P.ldr : P.ncol;
+    prog.touch(rtop, ((size_t)(P.ncol - 1) * ldr_top + P.ncol) * sizeof(double), "tsqr R");
+    prog.touch(P.A, ((size_t)(P.ncol - 1) * P.lda + P.m) * sizeof(double), "tsqr panel");
+    if (P.Q)
+      prog.touch(P.Q, ((size_t)(std::max(P.nq, 1) - 1) * P.ldq + P.m) * sizeof(double),
+                 "tsqr Q out");
+    if (P.Mt) prog.touch(P.Mt, (size_t)P.ncol * P.ncol * sizeof(double), "tsqr R^T");
+    if (P.est) prog.touch(P.est, sizeof(double), "tsqr est");
     for (size_t L = 0; L < nl; L++) {
       const long long rows = rows_per_level[p][L];
       const long long nch = std::max<long long>(1, rows / CR);
       const bool top = (L + 1 == nl);
-      double *store = L == 0 ? P.A : sc + s_off[p][L];
+      double *store = L == 0 ? P.A : st[L];
       const long long ld = L == 0 ? P.lda : rows;
-      double *qstore = L == 0 ? P.Q : sc + q_off[p][L];
+      double *qstore = L == 0 ? P.Q : qs[L];
       const long long ldq = L == 0 ? P.ldq : rows;
       const long long rows_next = top ? 0 : rows_per_level[p][L + 1];
       for (long long i = 0; i < nch; i++) {
@@ -261,21 +255,21 @@ int tsqr_run(const std::vector<TsqrPanel> &panels, hipStream_t s) {
         it.rows = (int)((i == nch - 1) ? rows - i * CR : CR);
         it.ncol = P.ncol;
         it.nq = P.nq;
-        it.tau = sc + t_off[p][L] + i * P.ncol;
+        it.tau = tau[L] + i * P.ncol;
         it.is_top = top ? 1 : 0;
         if (top) {
-          it.rout = P.R ? P.R : sc + rtop_off[p];
-          it.ldr = P.R ? P.ldr : P.ncol;
+          it.rout = rtop;
+          it.ldr = ldr_top;
           it.rrow0 = 0;
           it.mt = P.Mt;
           it.est = P.est;
           it.ell = P.ell;
           it.cin = nullptr;
         } else {
-          it.rout = sc + s_off[p][L + 1];
+          it.rout = st[L + 1];
           it.ldr = rows_next;
           it.rrow0 = (int)(i * P.ncol);
-          it.cin = sc + q_off[p][L + 1];
+          it.cin = qs[L + 1];
           it.ldc = rows_next;
           it.crow0 = (int)(i * P.ncol);
         }
@@ -286,44 +280,215 @@ int tsqr_run(const std::vector<TsqrPanel> &panels, hipStream_t s) {
       }
     }
   }
-  int rc = SUCCESS;
-  std::vector<QrItem *> dev(levels.size(), nullptr);
-  for (size_t L = 0; L < levels.size() && rc == SUCCESS; L++) {
-    auto &v = levels[L].up;
-    rc = dalloc(&dev[L], v.size() * 2 + 1);
-    if (rc == SUCCESS) {
-      std::vector<QrItem> both(v);
-      both.insert(both.end(), levels[L].down.begin(), levels[L].down.end());
-      hipError_t e = hipMemcpyAsync(dev[L], both.data(), both.size() * sizeof(QrItem),
-                                    hipMemcpyHostToDevice, s);
-      if (e != hipSuccess) {
-        set_error("tsqr: copy items: %s", hipGetErrorString(e));
-        rc = ALLOCATION_FAILURE;
+  std::vector<size_t> up_off(maxL), down_off(maxL);
+  for (size_t L = 0; L < maxL; L++) {
+    up_off[L] = prog.push(levels[L].up);
+    down_off[L] = prog.push(levels[L].down);
+  }
+  for (size_t L = 0; L < maxL; L++) {
+    const unsigned n = (unsigned)levels[L].up.size();
+    const size_t off = up_off[L];
+    if (n)
+      prog.add("tsqr_up", [=](hipStream_t s, const unsigned char *b) {
+        hipLaunchKernelGGL(k_tsqr_up, dim3(n), dim3(TQ_NT), 0, s,
+                           reinterpret_cast<const QrItem *>(b + off));
+      });
+  }
+  for (size_t L = maxL; L-- > 0;) {
+    const unsigned n = (unsigned)levels[L].down.size();
+    const size_t off = down_off[L];
+    if (n)
+      prog.add("tsqr_down", [=](hipStream_t s, const unsigned char *b) {
+        hipLaunchKernelGGL(k_tsqr_down, dim3(n), dim3(TQ_NT), 0, s,
+                           reinterpret_cast<const QrItem *>(b + off));
+      });
+  }
+  return SUCCESS;
+}
+
+int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
+  int maxp = 0;
+  for (const QrJob &J : jobs) {
+    if (J.L > J.m || J.L < 0) {
+      set_error("qr: job with m=%d L=%d", J.m, J.L);
+      return INPUT_ERROR;
+    }
+    maxp = std::max(maxp, (J.L + bw - 1) / bw);
+  }
+  // per-job temporaries: C1 (when R is not wanted) and C2 (L x bw), R1, R2
+  struct Tmp {
+    double *c1, *c2, *r1, *r2;
+  };
+  std::vector<Tmp> tmp(jobs.size());
+  std::vector<Copy2D> zero;
+  for (size_t q = 0; q < jobs.size(); q++) {
+    const QrJob &J = jobs[q];
+    if (J.L <= 0) continue;
+    tmp[q].c1 = J.R ? nullptr : ar.take<double>((size_t)J.L * bw);
+    tmp[q].c2 = ar.take<double>((size_t)J.L * bw);
+    tmp[q].r1 = ar.take<double>((size_t)bw * bw);
+    tmp[q].r2 = ar.take<double>((size_t)bw * bw);
+    if (J.R) zero.push_back({nullptr, J.R, 0, J.ldr, J.L, J.L, 0, 0});
+  }
+  HM_OK(copy2d_add(prog, zero));
+  for (int p = 0; p < maxp; p++) {
+    const int c0 = p * bw;
+    std::vector<GemmJob> g1, g2, g3, g4, g5, g6;
+    std::vector<TsqrPanel> t1, t2;
+    std::vector<Copy2D> cp;
+    for (size_t q = 0; q < jobs.size(); q++) {
+      const QrJob &J = jobs[q];
+      if (J.L <= c0) continue;
+      const int w = std::min(bw, J.L - c0);
+      double *Yp = J.Y + (long long)c0 * J.ldy;
+      double *Qp = J.Q + (long long)c0 * J.ldq;
+      double *C1 = J.R ? J.R + (long long)c0 * J.ldr : tmp[q].c1;
+      const long long ldc1 = J.R ? J.ldr : J.L;
+      if (p > 0) {
+        // C1 = Q_prev^T Y_p ; Y_p -= Q_prev C1
+        g1.push_back({J.Q, Yp, C1, nullptr, J.ldq, J.ldy, ldc1, c0, w, J.m, 1, 0, 0, 1.0, 0.0});
+        g2.push_back({J.Q, C1, Yp, nullptr, J.ldq, ldc1, J.ldy, J.m, w, c0, 1, 0, 0, -1.0, 1.0});
+      }
+      TsqrPanel P;
+      memset(&P, 0, sizeof(P));
+      P.A = Yp;
+      P.lda = J.ldy;
+      P.m = J.m;
+      P.ncol = w;
+      P.nq = w;
+      P.Q = Qp;
+      P.ldq = J.ldq;
+      if (p == 0 && J.R) {
+        P.R = J.R;
+        P.ldr = J.ldr;
+      } else {
+        P.R = tmp[q].r1;
+        P.ldr = bw;
+      }
+      t1.push_back(P);
+      if (p > 0) {
+        // reorthogonalise: C2 = Q_prev^T Q_p ; Q_p -= Q_prev C2 ; TSQR again
+        g3.push_back({J.Q, Qp, tmp[q].c2, nullptr, J.ldq, J.ldq, J.L, c0, w, J.m, 1, 0, 0, 1.0, 0.0});
+        g4.push_back({J.Q, tmp[q].c2, Qp, nullptr, J.ldq, J.L, J.ldq, J.m, w, c0, 1, 0, 0, -1.0, 1.0});
+        TsqrPanel P2;
+        memset(&P2, 0, sizeof(P2));
+        P2.A = Qp;
+        P2.lda = J.ldq;
+        P2.m = J.m;
+        P2.ncol = w;
+        P2.nq = w;
+        P2.Q = Yp;
+        P2.ldq = J.ldy;
+        P2.R = tmp[q].r2;
+        P2.ldr = bw;
+        t2.push_back(P2);
+        cp.push_back({Yp, Qp, J.ldy, J.ldq, J.m, w, 0, 0});
+        if (J.R) {
+          // R[:c0, p] = C1 + C2 R1 ; R[p, p] = R2 R1
+          g5.push_back({tmp[q].c2, tmp[q].r1, J.R + (long long)c0 * J.ldr, nullptr, J.L, bw,
+                        J.ldr, c0, w, w, 1, 0, 0, 1.0, 1.0});
+          g6.push_back({tmp[q].r2, tmp[q].r1, J.R + c0 + (long long)c0 * J.ldr, nullptr, bw, bw,
+                        J.ldr, w, w, w, 1, 0, 0, 1.0, 0.0});
+        }
       }
     }
+    HM_OK(gemm_add(prog, ar, g1, true, false, "qr proj1"));
+    HM_OK(gemm_add(prog, ar, g2, false, false, "qr upd1"));
+    HM_OK(tsqr_add(prog, ar, t1));
+    HM_OK(gemm_add(prog, ar, g3, true, false, "qr proj2"));
+    HM_OK(gemm_add(prog, ar, g4, false, false, "qr upd2"));
+    HM_OK(tsqr_add(prog, ar, t2));
+    HM_OK(copy2d_add(prog, cp));
+    HM_OK(gemm_add(prog, ar, g5, false, false, "qr r12"));
+    HM_OK(gemm_add(prog, ar, g6, false, false, "qr r22"));
   }
+  return SUCCESS;
+}
+
+int program_exec(const std::function<int(Program &, Arena &)> &build,
+                 const std::vector<MemRange> &inputs, hipStream_t s) {
+  Program dry;
+  Arena count;
+  dry.record = true;
+  HM_OK(build(dry, count));
+  for (const auto &r : inputs) dry.allow(r.lo, r.bytes);
+  dry.allow(Arena::fake_base(), count.off + 256);
+  std::string msg;
+  if (dry.audit(&msg) != SUCCESS) {
+    set_error("launch program audit: %s", msg.c_str());
+    return INPUT_ERROR;
+  }
+  Arena ar;
+  ar.counting = false;
+  ar.cap = count.off + 256;
+  HM_OK(dalloc(&ar.base, ar.cap));
+  Program prog;
+  int rc = build(prog, ar);
+  if (rc == SUCCESS && ar.off > ar.cap) {
+    set_error("launch program: scratch grew between passes");
+    rc = INPUT_ERROR;
+  }
+  if (rc == SUCCESS) rc = prog.upload(s);
+  if (rc == SUCCESS) rc = prog.run(s);
   if (rc == SUCCESS) {
-    Timed tu = timing_begin("tsqr_up", s);
-    for (size_t L = 0; L < levels.size(); L++)
-      hipLaunchKernelGGL(k_tsqr_up, dim3((unsigned)levels[L].up.size()), dim3(TQ_NT), 0, s,
-                         dev[L]);
-    timing_end(tu, s);
-    Timed td = timing_begin("tsqr_down", s);
-    for (size_t L = levels.size(); L-- > 0;)
-      if (!levels[L].down.empty())
-        hipLaunchKernelGGL(k_tsqr_down, dim3((unsigned)levels[L].down.size()), dim3(TQ_NT),
-                           0, s, dev[L] + levels[L].up.size());
-    timing_end(td, s);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
-      set_error("tsqr: %s", hipGetErrorString(e));
+      set_error("launch program: %s", hipGetErrorString(e));
       rc = ALLOCATION_FAILURE;
     }
   }
-  for (auto *d : dev) dfree(d);
-  dfree(sc);
+  (void)hipStreamSynchronize(s);
+  dfree(ar.base);
   return rc;
 }
 
 }  // namespace hm
+
+using namespace hm;
+
+// Host-only check of the QR program builder: builds the program for jobs of
+// the given shapes on fake buffers (nothing is allocated or launched) and
+// verifies that every GEMM, TSQR, copy and norm item stays inside them.
+extern "C" int hm_hip_audit_qr(int njobs, const int *m, const int *L, int with_r,
+                               char *msg, int msglen) {
+  std::vector<QrJob> jobs;
+  std::vector<MemRange> bufs;
+  char *fake = reinterpret_cast<char *>((uintptr_t)1 << 40);
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    char *p = fake + off;
+    off += (n * sizeof(double) + 4095) & ~(size_t)4095;
+    bufs.push_back({p, n * sizeof(double)});
+    return reinterpret_cast<double *>(p);
+  };
+  for (int i = 0; i < njobs; i++) {
+    QrJob J;
+    J.m = m[i];
+    J.L = L[i];
+    J.ldy = m[i] + (i % 3);
+    J.Y = take((size_t)J.ldy * std::max(1, J.L));
+    J.ldq = m[i] + (i % 2);
+    J.Q = take((size_t)J.ldq * std::max(1, J.L));
+    J.ldr = J.L + 1;
+    J.R = with_r ? take((size_t)J.ldr * std::max(1, J.L)) : nullptr;
+    jobs.push_back(J);
+  }
+  Program p;
+  Arena ar;
+  p.record = true;
+  int rc = qr_add(p, ar, jobs);
+  std::string text = rc == SUCCESS ? "" : "builder failed";
+  if (rc == SUCCESS) {
+    for (const auto &b : bufs) p.allow(b.lo, b.bytes);
+    p.allow(Arena::fake_base(), ar.off + 256);
+    rc = p.audit(&text);
+    if (rc == SUCCESS) {
+      char buf[128];
+      snprintf(buf, sizeof(buf), "%zu ops, %zu items bytes, %zu scratch bytes", p.ops.size(),
+               p.blob.size(), ar.off);
+      text = buf;
+    }
+  }
+  if (msg && msglen > 0) snprintf(msg, (size_t)msglen, "%s", text.c_str());
+  return rc;
+}

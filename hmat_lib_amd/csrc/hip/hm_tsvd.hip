@@ -270,26 +270,6 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
   }
 }
 
-// U = Q G (m x k) and V = Q2 Z (n x k).  work = (block, side, row chunk).
-__global__ __launch_bounds__(256) void k_form(TsvdBlock *blocks, const int4 *work,
-                                              TsvdBufs bf) {
-  const int4 w = work[blockIdx.x];
-  const TsvdBlock B = blocks[w.x];
-  const int side = w.y;
-  const int rows = side ? B.n : B.m;
-  const int i = w.z * 256 + threadIdx.x;
-  if (i >= rows) return;
-  const int nc = B.ell, k = B.k;
-  const double *Qb = side ? bf.Q2 + B.boff : bf.Q + B.qoff;
-  const double *Gb = side ? bf.Z + B.moff : bf.G + B.moff;
-  double *O = side ? B.vout : B.uout;
-  for (int j = 0; j < k; j++) {
-    double acc = 0.0;
-    for (int e = 0; e < nc; e++) acc += Qb[i + (long long)e * rows] * Gb[e + (long long)j * nc];
-    O[i + (long long)j * rows] = acc;
-  }
-}
-
 struct CopyItem {
   const double *src;
   double *dst;
@@ -466,64 +446,70 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     HM_OK(to_device(&d_ws, ws, s)); mem.keep(d_ws);
     HM_OK(to_device(&d_wp, wp, s)); mem.keep(d_wp);
     HM_OK(to_device(&d_list, order, s)); mem.keep(d_list);
-    Timed t0 = timing_begin("tsvd_omega", s);
-    hipLaunchKernelGGL(k_gen_omega, dim3((unsigned)wo.size()), dim3(256), 0, s, d_blocks,
-                       d_wo, bf, opt.seed);
-    timing_end(t0, s);
-    HM_LAUNCH_CHECK();
-    Timed t1 = timing_begin("tsvd_sketch", s);
-    hipLaunchKernelGGL(k_sketch, dim3((unsigned)ws.size()), dim3(256), 0, s, d_blocks, d_ws,
-                       bf);
-    timing_end(t1, s);
-    HM_LAUNCH_CHECK();
-    {
-      std::vector<TsqrPanel> pan;
+    // the round as one launch program: sketch, QR of the sketch, posterior
+    // test, projection, QR of B^T with R, Jacobi SVD of R^T
+    std::vector<MemRange> inputs = {
+        {bf.Y, (size_t)ytot * 8 + 16}, {bf.Om, (size_t)otot * 8 + 16},
+        {bf.Q, (size_t)qtot * 8 + 16}, {bf.Bt, (size_t)btot * 8 + 16},
+        {bf.Q2, (size_t)btot * 8 + 16}, {bf.M, (size_t)mtot * 8 + 16},
+        {d_blocks, (size_t)nb * sizeof(TsvdBlock)}};
+    const unsigned long long seed = opt.seed;
+    const unsigned nwo = (unsigned)wo.size(), nws = (unsigned)ws.size(),
+                   nwp = (unsigned)wp.size(), nord = (unsigned)order.size();
+    const double accept = opt.accept;
+    auto build = [&](Program &pg, Arena &ar) -> int {
+      pg.add("omega+sketch", [=](hipStream_t st, const unsigned char *) {
+        Timed t0 = timing_begin("tsvd_omega", st);
+        hipLaunchKernelGGL(k_gen_omega, dim3(nwo), dim3(256), 0, st, d_blocks, d_wo, bf, seed);
+        timing_end(t0, st);
+        Timed t1 = timing_begin("tsvd_sketch", st);
+        hipLaunchKernelGGL(k_sketch, dim3(nws), dim3(256), 0, st, d_blocks, d_ws, bf);
+        timing_end(t1, st);
+      });
+      std::vector<QrJob> qy, qb;
+      std::vector<GemmJob> t1, t2;
+      std::vector<ColNorm> cn;
+      std::vector<Copy2D> tr;
       for (int i : order) {
         const TsvdBlock &b = blocks[i];
-        TsqrPanel P;
-        memset(&P, 0, sizeof(P));
-        P.A = bf.Y + b.yoff;
-        P.lda = b.m;
-        P.m = b.m;
-        P.ncol = b.ell + b.ntest;
-        P.nq = b.ell;
-        P.Q = bf.Q + b.qoff;
-        P.ldq = b.m;
-        P.est = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
-                                           offsetof(TsvdBlock, err));
-        P.ell = b.ell;
-        pan.push_back(P);
+        double *Y = bf.Y + b.yoff;
+        qy.push_back({Y, b.m, b.m, b.ell, bf.Q + b.qoff, b.m, nullptr, 0});
+        double *R2 = ar.take<double>((size_t)b.ell * b.ell);
+        qb.push_back({bf.Bt + b.boff, b.n, b.n, b.ell, bf.Q2 + b.boff, b.n, R2, b.ell});
+        tr.push_back({R2, bf.M + b.moff, b.ell, b.ell, b.ell, b.ell, 1, 0});
+        if (b.ntest > 0) {
+          double *Yt = Y + (long long)b.ell * b.m;
+          double *T = ar.take<double>((size_t)b.ell * b.ntest);
+          t1.push_back({bf.Q + b.qoff, Yt, T, nullptr, b.m, b.m, b.ell, b.ell, b.ntest, b.m,
+                        1, 0, 0, 1.0, 0.0});
+          t2.push_back({bf.Q + b.qoff, T, Yt, nullptr, b.m, b.ell, b.m, b.m, b.ntest, b.ell,
+                        1, 0, 0, -1.0, 1.0});
+          double *err = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
+                                                   offsetof(TsvdBlock, err));
+          cn.push_back({Yt, b.m, b.m, b.ntest, err});
+        }
       }
-      HM_OK(tsqr_run(pan, s));
-    }
-    Timed t3 = timing_begin("tsvd_project", s);
-    hipLaunchKernelGGL(k_project, dim3((unsigned)wp.size()), dim3(256), 0, s, d_blocks, d_wp,
-                       bf);
-    timing_end(t3, s);
-    HM_LAUNCH_CHECK();
-    {
-      std::vector<TsqrPanel> pan;
-      for (int i : order) {
-        const TsvdBlock &b = blocks[i];
-        TsqrPanel P;
-        memset(&P, 0, sizeof(P));
-        P.A = bf.Bt + b.boff;
-        P.lda = b.n;
-        P.m = b.n;
-        P.ncol = b.ell;
-        P.nq = b.ell;
-        P.Q = bf.Q2 + b.boff;
-        P.ldq = b.n;
-        P.Mt = bf.M + b.moff;
-        pan.push_back(P);
+      HM_OK(qr_add(pg, ar, qy));
+      for (int rep = 0; rep < 2; rep++) {  // residual of the test columns, twice
+        HM_OK(gemm_add(pg, ar, t1, true, false, "test proj"));
+        HM_OK(gemm_add(pg, ar, t2, false, false, "test upd"));
       }
-      HM_OK(tsqr_run(pan, s));
-    }
-    Timed t5 = timing_begin("tsvd_jacobi", s);
-    hipLaunchKernelGGL(k_jacobi, dim3((unsigned)order.size()), dim3(256), 0, s, d_blocks,
-                       d_list, bf, opt.accept);
-    timing_end(t5, s);
-    HM_LAUNCH_CHECK();
+      HM_OK(colnorm_add(pg, cn));
+      pg.add("project", [=](hipStream_t st, const unsigned char *) {
+        Timed t3 = timing_begin("tsvd_project", st);
+        hipLaunchKernelGGL(k_project, dim3(nwp), dim3(256), 0, st, d_blocks, d_wp, bf);
+        timing_end(t3, st);
+      });
+      HM_OK(qr_add(pg, ar, qb));
+      HM_OK(copy2d_add(pg, tr));
+      pg.add("jacobi", [=](hipStream_t st, const unsigned char *) {
+        Timed t5 = timing_begin("tsvd_jacobi", st);
+        hipLaunchKernelGGL(k_jacobi, dim3(nord), dim3(256), 0, st, d_blocks, d_list, bf, accept);
+        timing_end(t5, st);
+      });
+      return SUCCESS;
+    };
+    HM_OK(program_exec(build, inputs, s));
     std::vector<TsvdBlock> hb(nb);
     HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
                           hipMemcpyDeviceToHost, s));
@@ -563,21 +549,28 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       }
       HM_HIP(hipMemcpyAsync(d_blocks, blocks.data(), nb * sizeof(TsvdBlock),
                             hipMemcpyHostToDevice, s));
-      std::vector<int4> wf;
-      for (int i : done) {
-        const TsvdBlock &b = blocks[i];
-        for (int side = 0; side < 2; side++) {
-          const int rows = side ? b.n : b.m;
-          for (int c = 0; c < (rows + 255) / 256; c++) wf.push_back(make_int4(i, side, c, 0));
+      {
+        std::vector<GemmJob> gu;
+        for (int i : done) {
+          const TsvdBlock &b = blocks[i];
+          gu.push_back({bf.Q + b.qoff, bf.G + b.moff, b.uout, nullptr, b.m, b.ell, b.m, b.m, b.k,
+                        b.ell, 1, 0, 0, 1.0, 0.0});
+          gu.push_back({bf.Q2 + b.boff, bf.Z + b.moff, b.vout, nullptr, b.n, b.ell, b.n, b.n,
+                        b.k, b.ell, 1, 0, 0, 1.0, 0.0});
         }
+        std::vector<MemRange> fin = {{bf.Q, (size_t)qtot * 8 + 16},
+                                     {bf.Q2, (size_t)btot * 8 + 16},
+                                     {bf.G, (size_t)mtot * 8 + 16},
+                                     {bf.Z, (size_t)mtot * 8 + 16},
+                                     {ro.u, (size_t)ut * 8 + 16},
+                                     {ro.v, (size_t)vt * 8 + 16}};
+        HM_OK(program_exec(
+            [&](Program &pg, Arena &ar) {
+              pg.add("form", [](hipStream_t st, const unsigned char *) {});
+              return gemm_add(pg, ar, gu, false, false, "tsvd form");
+            },
+            fin, s));
       }
-      int4 *d_wf;
-      HM_OK(to_device(&d_wf, wf, s)); mem.keep(d_wf);
-      Timed t6 = timing_begin("tsvd_form", s);
-      hipLaunchKernelGGL(k_form, dim3((unsigned)wf.size()), dim3(256), 0, s,
-                         d_blocks, d_wf, bf);
-      timing_end(t6, s);
-      HM_LAUNCH_CHECK();
       if (sigma) {
         std::vector<double> hs(stot);
         HM_HIP(hipMemcpyAsync(hs.data(), bf.Sig, stot * sizeof(double),
@@ -706,19 +699,16 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
   HM_OK(to_device(&d_list, list, s)); mem.keep(d_list);
   Timed t0 = timing_begin("recompress_core", s);
   {
-    std::vector<TsqrPanel> pan;
+    std::vector<QrJob> qj;
     for (int i = 0; i < nn; i++) {
       const TsvdBlock &b = blocks[i];
-      TsqrPanel P;
-      memset(&P, 0, sizeof(P));
-      P.A = bf.Y + b.yoff;  P.lda = b.m;  P.m = b.m;  P.ncol = b.ell;  P.nq = b.ell;
-      P.Q = bf.Q + b.qoff;  P.ldq = b.m;  P.R = RU + b.moff;  P.ldr = b.ell;
-      pan.push_back(P);
-      P.A = bf.Bt + b.boff; P.lda = b.n;  P.m = b.n;
-      P.Q = bf.Q2 + b.boff; P.ldq = b.n;  P.R = RV + b.moff;
-      pan.push_back(P);
+      qj.push_back({bf.Y + b.yoff, b.m, b.m, b.ell, bf.Q + b.qoff, b.m, RU + b.moff, b.ell});
+      qj.push_back({bf.Bt + b.boff, b.n, b.n, b.ell, bf.Q2 + b.boff, b.n, RV + b.moff, b.ell});
     }
-    HM_OK(tsqr_run(pan, s));
+    std::vector<MemRange> inputs = {{bf.Y, (size_t)ytot * 8 + 16}, {bf.Q, (size_t)ytot * 8 + 16},
+                                    {bf.Bt, (size_t)btot * 8 + 16}, {bf.Q2, (size_t)btot * 8 + 16},
+                                    {RU, (size_t)mtot * 8 + 16}, {RV, (size_t)mtot * 8 + 16}};
+    HM_OK(program_exec([&](Program &pg, Arena &ar) { return qr_add(pg, ar, qj); }, inputs, s));
   }
   hipLaunchKernelGGL(k_rmul, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf, RU, RV);
   hipLaunchKernelGGL(k_jacobi, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf, 0.0);
@@ -736,7 +726,6 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
   HM_OK(dalloc(&uo, ut)); mem.keep(uo);
   HM_OK(dalloc(&vo, vt)); mem.keep(vo);
   ut = vt = 0;
-  std::vector<int4> wf;
   std::vector<CopyItem> outc;
   for (int i = 0; i < nn; i++) {
     TsvdBlock &b = blocks[i];
@@ -746,20 +735,28 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
     outc.push_back({b.vout, d_v[i], (long long)b.n * b.k});
     ut += (long long)b.m * b.k;
     vt += (long long)b.n * b.k;
-    for (int side = 0; side < 2; side++) {
-      const int rows = side ? b.n : b.m;
-      for (int c = 0; c < (rows + 255) / 256; c++) wf.push_back(make_int4(i, side, c, 0));
-    }
     k_out[i] = b.k;
   }
   HM_HIP(hipMemcpyAsync(d_blocks, blocks.data(), nn * sizeof(TsvdBlock),
                         hipMemcpyHostToDevice, s));
-  int4 *d_wf;
   CopyItem *d_out;
-  HM_OK(to_device(&d_wf, wf, s)); mem.keep(d_wf);
   HM_OK(to_device(&d_out, outc, s)); mem.keep(d_out);
-  hipLaunchKernelGGL(k_form, dim3((unsigned)wf.size()), dim3(256), 0, s, d_blocks, d_wf, bf);
-  HM_LAUNCH_CHECK();
+  {
+    std::vector<GemmJob> gu;
+    for (int i = 0; i < nn; i++) {
+      const TsvdBlock &b = blocks[i];
+      gu.push_back({bf.Q + b.qoff, bf.G + b.moff, b.uout, nullptr, b.m, b.ell, b.m, b.m, b.k,
+                    b.ell, 1, 0, 0, 1.0, 0.0});
+      gu.push_back({bf.Q2 + b.boff, bf.Z + b.moff, b.vout, nullptr, b.n, b.ell, b.n, b.n, b.k,
+                    b.ell, 1, 0, 0, 1.0, 0.0});
+    }
+    std::vector<MemRange> fin = {{bf.Q, (size_t)ytot * 8 + 16}, {bf.Q2, (size_t)btot * 8 + 16},
+                                 {bf.G, (size_t)mtot * 8 + 16}, {bf.Z, (size_t)mtot * 8 + 16},
+                                 {uo, (size_t)ut * 8 + 16}, {vo, (size_t)vt * 8 + 16}};
+    HM_OK(program_exec([&](Program &pg, Arena &ar) {
+      return gemm_add(pg, ar, gu, false, false, "recompress form");
+    }, fin, s));
+  }
   hipLaunchKernelGGL(k_copy_list, dim3((unsigned)outc.size(), 8), dim3(256), 0, s, d_out);
   HM_LAUNCH_CHECK();
   HM_HIP(hipStreamSynchronize(s));

@@ -161,6 +161,14 @@ HM_HIP_API int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
  * assume; out (host, 256 doubles) receives D row-major. */
 HM_HIP_API int hm_hip_selftest_mfma(double *out);
 
+/* Host-only check (no device needed) of the wide-panel QR launch program:
+ * builds it for njobs panels of m[i] x L[i] (with the triangular factor
+ * when with_r) on fake buffers and verifies that every work item's reads
+ * and writes stay inside the buffers and scratch it was given.  msg
+ * receives a summary or the first violation.  Returns 0 when clean. */
+HM_HIP_API int hm_hip_audit_qr(int njobs, const int *m, const int *L, int with_r,
+                               char *msg, int msglen);
+
 /* ---- kernel timing (HIP events around tracked kernels) ----------------- */
 HM_HIP_API int hm_hip_kernel_timing(int enable);
 /* Time only the kernel of this name (NULL: every named kernel). */

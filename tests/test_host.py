@@ -219,3 +219,27 @@ def test_no_device_fails_loudly(capfd):
     err = capfd.readouterr().err
     assert "hmat_lib (MI355X)" in err and "failed" in err
     lib.free_tree_hodlr(C.byref(t))
+
+
+QR_SHAPES = [
+    # (m, L): single panel, exact multiples of the 64-column panel, ragged
+    # last panel, square, long K that triggers split-K projections
+    [(1, 1), (7, 7), (64, 64), (65, 64)],
+    [(200, 65), (300, 128), (129, 129), (1000, 200)],
+    [(32768, 518), (16384, 390), (5000, 64), (70, 1)],
+]
+
+
+@pytest.mark.parametrize("with_r", [0, 1])
+@pytest.mark.parametrize("shapes", QR_SHAPES, ids=["small", "ragged", "wide"])
+def test_qr_program_stays_in_bounds(shapes, with_r):
+    """Every work item of the wide-panel QR launch program (projections,
+    TSQR levels, copies, triangular-factor updates) reads and writes only
+    inside the buffers it was given; checked on the host, no device."""
+    import ctypes as C
+    from hmat_lib_amd import capi
+    m = (C.c_int * len(shapes))(*[s[0] for s in shapes])
+    L = (C.c_int * len(shapes))(*[s[1] for s in shapes])
+    msg = C.create_string_buffer(256)
+    rc = capi.hip().hm_hip_audit_qr(len(shapes), m, L, with_r, msg, 256)
+    assert rc == 0, msg.value.decode()
