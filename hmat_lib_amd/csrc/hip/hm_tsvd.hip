@@ -151,9 +151,11 @@ __global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
 // One-sided Jacobi SVD of M (nc x nc); V accumulates the rotations.
 // Outputs G = (M V)[:, perm[:k]] (U carries sigma), Z = V[:, perm[:k]],
 // sorted singular values and the rank k by the block's rule.
+// finish_only: M and V were already rotated by the block sweeps (columns of
+// M V-orthogonal); only the ordering, rank rule and gathers remain.
 __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
                                                 const int *list, TsvdBufs bf,
-                                                double accept) {
+                                                double accept, int finish_only) {
   __shared__ double sig[MAX_ELL];
   __shared__ int perm[MAX_ELL];
   __shared__ int rotated;
@@ -163,12 +165,13 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
   constexpr int NW = 4;
   double *M = bf.M + B.moff;
   double *V = bf.Vj + B.moff;
-  for (int e = threadIdx.x; e < nc * nc; e += 256)
-    V[e] = ((e % nc) == (e / nc)) ? 1.0 : 0.0;
+  if (!finish_only)
+    for (int e = threadIdx.x; e < nc * nc; e += 256)
+      V[e] = ((e % nc) == (e / nc)) ? 1.0 : 0.0;
   __syncthreads();
   const int np = nc + (nc & 1);
   const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
-  for (int sweep = 0; sweep < 60 && nc > 1; sweep++) {
+  for (int sweep = 0; sweep < (finish_only ? 0 : 60) && nc > 1; sweep++) {
     // every thread has read the previous sweep's flag before it is cleared,
     // so all waves take the same exit decision
     __syncthreads();
@@ -270,6 +273,252 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
   }
 }
 
+template <class T>
+static int to_device(T **d, const std::vector<T> &h, hipStream_t s) {
+  HM_OK(dalloc(d, h.size()));
+  if (!h.empty())
+    HM_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T),
+                          hipMemcpyHostToDevice, s));
+  return SUCCESS;
+}
+
+// ------------------------------------------------ block one-sided Jacobi
+// For cores wider than BJ_SCALAR_MAX columns: the columns of M (nc x nc)
+// are cut into blocks of BJ_W; each round of a sweep pairs the blocks
+// round robin and one workgroup per pair (a, b) (1) forms the Gram matrix
+// G = M_S^T M_S of the <= 64 columns S = a u b on MFMA, (2) diagonalises G
+// in LDS by cyclic two-sided Jacobi (J accumulates the rotations), (3)
+// applies J to the columns S of M and of V.  A pair whose first inner
+// sweep rotates anything marks its matrix as not converged.
+static constexpr int BJ_W = 32;
+static constexpr int BJ_SCALAR_MAX = 64;
+typedef MfmaTile<2, 2, 2, 2, 16> BjTile;  // 64 x 64
+
+__global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list, TsvdBufs bf) {
+  const TsvdBlock B = blocks[list[blockIdx.x]];
+  const int nc = B.ell;
+  double *V = bf.Vj + B.moff;
+  const long long E = (long long)nc * nc;
+  for (long long e = (long long)blockIdx.y * 256 + threadIdx.x; e < E;
+       e += (long long)gridDim.y * 256)
+    V[e] = ((e % nc) == (e / nc)) ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4 *items,
+                                                   TsvdBufs bf, int *flags) {
+  __shared__ double lds[BjTile::LDS_DOUBLES];
+  __shared__ double G[64][65];
+  __shared__ double J[64][65];
+  __shared__ double cs[32][2];
+  __shared__ int pq[32][2];
+  __shared__ int rot;
+  const int4 it = items[blockIdx.x];
+  const TsvdBlock B = blocks[it.x];
+  const int nc = B.ell;
+  const int ca = it.y * BJ_W, wa = min(BJ_W, nc - ca);
+  const int cb = it.z * BJ_W, wb = min(BJ_W, nc - cb);
+  const int W = wa + wb;
+  auto col = [&](int i) -> int { return i < wa ? ca + i : cb + (i - wa); };
+  double *M = bf.M + B.moff;
+  double *Vm = bf.Vj + B.moff;
+  const int tid = threadIdx.x;
+  // (1) Gram matrix of the pair's columns
+  {
+    BjTile tile;
+    tile.zero();
+    auto la = [&](int i, int k) -> double {
+      return (i < W && k < nc) ? M[k + (long long)col(i) * nc] : 0.0;
+    };
+    auto lb = [&](int k, int j) -> double {
+      return (j < W && k < nc) ? M[k + (long long)col(j) * nc] : 0.0;
+    };
+    tile.template run<false, false>(lds, nc, la, lb);
+    tile.store([&](int i, int j, double v) { G[i][j] = v; });
+  }
+  for (int e = tid; e < 64 * 64; e += 256) J[e >> 6][e & 63] = ((e >> 6) == (e & 63)) ? 1.0 : 0.0;
+  if (tid == 0) rot = 0;
+  __syncthreads();
+  const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
+  // (2) cyclic Jacobi on G (round robin over 64 indices, 32 disjoint pairs)
+  for (int sweep = 0; sweep < 4; sweep++) {
+    __syncthreads();
+    int any = 0;
+    for (int r = 0; r < 63; r++) {
+      if (tid < 32) {
+        const int pa = tid, pb = 63 - tid;
+        const int a = (pa == 0) ? 0 : ((pa - 1 + r) % 63) + 1;
+        const int b = (pb == 0) ? 0 : ((pb - 1 + r) % 63) + 1;
+        const int p = min(a, b), q = max(a, b);
+        double c = 1.0, sn = 0.0;
+        if (q < W) {
+          const double gpp = G[p][p], gqq = G[q][q], gpq = G[p][q];
+          if (gpq != 0.0 && fabs(gpq) > tol * sqrt(gpp * gqq)) {
+            const double zeta = (gqq - gpp) / (2.0 * gpq);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            c = 1.0 / sqrt(1.0 + t * t);
+            sn = c * t;
+            any = 1;
+          }
+        }
+        cs[tid][0] = c;
+        cs[tid][1] = sn;
+        pq[tid][0] = p;
+        pq[tid][1] = q;
+      }
+      __syncthreads();
+      for (int e = tid; e < 32 * 64; e += 256) {  // rows p, q of G
+        const int t = e >> 6, k = e & 63;
+        const double sn = cs[t][1];
+        if (sn != 0.0) {
+          const double c = cs[t][0];
+          const int p = pq[t][0], q = pq[t][1];
+          const double gp = G[p][k], gq = G[q][k];
+          G[p][k] = c * gp - sn * gq;
+          G[q][k] = sn * gp + c * gq;
+        }
+      }
+      __syncthreads();
+      for (int e = tid; e < 32 * 64; e += 256) {  // columns p, q of G and J
+        const int t = e >> 6, k = e & 63;
+        const double sn = cs[t][1];
+        if (sn != 0.0) {
+          const double c = cs[t][0];
+          const int p = pq[t][0], q = pq[t][1];
+          const double gp = G[k][p], gq = G[k][q];
+          G[k][p] = c * gp - sn * gq;
+          G[k][q] = sn * gp + c * gq;
+          const double jp = J[k][p], jq = J[k][q];
+          J[k][p] = c * jp - sn * jq;
+          J[k][q] = sn * jp + c * jq;
+        }
+      }
+      __syncthreads();
+    }
+    // any rotation in this sweep (only threads < 32 decide rotations)
+    if (any) rot = 1;
+    __syncthreads();
+    const int go = rot;
+    __syncthreads();
+    if (sweep == 0 && go && tid == 0) flags[it.x] = 1;
+    if (!go) break;
+    if (tid == 0) rot = 0;
+  }
+  __syncthreads();
+  // (3) M_S <- M_S J and V_S <- V_S J, 64-row tiles, in place
+  for (int side = 0; side < 2; side++) {
+    double *X = side ? Vm : M;
+    for (int r0 = 0; r0 < nc; r0 += 64) {
+      BjTile tile;
+      tile.zero();
+      auto la = [&](int i, int k) -> double {
+        return (r0 + i < nc && k < W) ? X[(r0 + i) + (long long)col(k) * nc] : 0.0;
+      };
+      auto lb = [&](int k, int j) -> double { return J[k][j]; };
+      tile.template run<true, false>(lds, 64, la, lb);
+      tile.store([&](int i, int j, double v) {
+        if (r0 + i < nc && j < W) X[(r0 + i) + (long long)col(j) * nc] = v;
+      });
+    }
+  }
+}
+
+// SVD cores of the listed blocks (M = the core, nc = ell): scalar Jacobi for
+// narrow cores, block sweeps + finish for wide ones.  Synchronizes.
+static int svd_cores(TsvdBlock *d_blocks, const std::vector<TsvdBlock> &blocks,
+                     const std::vector<int> &list, const TsvdBufs &bf, double accept,
+                     hipStream_t s) {
+  std::vector<int> small, big;
+  for (int i : list) (blocks[i].ell > BJ_SCALAR_MAX ? big : small).push_back(i);
+  std::vector<void *> mem;
+  auto cleanup = [&]() {
+    for (void *p : mem) dfree(p);
+  };
+  int rc = SUCCESS;
+  if (!big.empty()) {
+    // round-robin schedule per matrix; items of round r of every matrix
+    int maxr = 0;
+    std::vector<std::vector<int4>> rounds;
+    for (int i : big) {
+      const int nb = (blocks[i].ell + BJ_W - 1) / BJ_W;
+      const int np = nb + (nb & 1);
+      maxr = std::max(maxr, np - 1);
+    }
+    rounds.resize(maxr);
+    for (int i : big) {
+      const int nb = (blocks[i].ell + BJ_W - 1) / BJ_W;
+      const int np = nb + (nb & 1);
+      for (int r = 0; r < np - 1; r++)
+        for (int t = 0; t < np / 2; t++) {
+          const int pa = t, pb = np - 1 - t;
+          const int a = (pa == 0) ? 0 : ((pa - 1 + r) % (np - 1)) + 1;
+          const int b = (pb == 0) ? 0 : ((pb - 1 + r) % (np - 1)) + 1;
+          if (a >= nb || b >= nb) continue;
+          rounds[r].push_back(make_int4(i, std::min(a, b), std::max(a, b), 0));
+        }
+    }
+    std::vector<int4> all;
+    std::vector<size_t> roff(maxr + 1, 0);
+    for (int r = 0; r < maxr; r++) {
+      roff[r] = all.size();
+      all.insert(all.end(), rounds[r].begin(), rounds[r].end());
+    }
+    roff[maxr] = all.size();
+    int4 *d_items = nullptr;
+    int *d_big = nullptr, *d_flags = nullptr;
+    rc = to_device(&d_items, all, s);
+    if (rc == SUCCESS) { mem.push_back(d_items); rc = to_device(&d_big, big, s); }
+    if (rc == SUCCESS) { mem.push_back(d_big); rc = dalloc(&d_flags, blocks.size()); }
+    if (rc == SUCCESS) mem.push_back(d_flags);
+    if (rc != SUCCESS) {
+      cleanup();
+      return rc;
+    }
+    hipLaunchKernelGGL(k_eye, dim3((unsigned)big.size(), 64), dim3(256), 0, s, d_blocks, d_big, bf);
+    std::vector<int> hflags(blocks.size());
+    for (int sweep = 0; sweep < 30; sweep++) {
+      if (hipMemsetAsync(d_flags, 0, blocks.size() * sizeof(int), s) != hipSuccess) break;
+      for (int r = 0; r < maxr; r++) {
+        const unsigned n = (unsigned)(roff[r + 1] - roff[r]);
+        if (n)
+          hipLaunchKernelGGL(k_bjac_pair, dim3(n), dim3(256), 0, s, d_blocks, d_items + roff[r],
+                             bf, d_flags);
+      }
+      hipError_t e = hipMemcpyAsync(hflags.data(), d_flags, blocks.size() * sizeof(int),
+                                    hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        set_error("block Jacobi: %s", hipGetErrorString(e));
+        cleanup();
+        return ALLOCATION_FAILURE;
+      }
+      bool any = false;
+      for (int i : big) any |= hflags[i] != 0;
+      if (!any) break;
+    }
+    hipLaunchKernelGGL(k_jacobi, dim3((unsigned)big.size()), dim3(256), 0, s, d_blocks, d_big,
+                       bf, accept, 1);
+  }
+  if (!small.empty()) {
+    int *d_small = nullptr;
+    rc = to_device(&d_small, small, s);
+    if (rc != SUCCESS) {
+      cleanup();
+      return rc;
+    }
+    mem.push_back(d_small);
+    hipLaunchKernelGGL(k_jacobi, dim3((unsigned)small.size()), dim3(256), 0, s, d_blocks,
+                       d_small, bf, accept, 0);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess) {
+    set_error("svd cores: %s", hipGetErrorString(e));
+    return ALLOCATION_FAILURE;
+  }
+  return SUCCESS;
+}
+
 struct CopyItem {
   const double *src;
   double *dst;
@@ -327,14 +576,6 @@ __global__ __launch_bounds__(256) void k_outer(const OuterJob *jobs, const int4 
 }
 
 // ------------------------------------------------------------ driver
-template <class T>
-static int to_device(T **d, const std::vector<T> &h, hipStream_t s) {
-  HM_OK(dalloc(d, h.size()));
-  if (!h.empty())
-    HM_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T),
-                          hipMemcpyHostToDevice, s));
-  return SUCCESS;
-}
 
 namespace {
 struct Scoped {
@@ -455,7 +696,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         {d_blocks, (size_t)nb * sizeof(TsvdBlock)}};
     const unsigned long long seed = opt.seed;
     const unsigned nwo = (unsigned)wo.size(), nws = (unsigned)ws.size(),
-                   nwp = (unsigned)wp.size(), nord = (unsigned)order.size();
+                   nwp = (unsigned)wp.size();
     const double accept = opt.accept;
     auto build = [&](Program &pg, Arena &ar) -> int {
       pg.add("omega+sketch", [=](hipStream_t st, const unsigned char *) {
@@ -502,14 +743,14 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       });
       HM_OK(qr_add(pg, ar, qb));
       HM_OK(copy2d_add(pg, tr));
-      pg.add("jacobi", [=](hipStream_t st, const unsigned char *) {
-        Timed t5 = timing_begin("tsvd_jacobi", st);
-        hipLaunchKernelGGL(k_jacobi, dim3(nord), dim3(256), 0, st, d_blocks, d_list, bf, accept);
-        timing_end(t5, st);
-      });
       return SUCCESS;
     };
     HM_OK(program_exec(build, inputs, s));
+    {
+      Timed t5 = timing_begin("tsvd_jacobi", s);
+      HM_OK(svd_cores(d_blocks, blocks, order, bf, accept, s));
+      timing_end(t5, s);
+    }
     std::vector<TsvdBlock> hb(nb);
     HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
                           hipMemcpyDeviceToHost, s));
@@ -711,7 +952,7 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
     HM_OK(program_exec([&](Program &pg, Arena &ar) { return qr_add(pg, ar, qj); }, inputs, s));
   }
   hipLaunchKernelGGL(k_rmul, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf, RU, RV);
-  hipLaunchKernelGGL(k_jacobi, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf, 0.0);
+  HM_OK(svd_cores(d_blocks, blocks, list, bf, 0.0, s));
   timing_end(t0, s);
   HM_LAUNCH_CHECK();
   HM_HIP(hipMemcpyAsync(blocks.data(), d_blocks, nn * sizeof(TsvdBlock),

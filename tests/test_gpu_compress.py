@@ -244,3 +244,53 @@ def test_inverse_distance_2d_vs_oracle(gpu):
     assert H.rel_err(dy.numpy(), a @ x) < 1e-7
     H.oracle_free(ot)
     lib.free_tree_hodlr(C.byref(t))
+
+
+def test_wide_core_known_spectrum(gpu):
+    """Blocks of exact numerical rank 300 with a known geometric spectrum,
+    sigma_i / sigma_0 = 10^(-i/20.3): the relative rule keeps i <= 162 (the
+    nearest singular values sit 5-7 % either side of thr * sigma_0).  The
+    sketch widens to 256 columns, so the SVD core runs the block Jacobi
+    sweeps.  Checked: ranks, singular values (U carries them), and the
+    truncation error against the exact truncated SVD."""
+    from hmat_lib_amd import device as D
+    _say("case: wide core m=600 rank 300")
+    rng = np.random.default_rng(11)
+    mb, r, thr = 600, 300, 1e-8
+    sig = 0.75 * 10.0 ** (-np.arange(r) / 20.3)
+    n = 2 * mb
+    a = np.zeros((n, n), order="F")
+    a[:mb, :mb] = rng.standard_normal((mb, mb))
+    a[mb:, mb:] = rng.standard_normal((mb, mb))
+    facts = []
+    for (r0, c0) in ((0, mb), (mb, 0)):
+        qa, _ = np.linalg.qr(rng.standard_normal((mb, r)))
+        qb, _ = np.linalg.qr(rng.standard_normal((mb, r)))
+        a[r0:r0 + mb, c0:c0 + mb] = (qa * sig) @ qb.T
+        facts.append((qa, qb))
+    da = D.DeviceArray.from_numpy(a)
+    dt = D.DeviceTree.compress(da, n, 1, thr)
+    st = dt.stats
+    assert st.max_sketch >= 256
+    lib = capi.hmat()
+    ierr = C.c_int(0)
+    t = lib.allocate_tree_monolithic(1, C.byref(ierr))
+    dt.download(lib, t)
+    ft = T.flatten(t)
+    k = 163
+    for key in ((1, 0), (1, 1)):
+        u, v = ft.blocks[key]
+        assert u.shape[1] == k, (key, u.shape)
+        got = np.linalg.norm(u, axis=0)
+        np.testing.assert_allclose(got[:120], sig[:120], rtol=1e-6)
+        assert np.max(np.abs(got - sig[:k])) <= 1e-9 * sig[0]
+        np.testing.assert_allclose(np.linalg.norm(v, axis=0), 1.0, atol=1e-9)
+        (r0, r1), (c0, c1) = ft.block_ranges(*key)
+        blk = a[r0:r1, c0:c1]
+        qa, qb = facts[0] if key == (1, 0) else facts[1]
+        best = (qa[:, :k] * sig[:k]) @ qb[:, :k].T
+        assert np.linalg.norm(u @ v.T - best, 2) <= 2 * thr * sig[0]
+        assert np.linalg.norm(u @ v.T - blk, 2) <= 2 * thr * sig[0]
+    lib.free_tree_hodlr(C.byref(t))
+    dt.free()
+    da.free()

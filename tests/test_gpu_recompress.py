@@ -113,3 +113,43 @@ def test_random_vs_oracle(gpu, seed):
         assert nd.s == k
         uo, vo = _read(nd)
         np.testing.assert_allclose(nu[i] @ nv[i].T, uo @ vo.T, atol=1e-9)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_random_wide_vs_oracle(gpu, seed):
+    """Nodes with s = 70..200 redundant columns: the QR runs several 64-column
+    panels (block Gram-Schmidt twice over TSQR panels) and the s x s core the
+    block Jacobi sweeps; rank and U V^T vs the oracle's dgeqrt/dgesdd path."""
+    from hmat_lib_amd import device as D
+    from oracle import oracle as O
+    rng = np.random.default_rng(100 + seed)
+    us, vs, exp = [], [], []
+    for _ in range(3):
+        m, n = int(rng.integers(260, 420)), int(rng.integers(260, 420))
+        s = int(rng.integers(70, 200))
+        r = int(rng.integers(s // 2, s))
+        qa, _ = np.linalg.qr(rng.standard_normal((m, r)))
+        qb, _ = np.linalg.qr(rng.standard_normal((n, r)))
+        # absolute rule sigma_i > 1e-8 keeps the first kk; the nearest
+        # singular values sit a factor 10^0.2 either side of the cut
+        kk = int(rng.integers(r // 3, r - 5))
+        sig = np.concatenate([np.logspace(0, -7.8, kk), np.logspace(-8.2, -13, r - kk)])
+        core = qa * sig @ qb.T
+        mix = rng.standard_normal((r, s))
+        u = qa * sig @ mix
+        v = qb @ np.linalg.pinv(mix).T
+        us.append(np.asfortranarray(u))
+        vs.append(np.asfortranarray(v))
+        exp.append((core, kk))
+    nu, nv = D.recompress_batch(us, vs, 1e-8, method=0)
+    for i in range(len(us)):
+        core, k = exp[i]
+        assert nu[i].shape[1] == k, (i, nu[i].shape, k)
+        assert np.linalg.norm(nu[i] @ nv[i].T - core, 2) < 2e-8
+        nd = _node(us[i], vs[i])
+        ierr = C.c_int(0)
+        O.oracle().orc_recompress(C.byref(nd), min(us[i].shape[0], vs[i].shape[0]), 1e-8,
+                                  C.byref(ierr))
+        assert nd.s == k
+        uo, vo = _read(nd)
+        np.testing.assert_allclose(nu[i] @ nv[i].T, uo @ vo.T, atol=1e-9)
