@@ -170,6 +170,15 @@ def _declare_hip_api(lib):
         "hm_hip_recompress_batch": ([c_int, PInt, PInt, PInt, POINTER(V), POINTER(V),
                                      c_double, c_int, PInt, V], c_int),
         "hm_hip_selftest_mfma": ([PDouble], c_int),
+        "hm_hip_compress_batch": ([c_int, POINTER(V), POINTER(c_long), POINTER(c_int),
+                                   POINTER(c_int), c_double, V, POINTER(V), V, V], c_int),
+        "hm_hip_batch_create": ([c_int, POINTER(V), c_int, POINTER(c_int), POINTER(V),
+                                 POINTER(c_long), POINTER(V), POINTER(c_long), POINTER(V)],
+                                c_int),
+        "hm_hip_batch_tune": ([V, V], c_int),
+        "hm_hip_batch_run": ([V, c_int, V], c_int),
+        "hm_hip_batch_variants": ([V, POINTER(c_int), c_int], c_int),
+        "hm_hip_batch_destroy": ([V], c_int),
         "hm_hip_audit_qr": ([c_int, POINTER(c_int), POINTER(c_int), c_int, C.c_char_p,
                              c_int], c_int),
         "hm_hip_matmat_variant": ([V, c_int, c_int], c_int),

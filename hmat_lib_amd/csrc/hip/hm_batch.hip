@@ -1,0 +1,218 @@
+// Batched products over many independent HODLR trees of uneven size, rank
+// and RHS count (FP64).
+//
+// The reference multiplies one tree at a time, two BLAS calls per block
+// (src/dense_algebra.c:82-429).  A batch here is one launch program, built
+// once for fixed operands:
+//
+//   W       W_b = F_in^T X_src  for every block of every tree
+//   D       Y_c = D_c X_c       (D_c^T for H^T X) for every leaf of every tree
+//   level l Y_rows(b) += F_out W_b for every block of level l of every tree
+//
+// so a product over the whole batch is 2 + max height launches of the
+// persistent batched GEMM kernel, whatever the number of trees.  Blocks of
+// one level write disjoint rows of their tree's Y, so each level is one
+// launch; the launches are ordered on the stream.  Tile shapes of every
+// launch are chosen by timing (program_tune), and the program can be
+// captured once into a HIP graph and replayed.
+#include <algorithm>
+#include <cstring>
+
+#include "hm_prog.h"
+
+struct hm_batch {
+  std::vector<hm_dtree *> trees;
+  int trans = 0;
+  hm::Program prog;
+  char *arena = nullptr;
+  bool tuned = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  std::vector<hm::MemRange> inputs;
+};
+
+namespace hm {
+
+struct BatchOperands {
+  int R;
+  const double *X;
+  long ldx;
+  double *Y;
+  long ldy;
+};
+
+static int build_batch(Program &p, Arena &ar, const std::vector<hm_dtree *> &trees, int trans,
+                       const std::vector<BatchOperands> &ops) {
+  std::vector<GemmJob> wj, dj;
+  int maxh = 0;
+  for (auto *t : trees) maxh = std::max(maxh, t->height);
+  std::vector<std::vector<GemmJob>> lj(maxh + 1);
+  for (size_t i = 0; i < trees.size(); i++) {
+    const hm_dtree *t = trees[i];
+    const BatchOperands &o = ops[i];
+    if (o.R <= 0) continue;
+    std::vector<double *> W(t->nblock, nullptr);
+    for (int g = 0; g < t->nblock; g++) {
+      const BlockMeta &b = t->blk[g];
+      if (b.s <= 0) continue;
+      W[g] = ar.take<double>((size_t)b.s * o.R);
+      const double *F = trans ? t->d_U + b.uoff : t->d_V + b.voff;
+      const int K = trans ? b.m : b.n;
+      const int src = trans ? b.r0 : b.c0;
+      // W_b (s x R, ld s) = F^T X[src : src + K, :]
+      wj.push_back({F, o.X + src, W[g], nullptr, std::max(K, 1), o.ldx, b.s, b.s, o.R, K, 1, 0,
+                    0, 1.0, 0.0});
+      const double *G = trans ? t->d_V + b.voff : t->d_U + b.uoff;
+      const int rows = trans ? b.n : b.m;
+      const int dst = trans ? b.c0 : b.r0;
+      lj[b.level].push_back({G, W[g], o.Y + dst, nullptr, std::max(rows, 1), b.s, o.ldy, rows,
+                             o.R, b.s, 1, 0, 0, 1.0, 1.0});
+    }
+    for (int c = 0; c < t->nleaf; c++) {
+      const int lo = t->leaf_off[c], mc = t->leaf_off[c + 1] - lo;
+      if (mc <= 0) continue;
+      dj.push_back({t->d_diag + t->diag_off[c], o.X + lo, o.Y + lo, nullptr, mc, o.ldx, o.ldy,
+                    mc, o.R, mc, 1, 0, 0, 1.0, 0.0});
+    }
+  }
+  HM_OK(gemm_add(p, ar, wj, true, false, "batch W", -1));
+  HM_OK(gemm_add(p, ar, dj, trans != 0, false, "batch diag", -1));
+  for (int l = maxh; l >= 1; l--) HM_OK(gemm_add(p, ar, lj[l], false, false, "batch level", -1));
+  return SUCCESS;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" {
+
+int hm_hip_batch_create(int ntrees, hm_dtree *const *trees, int trans, const int *R,
+                        const double *const *d_X, const long *ldx, double *const *d_Y,
+                        const long *ldy, hm_batch **out) {
+  HM_OK(ensure_init());
+  if (!out || ntrees < 0 || (ntrees > 0 && (!trees || !R || !d_X || !ldx || !d_Y || !ldy))) {
+    set_error("hm_hip_batch_create: NULL argument");
+    return INPUT_ERROR;
+  }
+  *out = nullptr;
+  std::vector<hm_dtree *> tv(trees, trees + ntrees);
+  std::vector<BatchOperands> ops(ntrees);
+  std::vector<MemRange> inputs;
+  for (int i = 0; i < ntrees; i++) {
+    const hm_dtree *t = tv[i];
+    if (!t || R[i] < 0 || (R[i] > 0 && (!d_X[i] || !d_Y[i] || ldx[i] < std::max(1, t->n) ||
+                                        ldy[i] < std::max(1, t->n)))) {
+      set_error("hm_hip_batch_create: tree %d: bad operands", i);
+      return INPUT_ERROR;
+    }
+    ops[i] = {R[i], d_X[i], ldx[i], d_Y[i], ldy[i]};
+    if (R[i] > 0) {
+      inputs.push_back({d_X[i], ((size_t)(R[i] - 1) * ldx[i] + t->n) * sizeof(double)});
+      inputs.push_back({d_Y[i], ((size_t)(R[i] - 1) * ldy[i] + t->n) * sizeof(double)});
+    }
+    long long u = 0, v = 0;
+    for (const auto &b : t->blk) {
+      u += (long long)b.m * b.s;
+      v += (long long)b.n * b.s;
+    }
+    inputs.push_back({t->d_U, (size_t)u * sizeof(double)});
+    inputs.push_back({t->d_V, (size_t)v * sizeof(double)});
+    inputs.push_back({t->d_diag, (size_t)t->diag_off[t->nleaf] * sizeof(double)});
+  }
+  // counting pass + audit, then the real program on its own arena
+  Program dry;
+  Arena count;
+  dry.record = true;
+  HM_OK(build_batch(dry, count, tv, trans, ops));
+  for (const auto &r : inputs) dry.allow(r.lo, r.bytes);
+  dry.allow(Arena::fake_base(), count.off + 256);
+  std::string msg;
+  if (dry.audit(&msg) != SUCCESS) {
+    set_error("hm_hip_batch_create: audit: %s", msg.c_str());
+    return INPUT_ERROR;
+  }
+  hm_batch *b = new hm_batch();
+  b->trees = tv;
+  b->trans = trans ? 1 : 0;
+  Arena ar;
+  ar.counting = false;
+  ar.cap = count.off + 256;
+  int rc = dalloc(&ar.base, ar.cap);
+  if (rc == SUCCESS) {
+    b->arena = ar.base;
+    rc = build_batch(b->prog, ar, tv, trans, ops);
+  }
+  if (rc == SUCCESS) rc = b->prog.upload(stream_of(nullptr));
+  if (rc == SUCCESS) {
+    hipError_t e = hipStreamSynchronize(stream_of(nullptr));
+    if (e != hipSuccess) {
+      set_error("hm_hip_batch_create: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
+    }
+  }
+  if (rc != SUCCESS) {
+    dfree(b->arena);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return SUCCESS;
+}
+
+int hm_hip_batch_tune(hm_batch *b, void *stream) {
+  if (!b) return INPUT_ERROR;
+  HM_OK(ensure_init());
+  hipStream_t s = stream_of(stream);
+  HM_OK(program_tune(b->prog, s));
+  b->tuned = true;
+  // a captured graph holds the old shapes
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  HM_HIP(hipStreamSynchronize(s));
+  return SUCCESS;
+}
+
+int hm_hip_batch_run(hm_batch *b, int use_graph, void *stream) {
+  if (!b) return INPUT_ERROR;
+  HM_OK(ensure_init());
+  hipStream_t s = stream_of(stream);
+  if (!use_graph) return b->prog.run(s);
+  if (!b->gexec) {
+    HM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = b->prog.run(s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != SUCCESS || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc == SUCCESS) set_error("hm_hip_batch_run: capture: %s", hipGetErrorString(e));
+      return rc != SUCCESS ? rc : ALLOCATION_FAILURE;
+    }
+    b->graph = g;
+    HM_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  }
+  HM_HIP(hipGraphLaunch(b->gexec, s));
+  return SUCCESS;
+}
+
+int hm_hip_batch_variants(const hm_batch *b, int *out, int cap) {
+  if (!b) return -1;
+  const int n = (int)b->prog.tunables.size();
+  for (int i = 0; i < n && i < cap; i++) out[i] = *b->prog.tunables[i].choice;
+  return n;
+}
+
+int hm_hip_batch_destroy(hm_batch *b) {
+  if (!b) return SUCCESS;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipStreamSynchronize(stream_of(nullptr));
+  dfree(b->arena);
+  delete b;
+  return SUCCESS;
+}
+
+}  // extern "C"

@@ -109,36 +109,31 @@ int hm_hip_fill_kernel_matrix(int kind, int n, int dim, const double *d_pts,
   return SUCCESS;
 }
 
-int hm_hip_compress_dense(const double *d_A, long lda, int m, int height,
-                          const int *ms, double thr,
-                          const hm_compress_opts *opts_in, hm_dtree **out,
-                          hm_compress_stats *stats, void *stream) {
+}  // extern "C"
+
+namespace hm {
+
+// Tree skeleton (leaf offsets, block geometry) of an m x m matrix and one
+// truncated-SVD entry per off-diagonal block (uid_base distinguishes the
+// random sketches of different matrices).
+static int plan_tree(const double *d_A, long lda, int m, int height, const int *ms,
+                     double thr, const hm_compress_opts &opts, unsigned long long uid_base,
+                     hm_dtree **out, std::vector<TsvdBlock> &tb) {
   *out = nullptr;
-  HM_OK(ensure_init());
-  if (!d_A || height < 1 || height > 24 || m < 0 || lda < std::max(1, m)) {
-    set_error("hm_hip_compress_dense: bad arguments (m=%d height=%d lda=%ld)",
-              m, height, lda);
-    return INPUT_ERROR;
-  }
-  hm_compress_opts opts;
-  hm_hip_compress_opts_default(&opts);
-  if (opts_in) opts = *opts_in;
-  hipStream_t s = stream_of(stream);
-  auto t0 = std::chrono::steady_clock::now();
   const int nleaf = 1 << height;
   std::vector<int> leaves;
   if (ms) {
     long long sum = 0;
     for (int c = 0; c < nleaf; c++) {
       if (ms[c] < 0) {
-        set_error("hm_hip_compress_dense: negative leaf size");
+        set_error("compress: negative leaf size");
         return INPUT_ERROR;
       }
       leaves.push_back(ms[c]);
       sum += ms[c];
     }
     if (sum != m) {
-      set_error("hm_hip_compress_dense: leaf sizes sum to %lld, not %d", sum, m);
+      set_error("compress: leaf sizes sum to %lld, not %d", sum, m);
       return INPUT_ERROR;
     }
   } else {
@@ -151,7 +146,7 @@ int hm_hip_compress_dense(const double *d_A, long lda, int m, int height,
   t->leaf_off.assign(nleaf + 1, 0);
   for (int c = 0; c < nleaf; c++) t->leaf_off[c + 1] = t->leaf_off[c] + leaves[c];
   t->blk.resize(t->nblock);
-  std::vector<TsvdBlock> tb(t->nblock);
+  tb.assign(t->nblock, TsvdBlock());
   for (int l = 1; l <= height; l++) {
     const int span = 1 << (height - l);
     for (int j = 0; j < (1 << l); j++) {
@@ -176,17 +171,67 @@ int hm_hip_compress_dense(const double *d_A, long lda, int m, int height,
       x.ell = ell;
       x.rule = RULE_RELATIVE;
       x.thr = thr;
-      x.uid = (unsigned long long)g;
+      x.uid = uid_base + (unsigned long long)g;
     }
   }
+  *out = t;
+  return SUCCESS;
+}
+
+// Metadata upload, diagonal extraction (t->d_U, d_V already set).
+static int finish_tree(hm_dtree *t, const double *d_A, long lda, hipStream_t s) {
+  HM_OK(tree_finalize_meta(t, s));
+  HM_OK(dalloc(&t->d_diag, (size_t)t->diag_off[t->nleaf]));
+  hipLaunchKernelGGL(k_extract_diag, dim3(t->nleaf, 64), dim3(256), 0, s, d_A,
+                     (long long)lda, t->d_leaf_off, t->d_diag_off, t->d_diag);
+  HM_LAUNCH_CHECK();
+  return SUCCESS;
+}
+
+static void fill_stats(hm_compress_stats *stats, const TsvdStats &ts, double ms) {
+  if (!stats) return;
+  stats->sketch_rounds = ts.rounds;
+  stats->blocks_retried = ts.retried;
+  stats->max_sketch = ts.max_sketch;
+  stats->fallback_blocks = ts.capped;
+  stats->ms_total = ms;
+}
+
+static TsvdOptions tsvd_options(const hm_compress_opts &opts) {
   TsvdOptions to;
   to.sketch_max = opts.sketch_max;
   to.n_test = opts.n_test;
   to.accept = opts.accept;
   to.seed = opts.seed;
+  return to;
+}
+
+}  // namespace hm
+
+extern "C" {
+
+int hm_hip_compress_dense(const double *d_A, long lda, int m, int height,
+                          const int *ms, double thr,
+                          const hm_compress_opts *opts_in, hm_dtree **out,
+                          hm_compress_stats *stats, void *stream) {
+  *out = nullptr;
+  HM_OK(ensure_init());
+  if (!d_A || height < 1 || height > 24 || m < 0 || lda < std::max(1, m)) {
+    set_error("hm_hip_compress_dense: bad arguments (m=%d height=%d lda=%ld)",
+              m, height, lda);
+    return INPUT_ERROR;
+  }
+  hm_compress_opts opts;
+  hm_hip_compress_opts_default(&opts);
+  if (opts_in) opts = *opts_in;
+  hipStream_t s = stream_of(stream);
+  auto t0 = std::chrono::steady_clock::now();
+  hm_dtree *t = nullptr;
+  std::vector<TsvdBlock> tb;
+  HM_OK(plan_tree(d_A, lda, m, height, ms, thr, opts, 0, &t, tb));
   TsvdStats ts;
   double *up = nullptr, *vp = nullptr;
-  int rc = tsvd_run(tb, to, &up, &vp, &ts, nullptr, 0, s);
+  int rc = tsvd_run(tb, tsvd_options(opts), &up, &vp, &ts, nullptr, 0, s);
   if (rc != SUCCESS) {
     tree_free(t);
     return rc;
@@ -194,30 +239,110 @@ int hm_hip_compress_dense(const double *d_A, long lda, int m, int height,
   for (int g = 0; g < t->nblock; g++) t->blk[g].s = tb[g].k;
   t->d_U = up;
   t->d_V = vp;
-  rc = tree_finalize_meta(t, s);
-  if (rc == SUCCESS) rc = dalloc(&t->d_diag, (size_t)t->diag_off[nleaf]);
+  rc = finish_tree(t, d_A, lda, s);
+  if (rc == SUCCESS) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error("hm_hip_compress_dense: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
+    }
+  }
   if (rc != SUCCESS) {
     tree_free(t);
     return rc;
   }
-  hipLaunchKernelGGL(k_extract_diag, dim3(nleaf, 64), dim3(256), 0, s, d_A,
-                     (long long)lda, t->d_leaf_off, t->d_diag_off, t->d_diag);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    set_error("hm_hip_compress_dense: %s", hipGetErrorString(e));
-    tree_free(t);
-    return ALLOCATION_FAILURE;
-  }
   auto t1 = std::chrono::steady_clock::now();
-  if (stats) {
-    stats->sketch_rounds = ts.rounds;
-    stats->blocks_retried = ts.retried;
-    stats->max_sketch = ts.max_sketch;
-    stats->fallback_blocks = ts.capped;
-    stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  }
+  fill_stats(stats, ts, std::chrono::duration<double, std::milli>(t1 - t0).count());
   *out = t;
+  return SUCCESS;
+}
+
+int hm_hip_compress_batch(int nmat, const double *const *d_A, const long *lda, const int *m,
+                          const int *height, double thr, const hm_compress_opts *opts_in,
+                          hm_dtree **out, hm_compress_stats *stats, void *stream) {
+  HM_OK(ensure_init());
+  if (nmat < 0 || (nmat > 0 && (!d_A || !lda || !m || !height || !out))) {
+    set_error("hm_hip_compress_batch: NULL argument");
+    return INPUT_ERROR;
+  }
+  for (int i = 0; i < nmat; i++) {
+    out[i] = nullptr;
+    if (!d_A[i] || height[i] < 1 || height[i] > 24 || m[i] < 0 || lda[i] < std::max(1, m[i])) {
+      set_error("hm_hip_compress_batch: matrix %d: bad arguments (m=%d height=%d)", i, m[i],
+                height[i]);
+      return INPUT_ERROR;
+    }
+  }
+  hm_compress_opts opts;
+  hm_hip_compress_opts_default(&opts);
+  if (opts_in) opts = *opts_in;
+  if (opts.rank_hint) {
+    set_error("hm_hip_compress_batch: rank hints are per tree; not supported here");
+    return INPUT_ERROR;
+  }
+  hipStream_t s = stream_of(stream);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<hm_dtree *> trees(nmat, nullptr);
+  std::vector<TsvdBlock> all;
+  std::vector<size_t> first(nmat + 1, 0);
+  auto fail = [&](int rc) {
+    for (auto *t : trees) tree_free(t);
+    return rc;
+  };
+  for (int i = 0; i < nmat; i++) {
+    std::vector<TsvdBlock> tb;
+    int rc = plan_tree(d_A[i], lda[i], m[i], height[i], nullptr, thr, opts,
+                       (unsigned long long)i << 32, &trees[i], tb);
+    if (rc != SUCCESS) return fail(rc);
+    first[i] = all.size();
+    all.insert(all.end(), tb.begin(), tb.end());
+  }
+  first[nmat] = all.size();
+  TsvdStats ts;
+  double *up = nullptr, *vp = nullptr;
+  int rc = tsvd_run(all, tsvd_options(opts), &up, &vp, &ts, nullptr, 0, s);
+  if (rc != SUCCESS) return fail(rc);
+  // per-tree pools: the tree's blocks are contiguous in the batch pools
+  for (int i = 0; i < nmat && rc == SUCCESS; i++) {
+    hm_dtree *t = trees[i];
+    long long u = 0, v = 0;
+    for (int g = 0; g < t->nblock; g++) {
+      const TsvdBlock &b = all[first[i] + g];
+      t->blk[g].s = b.k;
+      u += (long long)b.m * b.k;
+      v += (long long)b.n * b.k;
+    }
+    rc = dalloc(&t->d_U, (size_t)u);
+    if (rc == SUCCESS) rc = dalloc(&t->d_V, (size_t)v);
+    if (rc != SUCCESS) break;
+    const TsvdBlock &b0 = all[first[i]];
+    {
+      hipError_t e = hipSuccess;
+      if (u > 0)
+        e = hipMemcpyAsync(t->d_U, b0.uout, u * sizeof(double), hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess && v > 0)
+        e = hipMemcpyAsync(t->d_V, b0.vout, v * sizeof(double), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) {
+        set_error("hm_hip_compress_batch: %s", hipGetErrorString(e));
+        rc = ALLOCATION_FAILURE;
+        break;
+      }
+    }
+    rc = finish_tree(t, d_A[i], lda[i], s);
+  }
+  if (rc == SUCCESS) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error("hm_hip_compress_batch: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
+    }
+  }
+  dfree(up);
+  dfree(vp);
+  if (rc != SUCCESS) return fail(rc);
+  auto t1 = std::chrono::steady_clock::now();
+  fill_stats(stats, ts, std::chrono::duration<double, std::milli>(t1 - t0).count());
+  for (int i = 0; i < nmat; i++) out[i] = trees[i];
   return SUCCESS;
 }
 

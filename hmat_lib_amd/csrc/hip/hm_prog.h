@@ -11,6 +11,7 @@
 
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <string>
 
 #include "hm_common.h"
@@ -38,10 +39,19 @@ struct MemRange {
   size_t bytes;
 };
 
+// An op with several launch shapes (tile variants); the autotuner times
+// each on the live operands and keeps the fastest in *choice.
+struct Tunable {
+  size_t op;                  // index into Program::ops
+  int nvariants;
+  std::shared_ptr<int> choice;
+};
+
 struct Program {
   std::vector<unsigned char> blob;
   std::vector<std::function<void(hipStream_t, const unsigned char *)>> ops;
   std::vector<std::string> names;
+  std::vector<Tunable> tunables;
   unsigned char *d_blob = nullptr;
   // audit data: address ranges touched by items / buffers they may touch
   bool record = false;
@@ -86,8 +96,19 @@ struct GemmJob {
   double alpha, beta;
 };
 
+// Tile variants (TM x TN): 0 = 64 x 64, 1 = 64 x 16 (narrow RHS),
+// 2 = 16 x 64 (few rows, e.g. rank-1 factors).  variant < 0 registers the
+// op as tunable (starts at 0); the kernel is persistent: at most
+// GEMM_MAX_GRID workgroups loop over the tiles.
+static constexpr int GEMM_VARIANTS = 3;
+static constexpr int GEMM_MAX_GRID = 4096;
 int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
-             const char *name);
+             const char *name, int variant = 0);
+
+// Times every variant of every tunable op (3 runs each after a warm-up)
+// and keeps the fastest.  Runs ops out of order: outputs are garbage
+// until the program is run again from the start.
+int program_tune(Program &p, hipStream_t s);
 
 // dst(rows x cols, ldd) = src(rows x cols, lds), or src^T when trans
 // (then dst is cols x rows); src == NULL fills dst with zeros.

@@ -56,14 +56,14 @@ int Program::run(hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ GEMM
-typedef MfmaTile<2, 2, 2, 2, 16> GTile;  // 64 x 64, K-chunk 16
+typedef MfmaTile<2, 2, 2, 2, 16> GTile0;  // 64 x 64
+typedef MfmaTile<4, 1, 1, 1, 16> GTile1;  // 64 x 16
+typedef MfmaTile<1, 4, 1, 1, 16> GTile2;  // 16 x 64
+static_assert(GTile0::NT == 256 && GTile1::NT == 256 && GTile2::NT == 256, "256 threads");
 
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm(const GemmJob *jobs, const int4 *tiles) {
-  __shared__ double lds[GTile::LDS_DOUBLES];
-  const int4 t = tiles[blockIdx.x];
-  const GemmJob J = jobs[t.x];
-  const int i0 = t.y * GTile::TM, j0 = t.z * GTile::TN;
+template <class Tile, bool TA, bool TB>
+__device__ __forceinline__ void gemm_tile(const GemmJob &J, int4 t, double *lds) {
+  const int i0 = t.y * Tile::TM, j0 = t.z * Tile::TN;
   const int kb = t.w * J.kseg;
   const int kl = min(J.K, kb + J.kseg) - kb;
   const double *A = J.A + (TA ? (long long)kb : (long long)kb * J.lda);
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob *jobs, const int4 *t
     if (k >= kl || j0 + j >= N) return 0.0;
     return TB ? B[(j0 + j) + (long long)k * ldb] : B[k + (long long)(j0 + j) * ldb];
   };
-  GTile tile;
+  Tile tile;
   tile.zero();
   tile.template run<!TA, TB>(lds, kl, la, lb);
   if (J.ksplit == 1) {
@@ -97,6 +97,17 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob *jobs, const int4 *t
       if (i0 + i < M && j0 + j < N) P[(i0 + i) + (long long)(j0 + j) * M] = v;
     });
   }
+  __syncthreads();
+}
+
+template <class Tile, bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(const GemmJob *jobs, const int4 *tiles,
+                                              int ntiles) {
+  __shared__ double lds[Tile::LDS_DOUBLES];
+  for (int b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const int4 t = tiles[b];
+    gemm_tile<Tile, TA, TB>(jobs[t.x], t, lds);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_gemm_reduce(const GemmJob *jobs, const int *list) {
@@ -112,22 +123,31 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const GemmJob *jobs, const 
   }
 }
 
+template <class Tile>
+static void launch_gemm(bool ta, bool tb, unsigned grid, hipStream_t s, const GemmJob *J,
+                        const int4 *T, int n) {
+  if (!ta && !tb) hipLaunchKernelGGL((k_gemm<Tile, false, false>), dim3(grid), dim3(256), 0, s, J, T, n);
+  if (!ta && tb) hipLaunchKernelGGL((k_gemm<Tile, false, true>), dim3(grid), dim3(256), 0, s, J, T, n);
+  if (ta && !tb) hipLaunchKernelGGL((k_gemm<Tile, true, false>), dim3(grid), dim3(256), 0, s, J, T, n);
+  if (ta && tb) hipLaunchKernelGGL((k_gemm<Tile, true, true>), dim3(grid), dim3(256), 0, s, J, T, n);
+}
+
+static const int kTileTM[GEMM_VARIANTS] = {GTile0::TM, GTile1::TM, GTile2::TM};
+static const int kTileTN[GEMM_VARIANTS] = {GTile0::TN, GTile1::TN, GTile2::TN};
+
 int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
-             const char *name) {
-  std::vector<int4> tiles;
-  std::vector<int> split;
+             const char *name, int variant) {
+  std::vector<int> split, live;
   long long max_e = 0;
   for (size_t q = 0; q < jobs.size(); q++) {
     GemmJob &J = jobs[q];
     if (J.M <= 0 || J.N <= 0) continue;
-    const int mt = (J.M + GTile::TM - 1) / GTile::TM;
-    const int nt = (J.N + GTile::TN - 1) / GTile::TN;
+    live.push_back((int)q);
+    const int mt = (J.M + 63) / 64, nt = (J.N + 63) / 64;
     int ks = 1;
     if (J.K >= 4096 && mt * nt < 128) ks = std::min(64, J.K / 2048);
-    J.ksplit = ks;
     J.kseg = ks > 1 ? ((J.K + ks - 1) / ks + 15) / 16 * 16 : std::max(J.K, 1);
-    ks = (J.K + J.kseg - 1) / J.kseg;
-    if (ks < 1) ks = 1;
+    ks = std::max(1, (J.K + J.kseg - 1) / J.kseg);
     J.ksplit = ks;
     J.part = nullptr;
     if (ks > 1) {
@@ -136,10 +156,6 @@ int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
       max_e = std::max(max_e, (long long)J.M * J.N);
       p.touch(J.part, (size_t)ks * J.M * J.N * sizeof(double), name);
     }
-    for (int w = 0; w < ks; w++)
-      for (int a = 0; a < mt; a++)
-        for (int b = 0; b < nt; b++) tiles.push_back(make_int4((int)q, a, b, w));
-    // ranges read and written
     if (J.K > 0) {
       const size_t abytes = ta ? ((size_t)(J.M - 1) * J.lda + J.K) : ((size_t)(J.K - 1) * J.lda + J.M);
       const size_t bbytes = tb ? ((size_t)(J.K - 1) * J.ldb + J.N) : ((size_t)(J.N - 1) * J.ldb + J.K);
@@ -148,16 +164,36 @@ int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
     }
     p.touch(J.C, ((size_t)(J.N - 1) * J.ldc + J.M) * sizeof(double), name);
   }
-  if (tiles.empty()) return SUCCESS;
-  const size_t joff = p.push(jobs), toff = p.push(tiles);
-  const unsigned nt = (unsigned)tiles.size();
+  if (live.empty()) return SUCCESS;
+  const size_t joff = p.push(jobs);
+  size_t toff[GEMM_VARIANTS];
+  int ntile[GEMM_VARIANTS];
+  for (int v = 0; v < GEMM_VARIANTS; v++) {
+    std::vector<int4> tiles;
+    for (int q : live) {
+      const GemmJob &J = jobs[q];
+      const int mt = (J.M + kTileTM[v] - 1) / kTileTM[v];
+      const int nt = (J.N + kTileTN[v] - 1) / kTileTN[v];
+      for (int w = 0; w < J.ksplit; w++)
+        for (int a = 0; a < mt; a++)
+          for (int b = 0; b < nt; b++) tiles.push_back(make_int4(q, a, b, w));
+    }
+    toff[v] = p.push(tiles);
+    ntile[v] = (int)tiles.size();
+  }
+  auto choice = std::make_shared<int>(variant < 0 ? 0 : std::min(variant, GEMM_VARIANTS - 1));
+  if (variant < 0) p.tunables.push_back({p.ops.size(), GEMM_VARIANTS, choice});
+  const size_t t0 = toff[0], t1 = toff[1], t2 = toff[2];
+  const int n0 = ntile[0], n1 = ntile[1], n2 = ntile[2];
   p.add(name, [=](hipStream_t s, const unsigned char *b) {
     const GemmJob *J = reinterpret_cast<const GemmJob *>(b + joff);
-    const int4 *T = reinterpret_cast<const int4 *>(b + toff);
-    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), dim3(nt), dim3(256), 0, s, J, T);
-    if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), dim3(nt), dim3(256), 0, s, J, T);
-    if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), dim3(nt), dim3(256), 0, s, J, T);
-    if (ta && tb) hipLaunchKernelGGL((k_gemm<true, true>), dim3(nt), dim3(256), 0, s, J, T);
+    const int v = *choice;
+    const int n = v == 0 ? n0 : (v == 1 ? n1 : n2);
+    const int4 *T = reinterpret_cast<const int4 *>(b + (v == 0 ? t0 : (v == 1 ? t1 : t2)));
+    const unsigned grid = (unsigned)std::min(n, GEMM_MAX_GRID);
+    if (v == 0) launch_gemm<GTile0>(ta, tb, grid, s, J, T, n);
+    if (v == 1) launch_gemm<GTile1>(ta, tb, grid, s, J, T, n);
+    if (v == 2) launch_gemm<GTile2>(ta, tb, grid, s, J, T, n);
   });
   if (!split.empty()) {
     const size_t loff = p.push(split);
@@ -170,6 +206,44 @@ int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
     });
   }
   return SUCCESS;
+}
+
+int program_tune(Program &p, hipStream_t s) {
+  if (p.tunables.empty()) return SUCCESS;
+  hipEvent_t e0, e1;
+  HM_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("program_tune: event creation failed");
+    return ALLOCATION_FAILURE;
+  }
+  int rc = SUCCESS;
+  for (const Tunable &t : p.tunables) {
+    float best = 1e30f;
+    int bestv = 0;
+    for (int v = 0; v < t.nvariants && rc == SUCCESS; v++) {
+      *t.choice = v;
+      p.ops[t.op](s, p.d_blob);
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < 3; r++) p.ops[t.op](s, p.d_blob);
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) {
+        set_error("program_tune: variant %d of op %s failed", v, p.names[t.op].c_str());
+        rc = ALLOCATION_FAILURE;
+        break;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        bestv = v;
+      }
+    }
+    *t.choice = bestv;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 // -------------------------------------------------------- copy, norms

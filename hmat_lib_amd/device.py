@@ -182,6 +182,66 @@ def recompress_batch(us, vs, thr, method=-1):
     return outu, outv
 
 
+def compress_batch(mats, heights, thr, opts: capi.CompressOpts | None = None):
+    """Compress device matrices (DeviceArray, square) in one batched call;
+    returns (list of DeviceTree, stats)."""
+    n = len(mats)
+    pa = (C.c_void_p * n)(*[a.ptr.value for a in mats])
+    ld = (C.c_long * n)(*[a.shape[0] for a in mats])
+    ms = (C.c_int * n)(*[a.shape[0] for a in mats])
+    hs = (C.c_int * n)(*[int(h) for h in heights])
+    out = (C.c_void_p * n)()
+    st = capi.CompressStats()
+    capi.check(capi.hip().hm_hip_compress_batch(
+        n, pa, ld, ms, hs, thr, C.byref(opts) if opts is not None else None, out,
+        C.byref(st), None), "compress_batch")
+    trees = []
+    for i in range(n):
+        t = DeviceTree(C.c_void_p(out[i]))
+        t.stats = st
+        trees.append(t)
+    return trees, st
+
+
+class Batch:
+    """A batched product over many trees with fixed operands (hm_batch*)."""
+
+    def __init__(self, trees, xs, ys, trans=0):
+        n = len(trees)
+        self.trees, self.xs, self.ys = trees, xs, ys
+        pt = (C.c_void_p * n)(*[t.h.value for t in trees])
+        R = (C.c_int * n)(*[(x.shape[1] if len(x.shape) > 1 else 1) for x in xs])
+        px = (C.c_void_p * n)(*[x.ptr.value for x in xs])
+        py = (C.c_void_p * n)(*[y.ptr.value for y in ys])
+        lx = (C.c_long * n)(*[x.shape[0] for x in xs])
+        ly = (C.c_long * n)(*[y.shape[0] for y in ys])
+        self.h = C.c_void_p()
+        capi.check(capi.hip().hm_hip_batch_create(n, pt, trans, R, px, lx, py, ly,
+                                                  C.byref(self.h)), "batch_create")
+
+    def tune(self):
+        capi.check(capi.hip().hm_hip_batch_tune(self.h, None), "batch_tune")
+
+    def run(self, graph=False):
+        capi.check(capi.hip().hm_hip_batch_run(self.h, 1 if graph else 0, None), "batch_run")
+
+    def variants(self):
+        buf = (C.c_int * 64)()
+        n = capi.hip().hm_hip_batch_variants(self.h, buf, 64)
+        return list(buf[:min(n, 64)])
+
+    def free(self):
+        if self.h:
+            capi.hip().hm_hip_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def kernel_timing(enable: bool, only: str | None = None):
     capi.hip().hm_hip_kernel_timing_select(only.encode() if only else None)
     capi.hip().hm_hip_kernel_timing(1 if enable else 0)

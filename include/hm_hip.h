@@ -155,6 +155,37 @@ HM_HIP_API int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
                             double *const *d_v, double thr, int method,
                             int *k_out, void *stream);
 
+/* ---- batches of independent trees --------------------------------------
+ * Compress nmat dense matrices (device, column-major, not modified) in one
+ * batched truncated SVD over all their off-diagonal blocks; leaf sizes by
+ * halving, rank rule as hm_hip_compress_dense.  out[i] receives tree i;
+ * stats (optional) covers the whole batch.  Replaces nmat calls of
+ * dense_to_tree_hodlr (src/constructors.c:629-704). */
+HM_HIP_API int hm_hip_compress_batch(int nmat, const double *const *d_A, const long *lda,
+                                     const int *m, const int *height, double thr,
+                                     const hm_compress_opts *opts, hm_dtree **out,
+                                     hm_compress_stats *stats, void *stream);
+
+/* A batched product Y_i = op(H_i) X_i over ntrees trees with R[i] columns
+ * each (X_i, Y_i device, column-major, leading dimensions ldx[i], ldy[i]),
+ * fixed at creation: one launch program of 2 + max height batched-GEMM
+ * launches.  The batch keeps pointers to the trees and operands (it owns
+ * neither).  Replaces per-tree loops of multiply_hodlr_dense /
+ * multiply_hodlr_transpose_dense (src/dense_algebra.c:361-648). */
+typedef struct hm_batch hm_batch;
+HM_HIP_API int hm_hip_batch_create(int ntrees, hm_dtree *const *trees, int trans, const int *R,
+                                   const double *const *d_X, const long *ldx,
+                                   double *const *d_Y, const long *ldy, hm_batch **out);
+/* Time every tile variant of every launch on the live operands and keep the
+ * fastest.  Leaves the Y_i undefined until the next run. */
+HM_HIP_API int hm_hip_batch_tune(hm_batch *b, void *stream);
+/* Run the product; use_graph = 1 captures it into a HIP graph on first use
+ * and replays the graph afterwards. */
+HM_HIP_API int hm_hip_batch_run(hm_batch *b, int use_graph, void *stream);
+/* Tile variant chosen per launch (up to cap); returns the launch count. */
+HM_HIP_API int hm_hip_batch_variants(const hm_batch *b, int *out, int cap);
+HM_HIP_API int hm_hip_batch_destroy(hm_batch *b);
+
 /* ---- diagnostics -------------------------------------------------------
  * One v_mfma_f64_16x16x4_f64 with A(i,k) = 1 + i + 16 k and
  * B(k,j) = 1 + 2 j - k, read back through the lane layout the kernels
