@@ -80,13 +80,17 @@ class TreeInfo(C.Structure):
 class CompressOpts(C.Structure):
     _fields_ = [("sketch_init", c_int), ("sketch_max", c_int), ("n_test", c_int),
                 ("seed", C.c_ulonglong), ("rank_hint", PInt),
-                ("oversample", c_int), ("accept", c_double)]
+                ("oversample", c_int), ("accept", c_double),
+                ("rank_model", c_void_p), ("block_features", POINTER(C.c_float)),
+                ("n_extra", c_int), ("margin", c_double), ("first_sigma", POINTER(c_double)),
+                ("need_out", PInt)]
 
 
 class CompressStats(C.Structure):
     _fields_ = [("sketch_rounds", c_int), ("blocks_retried", c_int),
                 ("max_sketch", c_int), ("fallback_blocks", c_int),
-                ("ms_total", c_double)]
+                ("ms_total", c_double), ("predicted_blocks", c_int),
+                ("predicted_failed", c_int)]
 
 
 def dptr(a: np.ndarray):
@@ -170,6 +174,18 @@ def _declare_hip_api(lib):
         "hm_hip_recompress_batch": ([c_int, PInt, PInt, PInt, POINTER(V), POINTER(V),
                                      c_double, c_int, PInt, V], c_int),
         "hm_hip_selftest_mfma": ([PDouble], c_int),
+        "hm_hip_mlp_create": ([c_int, c_int, C.c_ulonglong, POINTER(V)], c_int),
+        "hm_hip_mlp_destroy": ([V], c_int),
+        "hm_hip_mlp_nparams": ([V], C.c_longlong),
+        "hm_hip_mlp_get_params": ([V, POINTER(C.c_float)], c_int),
+        "hm_hip_mlp_set_params": ([V, POINTER(C.c_float)], c_int),
+        "hm_hip_mlp_forward": ([V, V, c_int, V, V], c_int),
+        "hm_hip_mlp_gradient": ([V, V, V, c_int, POINTER(C.c_float), POINTER(C.c_float), V],
+                                c_int),
+        "hm_hip_mlp_train": ([V, V, V, c_int, c_int, c_int, C.c_float, C.c_ulonglong,
+                              POINTER(C.c_float), V], c_int),
+        "hm_hip_rank_features": ([c_int, c_int, POINTER(C.c_float), POINTER(c_double), c_double,
+                                  POINTER(C.c_float)], c_int),
         "hm_hip_compress_batch": ([c_int, POINTER(V), POINTER(c_long), POINTER(c_int),
                                    POINTER(c_int), c_double, V, POINTER(V), V, V], c_int),
         "hm_hip_batch_create": ([c_int, POINTER(V), c_int, POINTER(c_int), POINTER(V),
@@ -179,6 +195,7 @@ def _declare_hip_api(lib):
         "hm_hip_batch_run": ([V, c_int, V], c_int),
         "hm_hip_batch_variants": ([V, POINTER(c_int), c_int], c_int),
         "hm_hip_batch_destroy": ([V], c_int),
+        "hm_hip_kernel_timing_dump": ([C.c_char_p, c_int], c_int),
         "hm_hip_audit_qr": ([c_int, POINTER(c_int), POINTER(c_int), c_int, C.c_char_p,
                              c_int], c_int),
         "hm_hip_matmat_variant": ([V, c_int, c_int], c_int),

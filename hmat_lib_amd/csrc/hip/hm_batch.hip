@@ -183,7 +183,7 @@ int hm_hip_batch_run(hm_batch *b, int use_graph, void *stream) {
   if (!use_graph) return b->prog.run(s);
   if (!b->gexec) {
     HM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = b->prog.run(s);
+    int rc = b->prog.run(s, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc != SUCCESS || e != hipSuccess) {

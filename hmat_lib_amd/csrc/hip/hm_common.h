@@ -45,6 +45,8 @@ struct Timed {
 };
 Timed timing_begin(const char *name, hipStream_t s);
 void timing_end(const Timed &t, hipStream_t s);
+// Adds host-side milliseconds under a name (when timing is on).
+void timing_host(const char *name, double ms);
 
 // ------------------------------------------------------------ memory
 int dmalloc(void **p, size_t bytes);

@@ -91,6 +91,34 @@ void hm_hip_compress_opts_default(hm_compress_opts *o) {
   o->rank_hint = nullptr;
   o->oversample = 8;
   o->accept = 0.1;
+  o->rank_model = nullptr;
+  o->block_features = nullptr;
+  o->n_extra = 0;
+  o->margin = 0.1;
+  o->first_sigma = nullptr;
+  o->need_out = nullptr;
+}
+
+int hm_hip_rank_features(int nblock, int n_extra, const float *extra, const double *sig,
+                         double thr, float *out) {
+  if (nblock < 0 || n_extra < 0 || (nblock > 0 && (!sig || !out || (n_extra > 0 && !extra))))
+    return INPUT_ERROR;
+  const int F = n_extra + 17;
+  for (int g = 0; g < nblock; g++) {
+    float *o = out + (size_t)g * F;
+    for (int e = 0; e < n_extra; e++) o[e] = extra[(size_t)g * n_extra + e];
+    const double *sv = sig + (size_t)g * FIRST_SIGMA;
+    const double s0 = sv[0];
+    int above = 0;
+    for (int i = 1; i < FIRST_SIGMA; i++) {
+      const double r = s0 > 0.0 ? sv[i] / s0 : 0.0;
+      o[n_extra + i - 1] = (float)(std::log10(std::max(r, 1e-20)) / 10.0);
+      above += (s0 > 0.0 && r >= thr) ? 1 : 0;
+    }
+    o[n_extra + 15] = (float)above / (float)FIRST_SIGMA;
+    o[n_extra + 16] = (float)(std::log10(thr) / 10.0);
+  }
+  return SUCCESS;
 }
 
 int hm_hip_fill_kernel_matrix(int kind, int n, int dim, const double *d_pts,
@@ -195,6 +223,8 @@ static void fill_stats(hm_compress_stats *stats, const TsvdStats &ts, double ms)
   stats->max_sketch = ts.max_sketch;
   stats->fallback_blocks = ts.capped;
   stats->ms_total = ms;
+  stats->predicted_blocks = ts.widened;
+  stats->predicted_failed = ts.widened_failed;
 }
 
 static TsvdOptions tsvd_options(const hm_compress_opts &opts) {
@@ -204,6 +234,52 @@ static TsvdOptions tsvd_options(const hm_compress_opts &opts) {
   to.accept = opts.accept;
   to.seed = opts.seed;
   return to;
+}
+
+// Width of the second round from the learned model: the blocks in `retry`
+// get ceil(2^y - 1) (1 + margin) + oversample with y the model output.
+static int learned_widen(const hm_compress_opts &opts, double thr, std::vector<TsvdBlock> &blocks,
+                         const std::vector<int> &retry, const std::vector<double> &sig,
+                         hipStream_t s) {
+  const int F = opts.n_extra + 17;
+  const int nr = (int)retry.size();
+  std::vector<float> ex((size_t)nr * std::max(opts.n_extra, 1), 0.0f), feats((size_t)nr * F);
+  std::vector<double> sg((size_t)nr * FIRST_SIGMA);
+  for (int r = 0; r < nr; r++) {
+    const int g = retry[r];
+    for (int e = 0; e < opts.n_extra; e++)
+      ex[(size_t)r * opts.n_extra + e] = opts.block_features[(size_t)g * opts.n_extra + e];
+    for (int j = 0; j < FIRST_SIGMA; j++)
+      sg[(size_t)r * FIRST_SIGMA + j] = sig[(size_t)g * FIRST_SIGMA + j];
+  }
+  HM_OK(hm_hip_rank_features(nr, opts.n_extra, ex.data(), sg.data(), thr, feats.data()));
+  float *d_x = nullptr, *d_y = nullptr;
+  HM_OK(dalloc(&d_x, feats.size()));
+  int rc = dalloc(&d_y, (size_t)nr);
+  std::vector<float> y(nr, 0.0f);
+  if (rc == SUCCESS) {
+    hipError_t e = hipMemcpyAsync(d_x, feats.data(), feats.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+      rc = hm_hip_mlp_forward(const_cast<hm_mlp *>(opts.rank_model), d_x, nr, d_y, s);
+      if (rc == SUCCESS) {
+        e = hipMemcpyAsync(y.data(), d_y, (size_t)nr * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+      }
+    }
+    if (rc == SUCCESS && e != hipSuccess) {
+      set_error("learned width: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
+    }
+  }
+  dfree(d_x);
+  dfree(d_y);
+  if (rc != SUCCESS) return rc;
+  for (int r = 0; r < nr; r++) {
+    const double yy = std::min(std::max((double)y[r], 0.0), 12.0);
+    const double k = std::exp2(yy) - 1.0;
+    blocks[retry[r]].ell = (int)std::ceil(k * (1.0 + opts.margin)) + opts.oversample;
+  }
+  return SUCCESS;
 }
 
 }  // namespace hm
@@ -229,9 +305,28 @@ int hm_hip_compress_dense(const double *d_A, long lda, int m, int height,
   hm_dtree *t = nullptr;
   std::vector<TsvdBlock> tb;
   HM_OK(plan_tree(d_A, lda, m, height, ms, thr, opts, 0, &t, tb));
+  if (opts.rank_model && opts.n_extra > 0 && !opts.block_features) {
+    tree_free(t);
+    set_error("hm_hip_compress_dense: rank_model needs block_features");
+    return INPUT_ERROR;
+  }
+  TsvdOptions to = tsvd_options(opts);
+  std::vector<double> first;
+  if (opts.first_sigma) to.first_sigma = &first;
+  if (opts.rank_model)
+    to.widen = [&](std::vector<TsvdBlock> &bl, const std::vector<int> &retry,
+                   const std::vector<double> &sig) {
+      return learned_widen(opts, thr, bl, retry, sig, s);
+    };
   TsvdStats ts;
   double *up = nullptr, *vp = nullptr;
-  int rc = tsvd_run(tb, tsvd_options(opts), &up, &vp, &ts, nullptr, 0, s);
+  int rc = tsvd_run(tb, to, &up, &vp, &ts, nullptr, 0, s);
+  if (rc == SUCCESS && opts.need_out)
+    for (int g = 0; g < t->nblock; g++) opts.need_out[g] = tb[g].need;
+  if (rc == SUCCESS && opts.first_sigma) {
+    if (first.empty()) first.assign((size_t)t->nblock * FIRST_SIGMA, 0.0);
+    memcpy(opts.first_sigma, first.data(), (size_t)t->nblock * FIRST_SIGMA * sizeof(double));
+  }
   if (rc != SUCCESS) {
     tree_free(t);
     return rc;

@@ -78,7 +78,9 @@ struct Program {
   // violation.  Returns SUCCESS or INPUT_ERROR.
   int audit(std::string *msg) const;
   int upload(hipStream_t s);
-  int run(hipStream_t s);  // launches every op in order (no sync)
+  // launches every op in order (no sync); timed: bracket each op with the
+  // kernel-timing events under its name (off while a graph is captured)
+  int run(hipStream_t s, bool timed = true);
 };
 
 // ---------------------------------------------------------- batched GEMM
@@ -122,11 +124,16 @@ struct Copy2D {
 int copy2d_add(Program &p, const std::vector<Copy2D> &items);
 
 // *out = max over columns of the 2-norm of a (rows x cols, ld) panel
+// *out = max over columns j of |a_j| (or of |a_j| / |b_j| when b is set,
+// b having b_rows rows and leading dimension ldb)
 struct ColNorm {
   const double *a;
   long long ld;
   int rows, cols;
   double *out;
+  const double *b;
+  long long ldb;
+  int b_rows, pad;
 };
 int colnorm_add(Program &p, const std::vector<ColNorm> &items);
 

@@ -42,9 +42,12 @@ int Program::upload(hipStream_t s) {
   return SUCCESS;
 }
 
-int Program::run(hipStream_t s) {
+int Program::run(hipStream_t s, bool timed) {
   for (size_t i = 0; i < ops.size(); i++) {
+    Timed t;
+    if (timed) t = timing_begin(names[i].c_str(), s);
     ops[i](s, d_blob);
+    if (timed) timing_end(t, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       set_error("program op %zu (%s): launch failed: %s", i, names[i].c_str(),
@@ -283,22 +286,29 @@ int copy2d_add(Program &p, const std::vector<Copy2D> &items) {
   return SUCCESS;
 }
 
+__device__ __forceinline__ double block_norm2(const double *a, int rows, double *red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double ss = 0.0;
+  for (int i = threadIdx.x; i < rows; i += 256) ss += a[i] * a[i];
+  ss = wave_sum(ss);
+  __syncthreads();
+  if (lane == 0) red[w] = ss;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
 __global__ __launch_bounds__(256) void k_colnorm(const ColNorm *items) {
   __shared__ double red[4];
   const ColNorm it = items[blockIdx.x];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double mx = 0.0;
   for (int c = 0; c < it.cols; c++) {
-    double ss = 0.0;
-    for (int i = threadIdx.x; i < it.rows; i += 256) {
-      const double x = it.a[i + (long long)c * it.ld];
-      ss += x * x;
+    const double na = sqrt(block_norm2(it.a + (long long)c * it.ld, it.rows, red));
+    if (it.b) {
+      const double nb = sqrt(block_norm2(it.b + (long long)c * it.ldb, it.b_rows, red));
+      if (nb > 0.0) mx = fmax(mx, na / nb);
+    } else {
+      mx = fmax(mx, na);
     }
-    ss = wave_sum(ss);
-    __syncthreads();
-    if (lane == 0) red[w] = ss;
-    __syncthreads();
-    mx = fmax(mx, sqrt(red[0] + red[1] + red[2] + red[3]));
   }
   if (threadIdx.x == 0) *it.out = mx;
 }
@@ -309,6 +319,8 @@ int colnorm_add(Program &p, const std::vector<ColNorm> &items) {
     v.push_back(c);
     if (c.rows > 0 && c.cols > 0)
       p.touch(c.a, ((size_t)(c.cols - 1) * c.ld + c.rows) * sizeof(double), "colnorm");
+    if (c.b && c.b_rows > 0 && c.cols > 0)
+      p.touch(c.b, ((size_t)(c.cols - 1) * c.ldb + c.b_rows) * sizeof(double), "colnorm den");
     p.touch(c.out, sizeof(double), "colnorm out");
   }
   if (v.empty()) return SUCCESS;

@@ -114,6 +114,15 @@ void timing_end(const Timed &t, hipStream_t s) {
   (void)hipEventRecord(g_slots[t.slot].b, s);
 }
 
+void timing_host(const char *name, double ms) {
+  if (!g_timing) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_timing_only.empty() && g_timing_only != name) return;
+  auto &acc = g_acc[name];
+  acc.total_ms += ms;
+  acc.count += 1;
+}
+
 }  // namespace hm
 
 using namespace hm;
@@ -241,6 +250,20 @@ int hm_hip_kernel_timing_read(const char *name, double *total_ms, long *count) {
   *total_ms = it == g_acc.end() ? 0.0 : it->second.total_ms;
   *count = it == g_acc.end() ? 0 : it->second.count;
   return SUCCESS;
+}
+
+int hm_hip_kernel_timing_dump(char *buf, int len) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  timing_drain_locked();
+  std::string out;
+  char line[256];
+  for (const auto &kv : g_acc) {
+    snprintf(line, sizeof(line), "%s\t%.4f\t%ld\n", kv.first.c_str(), kv.second.total_ms,
+             kv.second.count);
+    out += line;
+  }
+  if (buf && len > 0) snprintf(buf, (size_t)len, "%s", out.c_str());
+  return (int)out.size();
 }
 
 int hm_hip_kernel_timing_reset(void) {

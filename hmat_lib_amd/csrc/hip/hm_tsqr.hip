@@ -12,6 +12,7 @@
 // the trailing columns of R past column `ell` (for the randomized range
 // finder's posterior test: they equal |(I - Q_1 Q_1^T) y_t|).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -184,6 +185,146 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down(const QrItem *items) {
   }
 }
 
+// ------------------------------------------------ LDS-resident chunks
+// Panels of at most TQ_FAST_COLS columns use chunks of at most TQ_CR rows
+// (rows spread evenly, so every chunk of a multi-chunk level has >= ncol
+// rows) held entirely in LDS.  Column j belongs to wave j % 4: the owner of
+// column k forms reflector k, one barrier publishes it, and every wave
+// applies it to its own columns with wavefront reductions.  Building Q needs
+// no barrier per reflector at all (each wave owns its columns of Q).
+static constexpr int TQ_CR = 128;
+static constexpr int TQ_FAST_COLS = 64;
+
+__global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
+  __shared__ double Y[TQ_CR * TQ_FAST_COLS];
+  __shared__ double tauS[TQ_FAST_COLS];
+  const QrItem it = items[blockIdx.x];
+  const int rows = it.rows, nc = it.ncol, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  double *G = it.panel + it.r0;
+  const long long ld = it.ld;
+  for (int e = tid; e < rows * nc; e += TQ_NT) {
+    const int i = e % rows, j = e / rows;
+    Y[i + j * TQ_CR] = G[i + j * ld];
+  }
+  __syncthreads();
+  const int nref = min(rows, nc);
+  for (int k = 0; k < nref; k++) {
+    double *yk = Y + k * TQ_CR;
+    if ((k & 3) == wave) {
+      double x2 = 0.0;
+      for (int i = k + 1 + lane; i < rows; i += 64) x2 += yk[i] * yk[i];
+      x2 = wave_sum(x2);
+      const double alpha = yk[k];
+      double tau = 0.0, beta = alpha, scal = 1.0;
+      if (x2 > 0.0) {
+        const double hh = hypot(alpha, sqrt(x2));
+        beta = (alpha >= 0.0) ? -hh : hh;
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      if (tau != 0.0)
+        for (int i = k + 1 + lane; i < rows; i += 64) yk[i] *= scal;
+      if (lane == 0) {
+        yk[k] = beta;
+        tauS[k] = tau;
+      }
+    }
+    __syncthreads();
+    const double tau = tauS[k];
+    if (tau == 0.0) continue;
+    for (int j = k + 1 + ((wave - (k + 1)) & 3); j < nc; j += 4) {
+      double *yj = Y + j * TQ_CR;
+      double d = 0.0;
+      for (int i = k + lane; i < rows; i += 64) d += (i == k ? 1.0 : yk[i]) * yj[i];
+      d = tau * wave_sum(d);
+      for (int i = k + lane; i < rows; i += 64) yj[i] -= d * (i == k ? 1.0 : yk[i]);
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < nc; k += TQ_NT) it.tau[k] = k < nref ? tauS[k] : 0.0;
+  for (int e = tid; e < rows * nc; e += TQ_NT) {
+    const int i = e % rows, j = e / rows;
+    G[i + j * ld] = Y[i + j * TQ_CR];
+  }
+  for (int e = tid; e < nc * nc; e += TQ_NT) {
+    const int i = e % nc, j = e / nc;
+    const double r = (i <= j && i < rows) ? Y[i + j * TQ_CR] : 0.0;
+    it.rout[it.rrow0 + i + j * it.ldr] = r;
+    if (it.is_top && it.mt) it.mt[j + i * nc] = r;
+  }
+  if (it.is_top && it.est && tid == 0) {
+    double mx = 0.0;
+    for (int t = it.ell; t < nc; t++) {
+      double ss = 0.0;
+      for (int i = it.ell; i <= t && i < rows; i++) ss += Y[i + t * TQ_CR] * Y[i + t * TQ_CR];
+      mx = fmax(mx, sqrt(ss));
+    }
+    *it.est = mx;
+  }
+}
+
+__global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
+  __shared__ double V[TQ_CR * TQ_FAST_COLS];
+  __shared__ double Qs[TQ_CR * TQ_FAST_COLS];
+  __shared__ double tauS[TQ_FAST_COLS];
+  const QrItem it = items[blockIdx.x];
+  const int rows = it.rows, nc = it.ncol, nq = it.nq, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const double *G = it.panel + it.r0;
+  const long long ld = it.ld;
+  const int nref = min(rows, nc);
+  for (int e = tid; e < rows * nref; e += TQ_NT) {
+    const int i = e % rows, j = e / rows;
+    V[i + j * TQ_CR] = G[i + j * ld];
+  }
+  for (int k = tid; k < nref; k += TQ_NT) tauS[k] = it.tau[k];
+  for (int e = tid; e < rows * nq; e += TQ_NT) {
+    const int i = e % rows, j = e / rows;
+    double v = 0.0;
+    if (i < nc) v = it.cin ? it.cin[it.crow0 + i + (long long)j * it.ldc] : (i == j ? 1.0 : 0.0);
+    Qs[i + j * TQ_CR] = v;
+  }
+  __syncthreads();
+  for (int k = nref - 1; k >= 0; k--) {
+    const double tau = tauS[k];
+    if (tau == 0.0) continue;
+    const double *vk = V + k * TQ_CR;
+    for (int j = wave; j < nq; j += 4) {
+      double *qj = Qs + j * TQ_CR;
+      double d = 0.0;
+      for (int i = k + lane; i < rows; i += 64) d += (i == k ? 1.0 : vk[i]) * qj[i];
+      d = tau * wave_sum(d);
+      for (int i = k + lane; i < rows; i += 64) qj[i] -= d * (i == k ? 1.0 : vk[i]);
+    }
+  }
+  __syncthreads();
+  double *Q = it.qout + it.r0;
+  for (int e = tid; e < rows * nq; e += TQ_NT) {
+    const int i = e % rows, j = e / rows;
+    Q[i + (long long)j * it.ldq] = Qs[i + j * TQ_CR];
+  }
+}
+
+// chunking of one level: the LDS path spreads rows evenly over
+// ceil(rows / TQ_CR) chunks; the general path uses chunks of CR rows with
+// the remainder in the last
+static long long tq_nchunks(long long rows, int ncol) {
+  if (ncol <= TQ_FAST_COLS) return std::max<long long>(1, (rows + TQ_CR - 1) / TQ_CR);
+  return std::max<long long>(1, rows / std::max(128, 2 * ncol));
+}
+static void tq_chunk(long long rows, long long nch, long long i, int ncol, long long &r0,
+                     long long &rr) {
+  if (ncol <= TQ_FAST_COLS) {
+    r0 = i * rows / nch;
+    rr = (i + 1) * rows / nch - r0;
+  } else {
+    const int CR = std::max(128, 2 * ncol);
+    r0 = i * CR;
+    rr = (i == nch - 1) ? rows - i * CR : CR;
+  }
+}
+
 int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
   if (panels.empty()) return SUCCESS;
   // per panel: rows of each level (level 0 = the panel itself)
@@ -194,31 +335,30 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
       set_error("tsqr: panel %zu has m=%d ncol=%d nq=%d", p, P.m, P.ncol, P.nq);
       return INPUT_ERROR;
     }
-    const int CR = std::max(128, 2 * P.ncol);
     long long rows = P.m;
     for (;;) {
       rows_per_level[p].push_back(rows);
-      const long long nch = std::max<long long>(1, rows / CR);
+      const long long nch = tq_nchunks(rows, P.ncol);
       if (nch == 1) break;
       rows = nch * P.ncol;
     }
   }
   struct Level {
-    std::vector<QrItem> up, down;
+    std::vector<QrItem> up, down, up_fast, down_fast;
   };
   size_t maxL = 0;
   for (auto &r : rows_per_level) maxL = std::max(maxL, r.size());
   std::vector<Level> levels(maxL);
   for (size_t p = 0; p < panels.size(); p++) {
     const TsqrPanel &P = panels[p];
-    const int CR = std::max(128, 2 * P.ncol);
+    const bool fast = P.ncol <= TQ_FAST_COLS;
     const size_t nl = rows_per_level[p].size();
     // scratch of this panel: tau per level, stacked R and level Q for
     // levels >= 1, the top R when the caller wants none
     std::vector<double *> tau(nl), st(nl, nullptr), qs(nl, nullptr);
     for (size_t L = 0; L < nl; L++) {
       const long long rows = rows_per_level[p][L];
-      const long long nch = std::max<long long>(1, rows / CR);
+      const long long nch = tq_nchunks(rows, P.ncol);
       tau[L] = ar.take<double>((size_t)nch * P.ncol);
       prog.touch(tau[L], (size_t)nch * P.ncol * sizeof(double), "tsqr tau");
       if (L > 0) {
@@ -239,7 +379,7 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
     if (P.est) prog.touch(P.est, sizeof(double), "tsqr est");
     for (size_t L = 0; L < nl; L++) {
       const long long rows = rows_per_level[p][L];
-      const long long nch = std::max<long long>(1, rows / CR);
+      const long long nch = tq_nchunks(rows, P.ncol);
       const bool top = (L + 1 == nl);
       double *store = L == 0 ? P.A : st[L];
       const long long ld = L == 0 ? P.lda : rows;
@@ -249,10 +389,12 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
       for (long long i = 0; i < nch; i++) {
         QrItem it;
         memset(&it, 0, sizeof(it));
+        long long cr0, crr;
+        tq_chunk(rows, nch, i, P.ncol, cr0, crr);
         it.panel = store;
         it.ld = ld;
-        it.r0 = (int)(i * CR);
-        it.rows = (int)((i == nch - 1) ? rows - i * CR : CR);
+        it.r0 = (int)cr0;
+        it.rows = (int)crr;
         it.ncol = P.ncol;
         it.nq = P.nq;
         it.tau = tau[L] + i * P.ncol;
@@ -275,32 +417,45 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
         }
         it.qout = qstore;
         it.ldq = ldq;
-        levels[L].up.push_back(it);
-        if (P.Q) levels[L].down.push_back(it);
+        (fast ? levels[L].up_fast : levels[L].up).push_back(it);
+        if (P.Q) (fast ? levels[L].down_fast : levels[L].down).push_back(it);
       }
     }
   }
-  std::vector<size_t> up_off(maxL), down_off(maxL);
+  std::vector<size_t> up_off(maxL), down_off(maxL), upf_off(maxL), downf_off(maxL);
   for (size_t L = 0; L < maxL; L++) {
     up_off[L] = prog.push(levels[L].up);
     down_off[L] = prog.push(levels[L].down);
+    upf_off[L] = prog.push(levels[L].up_fast);
+    downf_off[L] = prog.push(levels[L].down_fast);
   }
   for (size_t L = 0; L < maxL; L++) {
-    const unsigned n = (unsigned)levels[L].up.size();
-    const size_t off = up_off[L];
+    const unsigned n = (unsigned)levels[L].up.size(), nf = (unsigned)levels[L].up_fast.size();
+    const size_t off = up_off[L], offf = upf_off[L];
     if (n)
       prog.add("tsqr_up", [=](hipStream_t s, const unsigned char *b) {
         hipLaunchKernelGGL(k_tsqr_up, dim3(n), dim3(TQ_NT), 0, s,
                            reinterpret_cast<const QrItem *>(b + off));
       });
+    if (nf)
+      prog.add("tsqr_up", [=](hipStream_t s, const unsigned char *b) {
+        hipLaunchKernelGGL(k_tsqr_up_lds, dim3(nf), dim3(TQ_NT), 0, s,
+                           reinterpret_cast<const QrItem *>(b + offf));
+      });
   }
   for (size_t L = maxL; L-- > 0;) {
-    const unsigned n = (unsigned)levels[L].down.size();
-    const size_t off = down_off[L];
+    const unsigned n = (unsigned)levels[L].down.size(),
+                   nf = (unsigned)levels[L].down_fast.size();
+    const size_t off = down_off[L], offf = downf_off[L];
     if (n)
       prog.add("tsqr_down", [=](hipStream_t s, const unsigned char *b) {
         hipLaunchKernelGGL(k_tsqr_down, dim3(n), dim3(TQ_NT), 0, s,
                            reinterpret_cast<const QrItem *>(b + off));
+      });
+    if (nf)
+      prog.add("tsqr_down", [=](hipStream_t s, const unsigned char *b) {
+        hipLaunchKernelGGL(k_tsqr_down_lds, dim3(nf), dim3(TQ_NT), 0, s,
+                           reinterpret_cast<const QrItem *>(b + offf));
       });
   }
   return SUCCESS;
@@ -407,6 +562,7 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
 
 int program_exec(const std::function<int(Program &, Arena &)> &build,
                  const std::vector<MemRange> &inputs, hipStream_t s) {
+  const auto h0 = std::chrono::steady_clock::now();
   Program dry;
   Arena count;
   dry.record = true;
@@ -429,6 +585,9 @@ int program_exec(const std::function<int(Program &, Arena &)> &build,
     rc = INPUT_ERROR;
   }
   if (rc == SUCCESS) rc = prog.upload(s);
+  timing_host("program build (host)",
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0)
+                  .count());
   if (rc == SUCCESS) rc = prog.run(s);
   if (rc == SUCCESS) {
     hipError_t e = hipStreamSynchronize(s);

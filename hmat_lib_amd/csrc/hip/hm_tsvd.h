@@ -1,6 +1,8 @@
 // Batched truncated SVD of dense blocks (internal interface).
 #pragma once
 
+#include <functional>
+
 #include "hm_common.h"
 
 namespace hm {
@@ -17,9 +19,10 @@ struct TsvdBlock {
   int rule;
   int status;    // out: 1 accepted
   int k;         // out: rank
-  int pad;
+  int need;      // out: singular values of the core >= accept/2 * thr (* sigma_0)
   double thr;
-  double err;    // out: residual estimate
+  double err;    // out: residual estimate (Frobenius-type bound)
+  double perr;   // out: residual estimate after one power step (spectral)
   double sigma0; // out: sigma_0 lower bound, then exact top singular value
   long long yoff, ooff, noff, qoff, boff, moff, soff;
   unsigned long long uid;
@@ -27,11 +30,22 @@ struct TsvdBlock {
   double *vout;
 };
 
+// Sorted singular values of the first round's core kept per block.
+static constexpr int FIRST_SIGMA = 16;
+
 struct TsvdOptions {
   int sketch_max = 512;
   int n_test = 6;
   double accept = 0.1;
   unsigned long long seed = 0x9e3779b97f4a7c15ull;
+  // After the first round: sig (nb x FIRST_SIGMA, zero padded) holds the
+  // first round's singular values of every block.  `widen`, when set,
+  // chooses the next sketch width of the blocks in `retry` (instead of
+  // doubling); later rounds double.
+  std::function<int(std::vector<TsvdBlock> &, const std::vector<int> &retry,
+                    const std::vector<double> &sig)>
+      widen;
+  std::vector<double> *first_sigma = nullptr;
 };
 
 struct TsvdStats {
@@ -40,6 +54,8 @@ struct TsvdStats {
   int max_sketch = 0;
   int exact_blocks = 0;
   int capped = 0;      // blocks that hit sketch_max without passing the test
+  int widened = 0;     // blocks whose second width came from `widen`
+  int widened_failed = 0;  // ... and failed the residual test at that width
 };
 
 // Factor every block.  On entry blocks[i].{A,lda,m,n,ell,rule,thr,uid} are

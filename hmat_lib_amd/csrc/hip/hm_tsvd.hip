@@ -23,6 +23,7 @@
 // sigma0 a rank decision can differ from a full SVD only when a singular
 // value lies within 0.5% above the threshold.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -250,16 +251,24 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
     B.k = cut;
     B.sigma0 = s0;
     kk = cut;
-    // posterior test (B.err was set by the sketch's QR)
+    // posterior test: B.err = max_t |(I - Q Q^T) A w_t| bounds |(I - Q Q^T) A|_2
+    // by 10 sqrt(2/pi) B.err (Halko et al. Lemma 4.1; loose by up to the
+    // square root of the tail length when the tail is flat); B.perr = the
+    // same residual after one power step, |E E^T z| / |E^T z| ~ sigma_1(E),
+    // taken with a safety factor 2.  Either passing accepts.
+    const double scale = accept * B.thr * (B.rule == RULE_RELATIVE ? s0 : 1.0);
     bool ok = true;
     if (B.ntest > 0) {
       const double est = 10.0 * 0.7978845608028654 * B.err;
-      const double bound = accept * B.thr * (B.rule == RULE_RELATIVE ? s0 : 1.0);
+      const double pest = 2.0 * B.perr;
       const bool tiny = (B.rule == RULE_RELATIVE) && !(s0 > B.thr);
-      ok = tiny || est <= bound;
+      ok = tiny || est <= scale || pest <= scale;
       if (ok && !tiny && cut == nc) ok = false;  // sketch saturated: widen
-      B.err = est;
+      B.err = fmin(est, pest);
     }
+    int need = 0;
+    while (need < nc && sig[perm[need]] >= 0.5 * scale) need++;
+    B.need = need;
     B.status = ok ? 1 : 0;
   }
   __syncthreads();
@@ -473,6 +482,7 @@ static int svd_cores(TsvdBlock *d_blocks, const std::vector<TsvdBlock> &blocks,
       cleanup();
       return rc;
     }
+    Timed tb = timing_begin("block jacobi", s);
     hipLaunchKernelGGL(k_eye, dim3((unsigned)big.size(), 64), dim3(256), 0, s, d_blocks, d_big, bf);
     std::vector<int> hflags(blocks.size());
     for (int sweep = 0; sweep < 30; sweep++) {
@@ -497,6 +507,7 @@ static int svd_cores(TsvdBlock *d_blocks, const std::vector<TsvdBlock> &blocks,
     }
     hipLaunchKernelGGL(k_jacobi, dim3((unsigned)big.size()), dim3(256), 0, s, d_blocks, d_big,
                        bf, accept, 1);
+    timing_end(tb, s);
   }
   if (!small.empty()) {
     int *d_small = nullptr;
@@ -624,6 +635,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
   std::vector<long long> uoff_final(nb, 0), voff_final(nb, 0);
   while (!active.empty()) {
     st.rounds++;
+    const auto r0 = std::chrono::steady_clock::now();
     Scoped mem;
     // offsets for this round
     long long ytot = 0, otot = 0, qtot = 0, btot = 0, mtot = 0, stot = 0;
@@ -638,6 +650,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       b.soff = stot; stot += b.ell;
       b.status = 0;
       b.err = 0.0;
+      b.perr = 0.0;
       st.max_sketch = std::max(st.max_sketch, b.ell);
     }
     TsvdBufs bf;
@@ -687,6 +700,8 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     HM_OK(to_device(&d_ws, ws, s)); mem.keep(d_ws);
     HM_OK(to_device(&d_wp, wp, s)); mem.keep(d_wp);
     HM_OK(to_device(&d_list, order, s)); mem.keep(d_list);
+    timing_host("round setup (host)", std::chrono::duration<double, std::milli>(
+                                          std::chrono::steady_clock::now() - r0).count());
     // the round as one launch program: sketch, QR of the sketch, posterior
     // test, projection, QR of B^T with R, Jacobi SVD of R^T
     std::vector<MemRange> inputs = {
@@ -694,6 +709,10 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         {bf.Q, (size_t)qtot * 8 + 16}, {bf.Bt, (size_t)btot * 8 + 16},
         {bf.Q2, (size_t)btot * 8 + 16}, {bf.M, (size_t)mtot * 8 + 16},
         {d_blocks, (size_t)nb * sizeof(TsvdBlock)}};
+    for (int i : order) {
+      const TsvdBlock &b = blocks[i];
+      inputs.push_back({b.A, ((size_t)(b.n - 1) * b.lda + b.m) * sizeof(double)});
+    }
     const unsigned long long seed = opt.seed;
     const unsigned nwo = (unsigned)wo.size(), nws = (unsigned)ws.size(),
                    nwp = (unsigned)wp.size();
@@ -708,8 +727,8 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         timing_end(t1, st);
       });
       std::vector<QrJob> qy, qb;
-      std::vector<GemmJob> t1, t2;
-      std::vector<ColNorm> cn;
+      std::vector<GemmJob> t1, t2, p1, p2, p3, p4;
+      std::vector<ColNorm> cn, pn;
       std::vector<Copy2D> tr;
       for (int i : order) {
         const TsvdBlock &b = blocks[i];
@@ -727,7 +746,23 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
                         1, 0, 0, -1.0, 1.0});
           double *err = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
                                                    offsetof(TsvdBlock, err));
-          cn.push_back({Yt, b.m, b.m, b.ntest, err});
+          cn.push_back({Yt, b.m, b.m, b.ntest, err, nullptr, 0, 0, 0});
+          // one power step on the residual E = (I - Q Q^T) A: W = A^T Z,
+          // U = (I - Q Q^T) A W, perr = max_t |U_t| / |W_t|
+          double *Wt = ar.take<double>((size_t)b.n * b.ntest);
+          double *Ut = ar.take<double>((size_t)b.m * b.ntest);
+          double *T2 = ar.take<double>((size_t)b.ell * b.ntest);
+          p1.push_back({b.A, Yt, Wt, nullptr, b.lda, b.m, b.n, b.n, b.ntest, b.m, 1, 0, 0,
+                        1.0, 0.0});
+          p2.push_back({b.A, Wt, Ut, nullptr, b.lda, b.n, b.m, b.m, b.ntest, b.n, 1, 0, 0,
+                        1.0, 0.0});
+          p3.push_back({bf.Q + b.qoff, Ut, T2, nullptr, b.m, b.m, b.ell, b.ell, b.ntest, b.m,
+                        1, 0, 0, 1.0, 0.0});
+          p4.push_back({bf.Q + b.qoff, T2, Ut, nullptr, b.m, b.ell, b.m, b.m, b.ntest, b.ell,
+                        1, 0, 0, -1.0, 1.0});
+          double *perr = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
+                                                    offsetof(TsvdBlock, perr));
+          pn.push_back({Ut, b.m, b.m, b.ntest, perr, Wt, b.n, b.n, 0});
         }
       }
       HM_OK(qr_add(pg, ar, qy));
@@ -736,6 +771,13 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         HM_OK(gemm_add(pg, ar, t2, false, false, "test upd"));
       }
       HM_OK(colnorm_add(pg, cn));
+      HM_OK(gemm_add(pg, ar, p1, true, false, "power At"));
+      HM_OK(gemm_add(pg, ar, p2, false, false, "power A"));
+      for (int rep = 0; rep < 2; rep++) {
+        HM_OK(gemm_add(pg, ar, p3, true, false, "power proj"));
+        HM_OK(gemm_add(pg, ar, p4, false, false, "power upd"));
+      }
+      HM_OK(colnorm_add(pg, pn));
       pg.add("project", [=](hipStream_t st, const unsigned char *) {
         Timed t3 = timing_begin("tsvd_project", st);
         hipLaunchKernelGGL(k_project, dim3(nwp), dim3(256), 0, st, d_blocks, d_wp, bf);
@@ -755,12 +797,28 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
                           hipMemcpyDeviceToHost, s));
     HM_HIP(hipStreamSynchronize(s));
+    const bool first_round = (st.rounds == 1);
+    std::vector<double> sig16;
+    if (first_round && (opt.widen || opt.first_sigma)) {
+      std::vector<double> hs(stot);
+      HM_HIP(hipMemcpyAsync(hs.data(), bf.Sig, stot * sizeof(double), hipMemcpyDeviceToHost, s));
+      HM_HIP(hipStreamSynchronize(s));
+      sig16.assign((size_t)nb * FIRST_SIGMA, 0.0);
+      for (int i : order) {
+        const TsvdBlock &b = blocks[i];
+        for (int j = 0; j < std::min(b.ell, FIRST_SIGMA); j++)
+          sig16[(size_t)i * FIRST_SIGMA + j] = hs[b.soff + j];
+      }
+      if (opt.first_sigma) *opt.first_sigma = sig16;
+    }
     std::vector<int> done, retry;
     for (int i : order) {
       TsvdBlock &b = blocks[i];
       b.k = hb[i].k;
+      b.need = hb[i].need;
       b.sigma0 = hb[i].sigma0;
       b.err = hb[i].err;
+      b.perr = hb[i].perr;
       b.status = hb[i].status;
       if (b.status || b.ell >= cap) {
         if (!b.status) st.capped++;
@@ -771,6 +829,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       }
     }
     st.retried += (int)retry.size();
+    if (st.rounds == 2 && st.widened > 0) st.widened_failed = (int)retry.size();
     if (!done.empty()) {
       long long ut = 0, vt = 0;
       for (int i : done) {
@@ -825,10 +884,19 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       }
       HM_HIP(hipStreamSynchronize(s));
     }
-    for (int i : retry) {
-      TsvdBlock &b = blocks[i];
+    std::vector<int> prev_ell;
+    for (int i : retry) prev_ell.push_back(blocks[i].ell);
+    if (first_round && opt.widen && !retry.empty()) {
+      HM_OK(opt.widen(blocks, retry, sig16));
+      st.widened = (int)retry.size();
+    } else {
+      for (int i : retry) blocks[i].ell = 2 * blocks[i].ell;
+    }
+    for (size_t r = 0; r < retry.size(); r++) {
+      TsvdBlock &b = blocks[retry[r]];
       const int mn = std::min(b.m, b.n);
-      b.ell = std::min(std::min(2 * b.ell, mn), cap);
+      // never narrower than the sketch that just failed
+      b.ell = std::min(std::min(std::max(b.ell, prev_ell[r] + 1), mn), cap);
       b.ntest = (b.ell >= mn) ? 0 : std::min(opt.n_test, b.m - b.ell);
     }
     active = retry;

@@ -254,5 +254,17 @@ def kernel_timing_read(name: str):
     return tot.value, cnt.value
 
 
+def kernel_timing_dump():
+    """{name: (total_ms, count)} of every timed kernel / host section."""
+    n = capi.hip().hm_hip_kernel_timing_dump(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    capi.hip().hm_hip_kernel_timing_dump(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, tot, cnt = line.split("\t")
+        out[name] = (float(tot), int(cnt))
+    return out
+
+
 def kernel_timing_reset():
     capi.hip().hm_hip_kernel_timing_reset()

@@ -109,14 +109,30 @@ HM_HIP_API int hm_hip_matmat_variant(const hm_dtree *tree, int trans, int R);
  * Rank rule of the reference (src/constructors.c:329-336): keep sigma_i
  * while sigma_i >= thr * sigma_0; rank 1 when sigma_0 <= thr.
  * Replaces src/constructors.c:212-546 and src/lapack_wrapper.c:8-59. */
+typedef struct hm_mlp hm_mlp;
 typedef struct hm_compress_opts {
   int sketch_init;      /* initial sketch width (columns), default 16 */
   int sketch_max;       /* largest sketch width before giving up, default 512 */
   int n_test;           /* posterior test vectors per block, default 6 */
   unsigned long long seed;
   const int *rank_hint; /* per block (level-major) predicted rank, or NULL */
-  int oversample;       /* added to rank_hint, default 8 */
+  int oversample;       /* added to rank_hint / predicted rank, default 8 */
   double accept;        /* residual acceptance factor vs thr*sigma0, default 0.1 */
+  /* Learned width: blocks that fail the first (sketch_init) round get the
+   * width the model predicts from block_features (host, nblock x n_extra,
+   * row-major, level-major blocks) and the first round's singular values
+   * (hm_hip_rank_features): with y the model output and k = 2^y - 1 the
+   * predicted count of singular values at or above accept/2 * thr * sigma_0
+   * (the width the residual test needs), ceil(k (1 + margin)) + oversample
+   * columns instead of doubling; the residual test still decides, and a
+   * failed prediction goes on doubling. */
+  const hm_mlp *rank_model;
+  const float *block_features;
+  int n_extra;
+  double margin;        /* default 0.1 */
+  double *first_sigma;  /* host out (nblock x 16): first round's singular values, or NULL */
+  int *need_out;        /* host out (nblock): singular values of the final core at or above
+                           accept/2 * thr * sigma_0 (the training label), or NULL */
 } hm_compress_opts;
 
 typedef struct hm_compress_stats {
@@ -125,6 +141,8 @@ typedef struct hm_compress_stats {
   int max_sketch;       /* widest sketch used */
   int fallback_blocks;  /* blocks that hit sketch_max (then exact path) */
   double ms_total;
+  int predicted_blocks; /* blocks whose second sketch width the rank model chose */
+  int predicted_failed; /* ... whose residual test failed at that width (went on doubling) */
 } hm_compress_stats;
 
 HM_HIP_API void hm_hip_compress_opts_default(hm_compress_opts *opts);
@@ -154,6 +172,31 @@ HM_HIP_API int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
                             const int *s, double *const *d_u,
                             double *const *d_v, double thr, int method,
                             int *k_out, void *stream);
+
+/* ---- learned rank predictor --------------------------------------------
+ * MLP nin -> nhidden -> nhidden -> 1 (tanh, FP32), Glorot-initialised from
+ * seed; trained and run by the library's own kernels.  Inputs are row-major
+ * (B x nin) FP32 device arrays; the target is log2(rank + 1). */
+HM_HIP_API int hm_hip_mlp_create(int nin, int nhidden, unsigned long long seed, hm_mlp **out);
+HM_HIP_API int hm_hip_mlp_destroy(hm_mlp *m);
+HM_HIP_API long long hm_hip_mlp_nparams(const hm_mlp *m);
+/* Flat parameters: W1 (nin x H), b1, W2 (H x H), b2, w3 (H), b3. */
+HM_HIP_API int hm_hip_mlp_get_params(const hm_mlp *m, float *host);
+HM_HIP_API int hm_hip_mlp_set_params(hm_mlp *m, const float *host);
+HM_HIP_API int hm_hip_mlp_forward(hm_mlp *m, const float *d_X, int B, float *d_y, void *stream);
+/* Full-batch mean-squared-error gradient (host out, nparams) and loss. */
+HM_HIP_API int hm_hip_mlp_gradient(hm_mlp *m, const float *d_X, const float *d_t, int B,
+                                   float *h_grad, float *h_loss, void *stream);
+/* Mini-batch Adam; loss_out (host, epochs) = mean mini-batch loss per epoch. */
+HM_HIP_API int hm_hip_mlp_train(hm_mlp *m, const float *d_X, const float *d_t, int B,
+                                int epochs, int batch, float lr, unsigned long long seed,
+                                float *loss_out, void *stream);
+/* Model inputs of nblock blocks (host): the n_extra caller features, then
+ * 17 from the first round's 16 singular values: log10(sigma_i / sigma_0) / 10
+ * for i = 1..15 (floored at 1e-20), the fraction of them >= thr, and
+ * log10(thr) / 10.  out is nblock x (n_extra + 17), row-major. */
+HM_HIP_API int hm_hip_rank_features(int nblock, int n_extra, const float *extra,
+                                    const double *first_sigma, double thr, float *out);
 
 /* ---- batches of independent trees --------------------------------------
  * Compress nmat dense matrices (device, column-major, not modified) in one
@@ -206,6 +249,9 @@ HM_HIP_API int hm_hip_kernel_timing(int enable);
 HM_HIP_API int hm_hip_kernel_timing_select(const char *name);
 HM_HIP_API int hm_hip_kernel_timing_read(const char *name, double *total_ms, long *count);
 HM_HIP_API int hm_hip_kernel_timing_reset(void);
+/* All accumulated entries as "name\ttotal_ms\tcount\n" lines (truncated to
+ * len); returns the full length. */
+HM_HIP_API int hm_hip_kernel_timing_dump(char *buf, int len);
 
 #ifdef __cplusplus
 }

@@ -272,3 +272,73 @@ def test_reference_fixture_laplacian_on_reference():
             exp[int(k)] = v
         np.testing.assert_allclose(y, exp, atol=1e-10)
         O.ref().free_tree_hodlr(C.byref(t))
+
+
+# ------------------------------------------------- rank predictor oracle
+def test_mlp_oracle_gradient_finite_differences():
+    """The oracle's MLP gradient equals central differences of its loss, for
+    every parameter block (a dropped chain-rule term, a transposed weight
+    or a wrong tanh derivative fails here)."""
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(3)
+    nin, nh, B = 5, 7, 11
+    p = rng.standard_normal(M.nparams(nin, nh)) * 0.5
+    X = rng.standard_normal((B, nin))
+    t = rng.standard_normal(B)
+    loss, g = M.loss_and_grad(p, X, t, nh)
+    h = 1e-6
+    num = np.empty_like(p)
+    for k in range(p.size):
+        e = np.zeros_like(p)
+        e[k] = h
+        num[k] = (M.loss_and_grad(p + e, X, t, nh)[0] - M.loss_and_grad(p - e, X, t, nh)[0]) / (2 * h)
+    np.testing.assert_allclose(g, num, rtol=1e-6, atol=1e-9)
+    assert loss > 0
+
+
+def test_mlp_oracle_forward_formula():
+    """Forward pass vs a per-sample loop over the formula."""
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(4)
+    nin, nh = 3, 4
+    p = rng.standard_normal(M.nparams(nin, nh))
+    X = rng.standard_normal((6, nin))
+    W1, b1, W2, b2, w3, b3 = M.unpack(p, nin, nh)
+    y, _, _ = M.forward(p, X, nh)
+    for b in range(6):
+        a1 = [np.tanh(sum(X[b, i] * W1[i, j] for i in range(nin)) + b1[j]) for j in range(nh)]
+        a2 = [np.tanh(sum(a1[i] * W2[i, j] for i in range(nh)) + b2[j]) for j in range(nh)]
+        assert abs(y[b] - (sum(a2[j] * w3[j] for j in range(nh)) + b3)) < 1e-12
+
+
+def test_rank_features_definition():
+    """Features of hand-made spectra: sigma_i / sigma_0 = 10^-i gives
+    log10 ratios -i/10, and 8 of the 15 ratios reach thr = 1e-8."""
+    from oracle import mlp_oracle as M
+    sig = np.array([[10.0 ** (-i) for i in range(16)], [0.0] * 16])
+    f = M.rank_features(np.array([[1.0, 2.0], [3.0, 4.0]]), sig, 1e-8)
+    assert f.shape == (2, 19)
+    np.testing.assert_allclose(f[0, 2:17], -np.arange(1, 16) / 10.0)
+    assert f[0, 17] == 8 / 16
+    np.testing.assert_allclose(f[:, 18], -0.8)
+    np.testing.assert_allclose(f[1, 2:17], -2.0)  # zero block: floored
+    assert f[1, 17] == 0
+
+
+def test_rank_features_library_matches_oracle():
+    """The library's host feature function (used by compression) equals the
+    oracle's on random spectra."""
+    import ctypes as C
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(5)
+    nb, ne = 9, 3
+    sig = np.sort(10.0 ** rng.uniform(-14, 0, size=(nb, 16)), axis=1)[:, ::-1].copy()
+    sig[2] = 0.0
+    extra = rng.standard_normal((nb, ne)).astype(np.float32)
+    out = np.zeros((nb, ne + 17), dtype=np.float32)
+    rc = capi.hip().hm_hip_rank_features(
+        nb, ne, extra.ctypes.data_as(C.POINTER(C.c_float)),
+        np.ascontiguousarray(sig).ctypes.data_as(C.POINTER(C.c_double)), 1e-8,
+        out.ctypes.data_as(C.POINTER(C.c_float)))
+    assert rc == 0
+    np.testing.assert_allclose(out, M.rank_features(extra, sig, 1e-8), rtol=1e-6, atol=1e-7)
