@@ -174,6 +174,8 @@ def _declare_hip_api(lib):
         "hm_hip_recompress_batch": ([c_int, PInt, PInt, PInt, POINTER(V), POINTER(V),
                                      c_double, c_int, PInt, V], c_int),
         "hm_hip_selftest_mfma": ([PDouble], c_int),
+        "hm_hip_selftest_mfma_bf16": ([POINTER(C.c_float), POINTER(C.c_float),
+                                       POINTER(C.c_float)], c_int),
         "hm_hip_mlp_create": ([c_int, c_int, C.c_ulonglong, POINTER(V)], c_int),
         "hm_hip_mlp_destroy": ([V], c_int),
         "hm_hip_mlp_nparams": ([V], C.c_longlong),
@@ -195,6 +197,7 @@ def _declare_hip_api(lib):
         "hm_hip_batch_run": ([V, c_int, V], c_int),
         "hm_hip_batch_variants": ([V, POINTER(c_int), c_int], c_int),
         "hm_hip_batch_destroy": ([V], c_int),
+        "hm_hip_tree_set_precision": ([V, c_int, V], c_int),
         "hm_hip_kernel_timing_dump": ([C.c_char_p, c_int], c_int),
         "hm_hip_audit_qr": ([c_int, POINTER(c_int), POINTER(c_int), c_int, C.c_char_p,
                              c_int], c_int),

@@ -118,6 +118,7 @@ struct ProductPlan {
 
 }  // namespace hm
 
+struct MixedFactors;
 struct hm_dtree {
   int height = 0;
   int n = 0;
@@ -137,6 +138,10 @@ struct hm_dtree {
   hm::ProductPlan plan;
   double *d_scratch = nullptr;        // P then W, grown on demand
   size_t scratch_elems = 0;
+  // sliced (BF16) copies of U and V for mixed-precision products; slices =
+  // 0 means FP64 products
+  int slices = 0;
+  struct MixedFactors *mixed = nullptr;
 };
 
 namespace hm {
@@ -149,6 +154,13 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s);
 int tree_alloc_pools(hm_dtree *t);
 int ensure_scratch(hm_dtree *t, int R);
 void tree_free(hm_dtree *t);
+void mixed_free(MixedFactors *m);
+// Y = H X with sliced BF16 factors (t->slices > 0); X, Y column-major.
+int mixed_matmat(hm_dtree *t, int R, const double *X, long ldx, double *Y, long ldy,
+                 hipStream_t s);
+// the FP64 diagonal part only: Y = D X
+int products_diag(hm_dtree *t, int R, const double *X, long x_cs, double *Y, long y_cs,
+                  hipStream_t s);
 
 // ----------------------------------------------------------- MFMA tiles
 typedef double d4 __attribute__((ext_vector_type(4)));

@@ -1,0 +1,735 @@
+// Mixed-precision HODLR x dense: the off-diagonal factors and products on
+// BF16 MFMA with exact FP32 accumulation ("sliced" FP64), diagonal leaves in
+// FP64.
+//
+// An FP64 operand is split, per row of K-chunk (at most MX_KCH entries
+// along the summed dimension), into S integer slices q_a with |q_a| <= 128:
+//   x = 2^e (q_0 2^-7 + q_1 2^-15 + ... + q_{S-1} 2^-(8S-1)) + r,
+//   |r| <= 2^(e - 8S), with 2^e > max |x| over the chunk row.
+// Integers of magnitude <= 256 are exact in BF16, products of two slices
+// are integers <= 2^14, and the MFMA sums them in FP32 exactly as long as
+// the running sum stays below 2^24: the slice pairs (a, b) with a + b = d
+// share one FP32 accumulator, at most S of them, over a chunk of MX_KCH
+// entries (S * 2^14 * 128 <= 2^23).  At the end of every chunk the S
+// accumulators are scaled by 2^(eA + eB - 14 - 8d) and added in FP64.
+// Every slice carries 8 bits: the representation error of an entry is at
+// most 2^-8S of its chunk maximum, and the dropped pairs (a + b >= S) are
+// of the same order, so S = 1 is BF16-like, S = 3 FP32-like (2^-24) and
+// S = 4 (10 MFMAs per 16 x 16 x 32 step) reaches 2^-32.
+//
+// Operand storage ("chunk-padded"): along K every chunk starts at a
+// multiple of 32 and is zero-padded to one, so MFMA k-steps never straddle
+// chunks and loads are 16-byte aligned; slices of one operand follow each
+// other at a fixed stride.  Scales are stored per (chunk, row).
+//
+// The product Y = H X (no transpose, X and Y column-major) runs:
+//   slice X   (per leaf chunk and column)
+//   W_b = V_b^T X_src   sliced GEMM, split-K over segments of chunks, then a
+//                       deterministic reduction of the segment partials
+//   slice W_b (per rank chunk and column)
+//   Y = D X   FP64 MFMA (the diagonal part of the FP64 apply kernel)
+//   Y += sum over levels U_b W_b   sliced GEMM with the levels concatenated
+//                                  along K, one pass over Y
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "hm_prog.h"
+
+namespace hm {
+
+static constexpr int MX_KCH = 128;      // exactness chunk along K
+static constexpr int MX_SEG_CHUNKS = 32;  // W split-K segment (chunks)
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------ slicing
+// One wavefront per item: chunk [k0, k0 + klen) of row p of the source
+// (element (p, k) at src[p * sp + k * sk]) -> S slices at
+// dst + a * dst_slice + p * dst_ld + dk0 (klen padded to a multiple of 32
+// with zeros), scale 2^e to *scale.
+struct SliceItem {
+  const double *src;
+  long long sp, sk;
+  __bf16 *dst;
+  long long dst_slice, dst_ld, dk0;
+  double *scale;
+  int p, k0, klen, pad;
+};
+
+__global__ __launch_bounds__(256) void k_slice(const SliceItem *items, int nitems, int S) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w >= nitems) return;
+  const SliceItem it = items[w];
+  const int kp = (it.klen + 31) & ~31;
+  double mx = 0.0;
+  for (int k = lane; k < it.klen; k += 64)
+    mx = fmax(mx, fabs(it.src[(long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk]));
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  int e = 0;
+  if (mx > 0.0) {
+    (void)frexp(mx, &e);  // mx < 2^e
+  }
+  const double sc = ldexp(1.0, e);
+  if (lane == 0) *it.scale = sc;
+  const double inv = ldexp(1.0, -e);
+  __bf16 *d = it.dst + (long long)it.p * it.dst_ld + it.dk0;
+  for (int k = lane; k < kp; k += 64) {
+    double x = k < it.klen
+                   ? it.src[(long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk] * inv
+                   : 0.0;
+    // x in (-1, 1): q_0 = rint(x 2^7), then the remainder scaled by 2^8
+    x *= 128.0;
+    for (int a = 0; a < S; a++) {
+      const double q = rint(x);
+      d[(long long)a * it.dst_slice + k] = (__bf16)(float)q;
+      x = (x - q) * 256.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------- sliced GEMM
+// One segment: C += op(A) op(B) over K entries given as chunks
+// [cs[c], cs[c+1]) (padded, multiples of 32) of the chunk table.
+struct MxSeg {
+  const __bf16 *A;   // element (row p, k) of slice a: A[a * a_slice + p * lda + k]
+  long long lda, a_slice;
+  const double *sA;  // scale of (chunk c, row p): sA[(a_c0 + c) * ldsa + p]
+  long long ldsa;
+  const __bf16 *B;   // element (k, col j) of slice b: B[b * b_slice + j * ldb + k]
+  long long ldb, b_slice;
+  const double *sB;
+  long long ldsb;
+  const int *cs;     // chunk boundaries (padded K), nch + 1 entries
+  int nch, a_c0, b_c0, pad;
+};
+
+// One work item: a 64 x 64 tile of C (rows i0.., cols j0.., within M x N)
+// accumulated over segments [s0, s1).  mode 0: C = acc; 1: C += acc.
+struct MxItem {
+  double *C;
+  long long ldc;
+  int M, N, i0, j0, s0, s1, mode, pad;
+};
+
+// MX_TM x MX_TN tile per workgroup (2 x 2 waves of 32 x 32), K-steps of
+// 32: each step's S slices of the A rows and B columns go global ->
+// registers (prefetched one step ahead) -> LDS, and every wave reads its
+// MFMA fragments from LDS.  LDS rows are padded to MX_RS entries.
+static constexpr int MX_RS = 40;
+static constexpr int MX_TM = 64, MX_TN = 64;
+
+template <int S>
+__global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const MxSeg *segs) {
+  constexpr int FM = 2, FN = 2;  // 16 x 16 blocks per wave
+  __shared__ __attribute__((aligned(16))) __bf16 As[S][MX_TM * MX_RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[S][MX_TN * MX_RS];
+  const MxItem it = items[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int L = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int li = L & 15, lk = L >> 4;
+  // global loads: A row lr, B row lr; 8 entries at kq
+  const int lr = tid >> 2, kq = (tid & 3) * 8;
+  const bool ldA0 = it.i0 + lr < it.M, ldB = it.j0 + lr < it.N;
+  double acc64[FM][FN][4];
+#pragma unroll
+  for (int f = 0; f < FM; f++)
+#pragma unroll
+    for (int g = 0; g < FN; g++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc64[f][g][v] = 0.0;
+  for (int sg = it.s0; sg < it.s1; sg++) {
+    const MxSeg sd = segs[sg];
+    if (sd.nch <= 0) continue;
+    const __bf16 *ga = sd.A + (long long)(it.i0 + lr) * sd.lda + kq;
+    const __bf16 *gb = sd.B + (long long)(it.j0 + lr) * sd.ldb + kq;
+    bf8 ra0[S], rb[S];
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int a = 0; a < S; a++) {
+        ra0[a] = ldA0 ? *reinterpret_cast<const bf8 *>(ga + a * sd.a_slice + k0) : bf8{};
+        rb[a] = ldB ? *reinterpret_cast<const bf8 *>(gb + a * sd.b_slice + k0) : bf8{};
+      }
+    };
+    const int kbeg = sd.cs[0], kend = sd.cs[sd.nch];
+    int c = 0, cend = sd.cs[1];
+    f4 acc[S][FM][FN];
+#pragma unroll
+    for (int d = 0; d < S; d++)
+#pragma unroll
+      for (int f = 0; f < FM; f++)
+#pragma unroll
+        for (int g = 0; g < FN; g++) acc[d][f][g] = f4{0.f, 0.f, 0.f, 0.f};
+    fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < S; a++) {
+        *reinterpret_cast<bf8 *>(&As[a][lr * MX_RS + kq]) = ra0[a];
+        *reinterpret_cast<bf8 *>(&Bs[a][lr * MX_RS + kq]) = rb[a];
+      }
+      __syncthreads();
+      if (k0 + 32 < kend) fetch(k0 + 32);
+      bf8 fb[S][FN];
+#pragma unroll
+      for (int a = 0; a < S; a++)
+#pragma unroll
+        for (int g = 0; g < FN; g++)
+          fb[a][g] = *reinterpret_cast<const bf8 *>(&Bs[a][(wn * 32 + g * 16 + li) * MX_RS + 8 * lk]);
+#pragma unroll
+      for (int f = 0; f < FM; f++) {
+        bf8 fa[S];
+#pragma unroll
+        for (int a = 0; a < S; a++)
+          fa[a] = *reinterpret_cast<const bf8 *>(&As[a][(wm * 32 + f * 16 + li) * MX_RS + 8 * lk]);
+#pragma unroll
+        for (int a = 0; a < S; a++)
+#pragma unroll
+          for (int b = 0; a + b < S; b++)
+#pragma unroll
+            for (int g = 0; g < FN; g++)
+              acc[a + b][f][g] = mfma_bf16(fa[a], fb[b][g], acc[a + b][f][g]);
+      }
+      if (k0 + 32 == cend) {
+        // fold the chunk into FP64 with its row and column scales
+#pragma unroll
+        for (int f = 0; f < FM; f++)
+#pragma unroll
+          for (int g = 0; g < FN; g++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              const int row = it.i0 + wm * 32 + f * 16 + 4 * lk + v;
+              const int col = it.j0 + wn * 32 + g * 16 + li;
+              double t = 0.0;
+#pragma unroll
+              for (int d = S - 1; d >= 0; d--) t = t * 0x1p-8 + (double)acc[d][f][g][v];
+              if (row < it.M && col < it.N)
+                acc64[f][g][v] += sd.sA[(long long)(sd.a_c0 + c) * sd.ldsa + row] *
+                                  sd.sB[(long long)(sd.b_c0 + c) * sd.ldsb + col] * 0x1p-14 * t;
+            }
+#pragma unroll
+        for (int d = 0; d < S; d++)
+#pragma unroll
+          for (int f = 0; f < FM; f++)
+#pragma unroll
+            for (int g = 0; g < FN; g++) acc[d][f][g] = f4{0.f, 0.f, 0.f, 0.f};
+        c++;
+        cend = c < sd.nch ? sd.cs[c + 1] : 0;
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < FM; f++)
+#pragma unroll
+    for (int g = 0; g < FN; g++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = it.i0 + wm * 32 + f * 16 + 4 * lk + v;
+        const int col = it.j0 + wn * 32 + g * 16 + li;
+        if (row < it.M && col < it.N) {
+          double *cp = it.C + row + (long long)col * it.ldc;
+          *cp = it.mode ? *cp + acc64[f][g][v] : acc64[f][g][v];
+        }
+      }
+}
+
+// sum of nseg partials (each M x N, ld M) into C (ld ldc)
+struct MxReduce {
+  const double *part;
+  double *C;
+  long long ldc;
+  int M, N, nseg, pad;
+};
+__global__ __launch_bounds__(256) void k_mx_reduce(const MxReduce *items) {
+  const MxReduce r = items[blockIdx.x];
+  const long long E = (long long)r.M * r.N;
+  for (long long e = (long long)blockIdx.y * 256 + threadIdx.x; e < E;
+       e += (long long)gridDim.y * 256) {
+    double s = 0.0;
+    for (int k = 0; k < r.nseg; k++) s += r.part[(long long)k * E + e];
+    r.C[(e % r.M) + (e / r.M) * r.ldc] = s;
+  }
+}
+
+}  // namespace hm
+
+// ---------------------------------------------------------- tree factors
+struct MixedFactors {
+  int S = 0;
+  // global K-chunk grid over the rows: chunks of <= MX_KCH rows inside
+  // leaves; chunk q covers rows [row[q], row[q] + len[q]) and padded
+  // coordinates [pad[q], pad[q + 1])
+  std::vector<int> row, len, leaf_chunk0;
+  std::vector<long long> pad;
+  long long npad = 0;
+  // per block: V^T slices (A of W = V^T X), U slices (A of Y += U W)
+  struct Blk {
+    long long v_off = 0, v_kp = 0;  // Vs: [S][s][v_kp] at v_off; scales [nch_node][s]
+    long long v_soff = 0;
+    int q0 = 0, q1 = 0;             // chunk range of the column node
+    long long u_off = 0;            // Us: [S][m][s_pad]; scales [nrk][m]
+    long long u_soff = 0;
+    int s_pad = 0, nrk = 0;
+    long long rk_cs = 0;            // offset of the rank chunk table (nrk + 1 ints)
+  };
+  std::vector<Blk> blk;
+  __bf16 *d_vs = nullptr, *d_us = nullptr;
+  double *d_sv = nullptr, *d_su = nullptr;
+  int *d_rk = nullptr;              // rank chunk tables
+  // cached product program (operands fixed)
+  hm::Program *prog = nullptr;
+  char *arena = nullptr;
+  int R = 0;
+  const double *X = nullptr;
+  double *Y = nullptr;
+  long ldx = 0, ldy = 0;
+};
+
+namespace hm {
+
+void mixed_free(MixedFactors *m) {
+  if (!m) return;
+  (void)hipDeviceSynchronize();
+  dfree(m->d_vs);
+  dfree(m->d_us);
+  dfree(m->d_sv);
+  dfree(m->d_su);
+  dfree(m->d_rk);
+  delete m->prog;
+  dfree(m->arena);
+  delete m;
+}
+
+static long long pad32(long long x) { return (x + 31) & ~31LL; }
+
+static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
+  MixedFactors *m = new MixedFactors();
+  m->S = S;
+  const int nl = t->nleaf;
+  m->leaf_chunk0.assign(nl + 1, 0);
+  m->pad.push_back(0);
+  for (int c = 0; c < nl; c++) {
+    m->leaf_chunk0[c] = (int)m->row.size();
+    const int lo = t->leaf_off[c], mc = t->leaf_off[c + 1] - lo;
+    for (int r = 0; r < mc; r += MX_KCH) {
+      m->row.push_back(lo + r);
+      m->len.push_back(std::min(MX_KCH, mc - r));
+      m->pad.push_back(m->pad.back() + pad32(m->len.back()));
+    }
+  }
+  m->leaf_chunk0[nl] = (int)m->row.size();
+  m->npad = m->pad.back();
+  m->blk.resize(t->nblock);
+  long long voff = 0, vsoff = 0, uoff = 0, usoff = 0, rk = 0;
+  std::vector<int> rktab;
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    MixedFactors::Blk &x = m->blk[g];
+    const int span = 1 << (t->height - b.level);
+    const int src = b.j ^ 1;
+    x.q0 = m->leaf_chunk0[src * span];
+    x.q1 = m->leaf_chunk0[(src + 1) * span];
+    x.v_kp = m->pad[x.q1] - m->pad[x.q0];
+    x.v_off = voff;
+    voff += (long long)S * b.s * x.v_kp;
+    x.v_soff = vsoff;
+    vsoff += (long long)(x.q1 - x.q0) * b.s;
+    x.s_pad = (int)pad32(b.s);
+    x.nrk = (x.s_pad + MX_KCH - 1) / MX_KCH;
+    x.u_off = uoff;
+    uoff += (long long)S * b.m * x.s_pad;
+    x.u_soff = usoff;
+    usoff += (long long)x.nrk * b.m;
+    x.rk_cs = rk;
+    for (int c = 0; c <= x.nrk; c++) rktab.push_back(std::min(c * MX_KCH, x.s_pad));
+    rk += x.nrk + 1;
+  }
+  int rc = dalloc(&m->d_vs, (size_t)voff + 8);
+  if (rc == SUCCESS) rc = dalloc(&m->d_us, (size_t)uoff + 8);
+  if (rc == SUCCESS) rc = dalloc(&m->d_sv, (size_t)vsoff + 1);
+  if (rc == SUCCESS) rc = dalloc(&m->d_su, (size_t)usoff + 1);
+  if (rc == SUCCESS) rc = dalloc(&m->d_rk, rktab.size() + 1);
+  if (rc == SUCCESS && !rktab.empty()) {
+    hipError_t e = hipMemcpyAsync(m->d_rk, rktab.data(), rktab.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      set_error("mixed: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
+    }
+  }
+  // slice V (rows of V^T = rank columns, K = node rows by chunk) and U
+  // (rows = block rows, K = rank by chunks of MX_KCH)
+  std::vector<SliceItem> items;
+  for (int g = 0; g < t->nblock && rc == SUCCESS; g++) {
+    const BlockMeta &b = t->blk[g];
+    const MixedFactors::Blk &x = m->blk[g];
+    if (b.s <= 0) continue;
+    const int node_lo = m->row.empty() ? 0 : (x.q0 < (int)m->row.size() ? m->row[x.q0] : 0);
+    for (int r = 0; r < b.s; r++)
+      for (int q = x.q0; q < x.q1; q++) {
+        SliceItem it;
+        it.src = t->d_V + b.voff;
+        it.sp = b.n;
+        it.sk = 1;
+        it.dst = m->d_vs + x.v_off;
+        it.dst_slice = (long long)b.s * x.v_kp;
+        it.dst_ld = x.v_kp;
+        it.dk0 = m->pad[q] - m->pad[x.q0];
+        it.scale = m->d_sv + x.v_soff + (long long)(q - x.q0) * b.s + r;
+        it.p = r;
+        it.k0 = m->row[q] - node_lo;
+        it.klen = m->len[q];
+        it.pad = 0;
+        items.push_back(it);
+      }
+    for (int i = 0; i < b.m; i++)
+      for (int c = 0; c < x.nrk; c++) {
+        SliceItem it;
+        it.src = t->d_U + b.uoff;
+        it.sp = 1;
+        it.sk = b.m;
+        it.dst = m->d_us + x.u_off;
+        it.dst_slice = (long long)b.m * x.s_pad;
+        it.dst_ld = x.s_pad;
+        it.dk0 = (long long)c * MX_KCH;
+        it.scale = m->d_su + x.u_soff + (long long)c * b.m + i;
+        it.p = i;
+        it.k0 = c * MX_KCH;
+        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
+        it.pad = 0;
+        items.push_back(it);
+      }
+  }
+  if (rc == SUCCESS && !items.empty()) {
+    SliceItem *d_items = nullptr;
+    rc = dalloc(&d_items, items.size());
+    if (rc == SUCCESS) {
+      hipError_t e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(SliceItem),
+                                    hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) {
+        const int n = (int)items.size();
+        hipLaunchKernelGGL(k_slice, dim3((n + 3) / 4), dim3(256), 0, s, d_items, n, S);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        set_error("mixed: slicing factors: %s", hipGetErrorString(e));
+        rc = ALLOCATION_FAILURE;
+      }
+      dfree(d_items);
+    }
+  }
+  if (rc != SUCCESS) {
+    mixed_free(m);
+    return rc;
+  }
+  mixed_free(t->mixed);
+  t->mixed = m;
+  t->slices = S;
+  return SUCCESS;
+}
+
+// Chunk tables of program-built segments live in the program's own blob,
+// whose device address is known only at launch: fix[i] >= 0 is the byte
+// offset of segment i's table in the blob.
+__global__ __launch_bounds__(256) void k_mx_fix_segs(MxSeg *segs, const long long *fix,
+                                                     const unsigned char *blob, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n && fix[i] >= 0) segs[i].cs = reinterpret_cast<const int *>(blob + fix[i]);
+}
+
+// The per-call program: slice X, W GEMMs (+ reduce), slice W, the U W
+// GEMM over all levels.  The diagonal FP64 part runs before it.
+static int mixed_build(Program &p, Arena &ar, hm_dtree *t, int R, const double *X, long ldx,
+                       double *Y, long ldy) {
+  MixedFactors *m = t->mixed;
+  const int S = m->S;
+  const int nq = (int)m->row.size();
+  __bf16 *xs = ar.take<__bf16>((size_t)S * R * m->npad + 8);
+  double *sx = ar.take<double>((size_t)nq * R + 1);
+  p.touch(xs, ((size_t)S * R * m->npad) * 2, "mixed X slices");
+  std::vector<SliceItem> sl;
+  for (int q = 0; q < nq; q++)
+    for (int j = 0; j < R; j++) {
+      SliceItem it;
+      it.src = X;
+      it.sp = ldx;
+      it.sk = 1;
+      it.dst = xs;
+      it.dst_slice = (long long)R * m->npad;
+      it.dst_ld = m->npad;
+      it.dk0 = m->pad[q];
+      it.scale = sx + (long long)q * R + j;
+      it.p = j;
+      it.k0 = m->row[q];
+      it.klen = m->len[q];
+      it.pad = 0;
+      sl.push_back(it);
+    }
+  // W = V^T X per block and segment
+  std::vector<MxSeg> segs;
+  std::vector<MxItem> wit;
+  std::vector<MxReduce> red;
+  std::vector<int> ctab;  // segment chunk tables
+  std::vector<size_t> seg_ctab;
+  std::vector<double *> Wp(t->nblock, nullptr);
+  long long wtot = 0;
+  for (int g = 0; g < t->nblock; g++) wtot += (long long)t->blk[g].s * R;
+  double *W = ar.take<double>((size_t)wtot + 1);
+  std::vector<long long> woff(t->nblock, 0);
+  {
+    long long o = 0;
+    for (int g = 0; g < t->nblock; g++) {
+      woff[g] = o;
+      o += (long long)t->blk[g].s * R;
+    }
+  }
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    const MixedFactors::Blk &x = m->blk[g];
+    if (b.s <= 0) continue;
+    double *Wg = W + woff[g];
+    const int nch = x.q1 - x.q0;
+    const int nseg = std::max(1, (nch + MX_SEG_CHUNKS - 1) / MX_SEG_CHUNKS);
+    double *part = nseg > 1 ? ar.take<double>((size_t)nseg * b.s * R) : nullptr;
+    if (nch == 0) {  // empty source: W_b = 0
+      segs.push_back({nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, 0});
+      seg_ctab.push_back(ctab.size());
+      ctab.push_back(0);
+      for (int i0 = 0; i0 < b.s; i0 += MX_TM)
+        for (int j0 = 0; j0 < R; j0 += MX_TN)
+          wit.push_back({Wg, b.s, b.s, R, i0, j0, (int)segs.size() - 1, (int)segs.size() - 1, 0, 0});
+      continue;
+    }
+    for (int sg = 0; sg < nseg; sg++) {
+      const int c0 = sg * MX_SEG_CHUNKS, c1 = std::min(nch, c0 + MX_SEG_CHUNKS);
+      const long long base = m->pad[x.q0 + c0] - m->pad[x.q0];
+      seg_ctab.push_back(ctab.size());
+      for (int c = c0; c <= c1; c++) ctab.push_back((int)(m->pad[x.q0 + c] - m->pad[x.q0 + c0]));
+      MxSeg sd;
+      sd.A = m->d_vs + x.v_off + base;
+      sd.lda = x.v_kp;
+      sd.a_slice = (long long)b.s * x.v_kp;
+      sd.sA = m->d_sv + x.v_soff;
+      sd.ldsa = b.s;
+      sd.B = xs + m->pad[x.q0 + c0];
+      sd.ldb = m->npad;
+      sd.b_slice = (long long)R * m->npad;
+      sd.sB = sx;
+      sd.ldsb = R;
+      sd.cs = nullptr;
+      sd.nch = c1 - c0;
+      sd.a_c0 = c0;
+      sd.b_c0 = x.q0 + c0;
+      sd.pad = 0;
+      segs.push_back(sd);
+      double *C = nseg > 1 ? part + (long long)sg * b.s * R : Wg;
+      for (int i0 = 0; i0 < b.s; i0 += MX_TM)
+        for (int j0 = 0; j0 < R; j0 += MX_TN)
+          wit.push_back({C, b.s, b.s, R, i0, j0, (int)segs.size() - 1, (int)segs.size(), 0, 0});
+      p.touch(C, (size_t)b.s * R * sizeof(double), "mixed W");
+    }
+    if (nseg > 1) red.push_back({part, Wg, b.s, b.s, R, nseg, 0});
+  }
+  // slices of W and the apply segments
+  const int S_ = S;
+  std::vector<SliceItem> wsl;
+  std::vector<MxItem> ait;
+  std::vector<__bf16 *> Ws(t->nblock, nullptr);
+  std::vector<double *> sW(t->nblock, nullptr);
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    const MixedFactors::Blk &x = m->blk[g];
+    if (b.s <= 0) continue;
+    Ws[g] = ar.take<__bf16>((size_t)S_ * R * x.s_pad + 8);
+    sW[g] = ar.take<double>((size_t)x.nrk * R + 1);
+    for (int c = 0; c < x.nrk; c++)
+      for (int j = 0; j < R; j++) {
+        SliceItem it;
+        it.src = W + woff[g];
+        it.sp = b.s;
+        it.sk = 1;
+        it.dst = Ws[g];
+        it.dst_slice = (long long)R * x.s_pad;
+        it.dst_ld = x.s_pad;
+        it.dk0 = (long long)c * MX_KCH;
+        it.scale = sW[g] + (long long)c * R + j;
+        it.p = j;
+        it.k0 = c * MX_KCH;
+        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
+        it.pad = 0;
+        wsl.push_back(it);
+      }
+  }
+  const size_t seg_apply0 = segs.size();
+  for (int c = 0; c < t->nleaf; c++) {
+    const int lo = t->leaf_off[c], mc = t->leaf_off[c + 1] - lo;
+    if (mc <= 0) continue;
+    const int s0 = (int)segs.size();
+    for (int l = 1; l <= t->height; l++) {
+      const int q = c >> (t->height - l);
+      const int g = blk_index(l, q);
+      const BlockMeta &b = t->blk[g];
+      const MixedFactors::Blk &x = m->blk[g];
+      if (b.s <= 0) continue;
+      const int roff = lo - b.r0;
+      MxSeg sd;
+      sd.A = m->d_us + x.u_off + (long long)roff * x.s_pad;
+      sd.lda = x.s_pad;
+      sd.a_slice = (long long)b.m * x.s_pad;
+      sd.sA = m->d_su + x.u_soff + roff;
+      sd.ldsa = b.m;
+      sd.B = Ws[g];
+      sd.ldb = x.s_pad;
+      sd.b_slice = (long long)R * x.s_pad;
+      sd.sB = sW[g];
+      sd.ldsb = R;
+      sd.cs = m->d_rk + x.rk_cs;
+      sd.nch = x.nrk;
+      sd.a_c0 = 0;
+      sd.b_c0 = 0;
+      sd.pad = 0;
+      seg_ctab.push_back((size_t)-1);  // device table already resident
+      segs.push_back(sd);
+    }
+    const int s1 = (int)segs.size();
+    if (s1 == s0) continue;
+    for (int i0 = 0; i0 < mc; i0 += MX_TM)
+      for (int j0 = 0; j0 < R; j0 += MX_TN)
+        ait.push_back({Y + lo, ldy, mc, R, i0, j0, s0, s1, 1, 0});
+  }
+  (void)seg_apply0;
+  // chunk tables of the W segments live in the program blob: patch the
+  // device pointers at launch (blob base + offset)
+  const size_t ctab_off = p.push(ctab);
+  std::vector<size_t> seg_tab_off(segs.size());
+  for (size_t i = 0; i < segs.size(); i++) seg_tab_off[i] = seg_ctab[i];
+  const size_t sl_off = p.push(sl), wsl_off = p.push(wsl);
+  const size_t red_off = p.push(red);
+  const size_t wit_off = p.push(wit), ait_off = p.push(ait);
+  // segments: stored with cs = NULL for W segments; fixed up by a tiny
+  // host pass before upload (the blob base is only known on the device)
+  const size_t seg_off = p.push(segs);
+  const int nsl = (int)sl.size(), nwsl = (int)wsl.size(), nred = (int)red.size();
+  const unsigned nwit = (unsigned)wit.size(), nait = (unsigned)ait.size();
+  const int nsegs = (int)segs.size();
+  long long maxE = 1;
+  for (const auto &r : red) maxE = std::max(maxE, (long long)r.M * r.N);
+  // record the W-segment table offsets for the fix-up op
+  std::vector<long long> fix;
+  for (int i = 0; i < nsegs; i++)
+    fix.push_back(seg_ctab[i] == (size_t)-1 ? -1 : (long long)(ctab_off + seg_ctab[i] * sizeof(int)));
+  const size_t fix_off = p.push(fix);
+  p.add("mixed fixup", [=](hipStream_t st, const unsigned char *blob) {
+    hipLaunchKernelGGL(k_mx_fix_segs, dim3((nsegs + 255) / 256), dim3(256), 0, st,
+                       reinterpret_cast<MxSeg *>(const_cast<unsigned char *>(blob) + seg_off),
+                       reinterpret_cast<const long long *>(blob + fix_off), blob, nsegs);
+  });
+  p.add("slice X", [=](hipStream_t st, const unsigned char *blob) {
+    if (nsl)
+      hipLaunchKernelGGL(k_slice, dim3((nsl + 3) / 4), dim3(256), 0, st,
+                         reinterpret_cast<const SliceItem *>(blob + sl_off), nsl, S_);
+  });
+  p.add("mixed W", [=](hipStream_t st, const unsigned char *blob) {
+    if (!nwit) return;
+    const MxItem *I = reinterpret_cast<const MxItem *>(blob + wit_off);
+    const MxSeg *G = reinterpret_cast<const MxSeg *>(blob + seg_off);
+    if (S_ == 1) hipLaunchKernelGGL(k_sliced_gemm<1>, dim3(nwit), dim3(256), 0, st, I, G);
+    if (S_ == 2) hipLaunchKernelGGL(k_sliced_gemm<2>, dim3(nwit), dim3(256), 0, st, I, G);
+    if (S_ == 3) hipLaunchKernelGGL(k_sliced_gemm<3>, dim3(nwit), dim3(256), 0, st, I, G);
+    if (S_ == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nwit), dim3(256), 0, st, I, G);
+  });
+  p.add("mixed W reduce", [=](hipStream_t st, const unsigned char *blob) {
+    if (nred)
+      hipLaunchKernelGGL(k_mx_reduce, dim3(nred, (unsigned)std::min<long long>(64, (maxE + 255) / 256)),
+                         dim3(256), 0, st, reinterpret_cast<const MxReduce *>(blob + red_off));
+  });
+  p.add("slice W", [=](hipStream_t st, const unsigned char *blob) {
+    if (nwsl)
+      hipLaunchKernelGGL(k_slice, dim3((nwsl + 3) / 4), dim3(256), 0, st,
+                         reinterpret_cast<const SliceItem *>(blob + wsl_off), nwsl, S_);
+  });
+  p.add("mixed apply", [=](hipStream_t st, const unsigned char *blob) {
+    if (!nait) return;
+    const MxItem *I = reinterpret_cast<const MxItem *>(blob + ait_off);
+    const MxSeg *G = reinterpret_cast<const MxSeg *>(blob + seg_off);
+    if (S_ == 1) hipLaunchKernelGGL(k_sliced_gemm<1>, dim3(nait), dim3(256), 0, st, I, G);
+    if (S_ == 2) hipLaunchKernelGGL(k_sliced_gemm<2>, dim3(nait), dim3(256), 0, st, I, G);
+    if (S_ == 3) hipLaunchKernelGGL(k_sliced_gemm<3>, dim3(nait), dim3(256), 0, st, I, G);
+    if (S_ == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nait), dim3(256), 0, st, I, G);
+  });
+  p.touch(X, ((size_t)(R - 1) * ldx + t->n) * sizeof(double), "mixed X");
+  p.touch(Y, ((size_t)(R - 1) * ldy + t->n) * sizeof(double), "mixed Y");
+  return SUCCESS;
+}
+
+int mixed_matmat(hm_dtree *t, int R, const double *X, long ldx, double *Y, long ldy,
+                 hipStream_t s) {
+  MixedFactors *m = t->mixed;
+  if (!m) {
+    set_error("mixed product: no sliced factors");
+    return INPUT_ERROR;
+  }
+  HM_OK(products_diag(t, R, X, ldx, Y, ldy, s));
+  if (!m->prog || m->R != R || m->X != X || m->Y != Y || m->ldx != ldx || m->ldy != ldy) {
+    delete m->prog;
+    m->prog = nullptr;
+    dfree(m->arena);
+    m->arena = nullptr;
+    Program dry;
+    Arena count;
+    dry.record = true;
+    HM_OK(mixed_build(dry, count, t, R, X, ldx, Y, ldy));
+    dry.allow(Arena::fake_base(), count.off + 256);
+    dry.allow(X, ((size_t)(R - 1) * ldx + t->n) * sizeof(double));
+    dry.allow(Y, ((size_t)(R - 1) * ldy + t->n) * sizeof(double));
+    std::string msg;
+    if (dry.audit(&msg) != SUCCESS) {
+      set_error("mixed product: audit: %s", msg.c_str());
+      return INPUT_ERROR;
+    }
+    Arena ar;
+    ar.counting = false;
+    ar.cap = count.off + 256;
+    HM_OK(dalloc(&ar.base, ar.cap));
+    m->arena = ar.base;
+    m->prog = new Program();
+    int rc = mixed_build(*m->prog, ar, t, R, X, ldx, Y, ldy);
+    if (rc == SUCCESS) rc = m->prog->upload(s);
+    if (rc != SUCCESS) {
+      delete m->prog;
+      m->prog = nullptr;
+      return rc;
+    }
+    m->R = R;
+    m->X = X;
+    m->Y = Y;
+    m->ldx = ldx;
+    m->ldy = ldy;
+  }
+  return m->prog->run(s);
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" int hm_hip_tree_set_precision(hm_dtree *t, int slices, void *stream) {
+  HM_OK(ensure_init());
+  if (!t || slices < 0 || slices > 4) {
+    set_error("hm_hip_tree_set_precision: slices must be 0 (FP64) or 1..4");
+    return INPUT_ERROR;
+  }
+  if (slices == 0) {
+    mixed_free(t->mixed);
+    t->mixed = nullptr;
+    t->slices = 0;
+    return SUCCESS;
+  }
+  return mixed_prepare(t, slices, stream_of(stream));
+}

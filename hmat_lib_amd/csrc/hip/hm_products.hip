@@ -635,6 +635,19 @@ static int run_reduce(hm_dtree *t, const ProdArgs &a, hipStream_t s) {
   return SUCCESS;
 }
 
+int products_diag(hm_dtree *t, int R, const double *X, long x_cs, double *Y, long y_cs,
+                  hipStream_t s) {
+  if (R <= 0) return SUCCESS;
+  HM_OK(ensure_scratch(t, R));
+  ProdArgs a = make_args(t, 0, R, X, 1, x_cs, Y, 1, y_cs);
+  a.h = 0;  // no off-diagonal levels: the tile computes D_c X_c only
+  Timed ta = timing_begin("mm_diag", s);
+  run_apply(t, a, 0, false, s);
+  timing_end(ta, s);
+  HM_LAUNCH_CHECK();
+  return SUCCESS;
+}
+
 int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
                  long x_cs, double *Y, long y_rs, long y_cs, hipStream_t s) {
   if (!t) {
@@ -646,6 +659,8 @@ int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
     set_error("product: NULL operand");
     return INPUT_ERROR;
   }
+  if (t->slices > 0 && !trans && R > 1 && x_rs == 1 && y_rs == 1)
+    return mixed_matmat(t, R, X, x_cs, Y, y_cs, s);
   HM_OK(ensure_scratch(t, R));
   ProdArgs a = make_args(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs);
   const int nchunk = t->plan.nchunk;
@@ -728,6 +743,44 @@ __global__ void k_selftest_mfma(double *out) {
   }
 }
 }  // namespace hm
+
+namespace hm {
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+// One v_mfma_f32_16x16x32_bf16: lane L takes A[L%16][8(L/16)+e] and
+// B[8(L/16)+e][L%16], e = 0..7 (A 16x32, B 32x16 row-major, values exact in
+// bf16); raw[4 L + v] receives accumulator value v of lane L.
+__global__ void k_selftest_bf16(const float *A, const float *B, float *raw) {
+  const int L = threadIdx.x;
+  bf8v a, b;
+  for (int e = 0; e < 8; e++) {
+    a[e] = (__bf16)A[(L & 15) * 32 + 8 * (L >> 4) + e];
+    b[e] = (__bf16)B[(8 * (L >> 4) + e) * 16 + (L & 15)];
+  }
+  f4v c = {0.f, 0.f, 0.f, 0.f};
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  for (int v = 0; v < 4; v++) raw[4 * L + v] = c[v];
+}
+}  // namespace hm
+
+extern "C" int hm_hip_selftest_mfma_bf16(const float *hA, const float *hB, float *raw) {
+  HM_OK(ensure_init());
+  float *d = nullptr;
+  HM_OK(dalloc(&d, 512 + 512 + 256));
+  hipError_t e = hipMemcpy(d, hA, 512 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + 512, hB, 512 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_selftest_bf16, dim3(1), dim3(64), 0, 0, d, d + 512, d + 1024);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(raw, d + 1024, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  dfree(d);
+  if (e != hipSuccess) {
+    set_error("selftest bf16: %s", hipGetErrorString(e));
+    return ALLOCATION_FAILURE;
+  }
+  return SUCCESS;
+}
 
 extern "C" int hm_hip_selftest_mfma(double *out) {
   HM_OK(ensure_init());

@@ -11,6 +11,8 @@ namespace hm {
 
 void tree_free(hm_dtree *t) {
   if (!t) return;
+  mixed_free(t->mixed);
+  t->mixed = nullptr;
   dfree(t->d_diag);
   dfree(t->d_U);
   dfree(t->d_V);

@@ -143,6 +143,10 @@ class DeviceTree:
         capi.check(capi.hip().hm_hip_matmat(self.h, trans, R, dX.ptr, xs[0], xs[1],
                                             dY.ptr, ys[0], ys[1], None), "matmat")
 
+    def set_precision(self, slices: int):
+        """0: FP64 products; 1..4: BF16-sliced off-diagonal products."""
+        capi.check(capi.hip().hm_hip_tree_set_precision(self.h, slices, None), "set_precision")
+
     def download(self, lib, tree_ptr):
         capi.check(capi.hip().hm_hip_tree_download(self.h, tree_ptr, None, None),
                    "tree download")

@@ -98,6 +98,16 @@ HM_HIP_API int hm_hip_matmat(hm_dtree *tree, int trans, int R, const double *d_X
                   long x_rs, long x_cs, double *d_Y, long y_rs, long y_cs,
                   void *stream);
 
+/* Precision of the off-diagonal products of this tree: 0 = FP64 (default);
+ * 1..4 = the factors split into that many BF16 slices (integers with a
+ * shared power-of-two scale per row of 128-entry chunks), products of
+ * slice pairs summed exactly in FP32 on MFMA and folded in FP64 (pairs
+ * beyond the slice count dropped).  Accuracy relative to the chunk maxima
+ * is ~2^-8 per slice: 2^-8 (1), 2^-16 (2), 2^-24 (3), 2^-32 (4); the
+ * diagonal leaves stay FP64.  Applies to Y = H X with column-major X, Y and R > 1; other forms
+ * keep FP64.  Builds the sliced copies of U and V on the device. */
+HM_HIP_API int hm_hip_tree_set_precision(hm_dtree *tree, int slices, void *stream);
+
 /* Apply-kernel tile variant the autotuner picked for this tree, transpose
  * mode and RHS width (-1 before the first product of that kind). */
 HM_HIP_API int hm_hip_matmat_variant(const hm_dtree *tree, int trans, int R);
@@ -234,6 +244,10 @@ HM_HIP_API int hm_hip_batch_destroy(hm_batch *b);
  * B(k,j) = 1 + 2 j - k, read back through the lane layout the kernels
  * assume; out (host, 256 doubles) receives D row-major. */
 HM_HIP_API int hm_hip_selftest_mfma(double *out);
+/* One v_mfma_f32_16x16x32_bf16 with lane L taking A[L%16][8(L/16)+e] and
+ * B[8(L/16)+e][L%16] (hA 16x32, hB 32x16 row-major host floats, exact in
+ * bf16); raw (host, 256) receives value v of lane L at 4 L + v. */
+HM_HIP_API int hm_hip_selftest_mfma_bf16(const float *hA, const float *hB, float *raw);
 
 /* Host-only check (no device needed) of the wide-panel QR launch program:
  * builds it for njobs panels of m[i] x L[i] (with the triangular factor

@@ -33,6 +33,25 @@ def test_mfma_lane_layout(gpu):
     np.testing.assert_array_equal(got, A @ B)
 
 
+def test_mfma_bf16_lane_layout(gpu):
+    """v_mfma_f32_16x16x32_bf16 lane layout the sliced kernels assume: lane L
+    holds A[L%16][8(L/16)+e], B[8(L/16)+e][L%16]; value v of lane L is
+    D[4(L/16)+v][L%16].  Integer inputs make every product exact."""
+    _say("case: mfma bf16 selftest")
+    rng = np.random.default_rng(0)
+    A = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)
+    B = rng.integers(-8, 9, size=(32, 16)).astype(np.float32)
+    raw = np.zeros(256, dtype=np.float32)
+    FP = C.POINTER(C.c_float)
+    capi.check(capi.hip().hm_hip_selftest_mfma_bf16(A.ctypes.data_as(FP), B.ctypes.data_as(FP),
+                                                    raw.ctypes.data_as(FP)), "selftest bf16")
+    L = np.arange(64)
+    got = np.zeros((16, 16), dtype=np.float32)
+    for v in range(4):
+        got[4 * (L // 16) + v, L % 16] = raw[4 * L + v]
+    np.testing.assert_array_equal(got, A @ B)
+
+
 CASES = [
     ("uniform h3 s2", [32] * 8, lambda l, j: 2),
     ("uniform h4 mixed", [64] * 16, lambda l, j: 1 + (l + j) % 5),
