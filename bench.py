@@ -12,6 +12,13 @@ HIP events recorded around it on its launch stream during the timed
 region.  cpu_baseline: the reference hmat_lib's multiply_hodlr_dense
 (built from its sources, MKL LAPACK/BLAS) on the same tree on the host
 cores, over a bounded number of calls.
+
+"additions" reports the three extensions on their own workloads (each
+measured here, none part of `value`): the batched product and batched
+compression of 512 uneven trees vs per-tree calls, the BF16-sliced
+off-diagonal products (1..4 slices) vs FP64 on the 2-D inverse-distance
+kernel, and the learned second sketch width vs the plain doubling on
+Matern-3/2 matrices.
 """
 import argparse
 import ctypes as C
@@ -45,6 +52,216 @@ def cpu_cores():
     return max(1, min(16, n))
 
 
+def _dense_rows(kind, pts, ell, shift, rows):
+    """Rows of the FP64 kernel matrix from the generator's formula (host)."""
+    import numpy as np
+    p = pts if pts.ndim > 1 else pts[:, None]
+    d = np.sqrt(((p[rows][:, None, :] - p[None, :, :]) ** 2).sum(-1))
+    if kind == 0:
+        a = np.exp(-d / ell)
+    elif kind == 1:
+        a = 1.0 / (d + ell)
+    else:
+        t = np.sqrt(3.0) * d / ell
+        a = (1.0 + t) * np.exp(-t)
+    a[np.arange(len(rows)), rows] += shift
+    return a
+
+
+def batch_leg(D, WL, np):
+    """512 uneven exp-kernel trees (leaf 128, thr 1e-8, R_i log-uniform in
+    [1, 128]): batched compression and the batched product (HIP graph
+    replay of the tuned launch program) vs per-tree calls of the same
+    library, FP64."""
+    specs = WL.batch_workloads(512)
+    mats, pts_l = [], []
+    for sp in specs:
+        pts = WL.points_1d_sorted(sp["seed"], sp["n"])
+        pts_l.append(pts)
+        mats.append(D.fill_kernel_matrix(0, pts, sp["ell"], 0.0))
+    D.synchronize()
+    hs = [sp["height"] for sp in specs]
+    t0 = time.perf_counter()
+    trees, st = D.compress_batch(mats, hs, 1e-8)
+    D.synchronize()
+    tb = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    singles = [D.DeviceTree.compress(a, sp["n"], sp["height"], 1e-8) for a, sp in zip(mats, specs)]
+    D.synchronize()
+    tp = (time.perf_counter() - t0) * 1e3
+    for t in singles:
+        t.free()
+    for a in mats:
+        a.free()
+    ranks = np.concatenate([t.ranks() for t in trees])
+    rng = np.random.default_rng(1)
+    xs, ys, ys2, xh = [], [], [], []
+    for sp in specs:
+        x = np.asfortranarray(rng.standard_normal((sp["n"], sp["rhs"])))
+        xh.append(x)
+        xs.append(D.DeviceArray.from_numpy(x))
+        ys.append(D.DeviceArray((sp["n"], sp["rhs"])))
+        ys2.append(D.DeviceArray((sp["n"], sp["rhs"])))
+    b = D.Batch(trees, xs, ys)
+    b.run()
+    b.tune()
+    for _ in range(3):
+        b.run(graph=True)
+    D.synchronize()
+    steps = 20
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(graph=True)
+    D.synchronize()
+    t_b = (time.perf_counter() - t0) / steps
+
+    def loop():
+        for t, x, y in zip(trees, xs, ys2):
+            if x.shape[1] == 1:
+                t.matvec(x, y)
+            else:
+                t.matmat(x, y)
+    loop()
+    D.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        loop()
+    D.synchronize()
+    t_l = (time.perf_counter() - t0) / 5
+    work = sum(sp["n"] * sp["rhs"] for sp in specs)
+    diff = max(float(np.linalg.norm(y.numpy() - y2.numpy()) / max(np.linalg.norm(y2.numpy()), 1e-300))
+               for y, y2 in zip(ys, ys2))
+    errs = []
+    for i in range(0, 512, 64):
+        sp = specs[i]
+        rows = np.sort(rng.choice(sp["n"], 32, replace=False))
+        ref = _dense_rows(0, pts_l[i], sp["ell"], 0.0, rows) @ xh[i]
+        errs.append(float(np.linalg.norm(ys[i].numpy()[rows] - ref) / np.linalg.norm(ref)))
+    out = {"workload": "512 trees, N log-uniform [512, 8192] (multiple of 64), exp(-r/ell) "
+                       "ell ~ U[0.02, 0.5] on sorted points, leaf 128, thr 1e-8, "
+                       "R log-uniform [1, 128], FP64",
+           "rows_rhs_per_s": work / t_b, "us_per_step": t_b * 1e6,
+           "per_tree_loop_rows_rhs_per_s": work / t_l, "per_tree_loop_us": t_l * 1e6,
+           "speedup_vs_per_tree": t_l / t_b,
+           "compress_ms": tb, "per_matrix_compress_ms": tp, "compress_speedup": tp / tb,
+           "launches_per_product": len(b.variants()) + 1, "tile_variants": b.variants(),
+           "rank_histogram": {str(k): int(v) for k, v in enumerate(np.bincount(ranks)) if v},
+           "max_rel_diff_batched_vs_per_tree": diff,
+           "max_rel_err_vs_dense_sampled": max(errs)}
+    b.free()
+    for t in trees:
+        t.free()
+    log(f"batch: {out['rows_rhs_per_s']:.3e} rows*RHS/s ({out['speedup_vs_per_tree']:.1f}x per-tree), "
+        f"compress {tb:.0f} ms ({out['compress_speedup']:.1f}x)")
+    return out
+
+
+def precision_leg(D, WL, np):
+    """inv2d_64k (N=65536 Morton 2-D points, 1/(r+1e-2), height 9, thr 1e-8,
+    R=256): FP64 products vs 1..4 BF16 slices for the off-diagonal factors;
+    error vs the FP64 product of the same tree and vs FP64 dense rows."""
+    w = WL.WORKLOADS["inv2d_64k"]
+    n, h, R = w.n, w.height, w.rhs
+    pts = w.points()
+    da = D.fill_kernel_matrix(w.kind, pts, w.ell, w.shift)
+    opts = D.default_compress_opts()
+    opts.sketch_max = 1024
+    t0 = time.perf_counter()
+    dt = D.DeviceTree.compress(da, n, h, w.thr, opts=opts)
+    D.synchronize()
+    cms = (time.perf_counter() - t0) * 1e3
+    da.free()
+    x = w.rhs_matrix(R)
+    dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((n, R))
+    rng = np.random.default_rng(7)
+    rows = np.sort(rng.choice(n, 64, replace=False))
+    ref_rows = _dense_rows(w.kind, pts, w.ell, w.shift, rows) @ x
+
+    def timed(steps=10):
+        for _ in range(2):
+            dt.matmat(dx, dy)
+        D.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dt.matmat(dx, dy)
+        D.synchronize()
+        return (time.perf_counter() - t0) / steps
+    t64 = timed()
+    y64 = dy.numpy()
+    e64 = float(np.linalg.norm(y64[rows] - ref_rows) / np.linalg.norm(ref_rows))
+    table = {"f64": {"us_per_step": t64 * 1e6, "rows_rhs_per_s": n * R / t64,
+                     "rel_err_vs_fp64_tree": 0.0, "rel_err_vs_dense_rows": e64}}
+    for S in (1, 2, 3, 4):
+        dt.set_precision(S)
+        t = timed()
+        y = dy.numpy()
+        table[f"bf16x{S}"] = {
+            "us_per_step": t * 1e6, "rows_rhs_per_s": n * R / t, "speedup_vs_f64": t64 / t,
+            "rel_err_vs_fp64_tree": float(np.linalg.norm(y - y64) / np.linalg.norm(y64)),
+            "rel_err_vs_dense_rows": float(np.linalg.norm(y[rows] - ref_rows) / np.linalg.norm(ref_rows))}
+    dt.set_precision(0)
+    ranks = dt.ranks()
+    dt.free()
+    log("precision: " + ", ".join(f"{k} {v['us_per_step']:.0f} us err {v['rel_err_vs_fp64_tree']:.1e}"
+                                  for k, v in table.items()))
+    return {"workload": "inv2d_64k: N=65536 Morton U[0,1]^2, 1/(r+1e-2), height 9 (leaf 128), "
+                        "thr 1e-8, R=256",
+            "compress_ms": cms, "max_rank": int(ranks.max()), "per_dtype": table}
+
+
+def learned_leg(D, WL, np):
+    """Matern-3/2 N=32768 (height 8, thr 1e-8) at held-out seeded length
+    scales: compression with the learned second sketch width vs the plain
+    doubling path (same residual test, same sketch cap)."""
+    from hmat_lib_amd import learned as Lm
+    model = Lm.RankModel.load()
+    u = WL.uniform(424242, 3)
+    cases = []
+    tot_p = tot_l = 0.0
+    for i, uu in enumerate(u):
+        ell = 0.01 + 0.99 * float(uu)
+        n, h = 32768, 8
+        pts = WL.points_2d_morton(9100 + i, n)
+        da = D.fill_kernel_matrix(2, pts, ell, 0.0)
+        res = {}
+        for mode in ("plain", "learned", "plain", "learned"):  # second pass timed
+            t0 = time.perf_counter()
+            t, _ = Lm.compress(da, pts, h, 1e-8, ell, model if mode == "learned" else None)
+            D.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            if mode in res:
+                res[mode]["tree"].free()
+            res[mode] = {"ms": ms, "tree": t}
+        rows = np.sort(np.random.default_rng(i).choice(n, 48, replace=False))
+        x = np.asfortranarray(np.random.default_rng(50 + i).standard_normal((n, 4)))
+        ref = _dense_rows(2, pts, ell, 0.0, rows) @ x
+        dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((n, 4))
+        errs = {}
+        for mode in ("plain", "learned"):
+            res[mode]["tree"].matmat(dx, dy)
+            errs[mode] = float(np.linalg.norm(dy.numpy()[rows] - ref) / np.linalg.norm(ref))
+        sp_, sl_ = res["plain"]["tree"].stats, res["learned"]["tree"].stats
+        rp, rl = res["plain"]["tree"].ranks(), res["learned"]["tree"].ranks()
+        cases.append({"ell": ell, "plain_ms": res["plain"]["ms"], "learned_ms": res["learned"]["ms"],
+                      "speedup": res["plain"]["ms"] / res["learned"]["ms"],
+                      "plain_rounds": sp_.sketch_rounds, "learned_rounds": sl_.sketch_rounds,
+                      "predicted_blocks": sl_.predicted_blocks,
+                      "predicted_failed": sl_.predicted_failed,
+                      "rank_mismatches": int((rp != rl).sum()), "max_rank": int(rp.max()),
+                      "rel_err_plain": errs["plain"], "rel_err_learned": errs["learned"]})
+        tot_p += res["plain"]["ms"]
+        tot_l += res["learned"]["ms"]
+        for mode in ("plain", "learned"):
+            res[mode]["tree"].free()
+        da.free()
+    model.free()
+    log(f"learned: plain {tot_p:.0f} ms vs learned {tot_l:.0f} ms ({tot_p / tot_l:.2f}x)")
+    return {"workload": "Matern-3/2 N=32768 Morton U[0,1]^2, height 8, thr 1e-8, held-out "
+                        "ell ~ U[0.01, 1] (3 seeded), sketch cap 1024",
+            "plain_ms_total": tot_p, "learned_ms_total": tot_l, "speedup": tot_p / tot_l,
+            "cases": cases}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +270,8 @@ def main():
     ap.add_argument("--workload", default="exp1d_64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched, mixed-precision and learned-sketch legs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 10 device-to-device copies of this many MiB "
                          "(the known-size kernel that calibrates HBM byte counters)")
@@ -214,6 +433,17 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {e}")
 
+    dt.free()
+    extras = {}
+    if not args.no_extras:
+        for name, leg in (("batched", batch_leg), ("precision", precision_leg),
+                          ("learned_sketch", learned_leg)):
+            try:
+                extras[name] = leg(D, WL, np)
+            except Exception as e:  # a failed leg is reported, the line still prints
+                log(f"{name} leg failed: {e}")
+                extras[name] = {"error": str(e)}
+
     line = {
         "metric": "HODLR matmat rows*RHS/s (FP64, rel err <= 1e-8 vs FP64 dense)",
         "value": value,
@@ -239,6 +469,7 @@ def main():
         "rel_err_vs_dense": rel_err,
         "rel_err_vs_reference_same_tree": ref_same_tree_err,
         "matvec_us": matvec_us,
+        "additions": extras,
     }
     print(json.dumps(line), flush=True)
 

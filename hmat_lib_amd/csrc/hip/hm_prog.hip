@@ -9,17 +9,27 @@ namespace hm {
 
 // ------------------------------------------------------------- program
 int Program::audit(std::string *msg) const {
+  // allowed ranges sorted by start, with the running maximum of their ends:
+  // a touched range is covered iff some allowed range starting at or before
+  // it ends at or after it; ranges covered only by a union of several
+  // allowed ranges are rejected (every item stays in one buffer)
+  std::vector<std::pair<const char *, const char *>> al;
+  for (const auto &a : allowed) {
+    const char *alo = static_cast<const char *>(a.lo);
+    al.push_back({alo, alo + a.bytes});
+  }
+  std::sort(al.begin(), al.end());
+  std::vector<const char *> maxend(al.size());
+  for (size_t i = 0; i < al.size(); i++)
+    maxend[i] = i ? std::max(maxend[i - 1], al[i].second) : al[i].second;
   for (const auto &t : touched) {
     const char *lo = static_cast<const char *>(t.first.lo);
     const char *hi = lo + t.first.bytes;
-    bool ok = false;
-    for (const auto &a : allowed) {
-      const char *alo = static_cast<const char *>(a.lo);
-      if (lo >= alo && hi <= alo + a.bytes) {
-        ok = true;
-        break;
-      }
-    }
+    // the allowed range with the furthest end among those starting at or
+    // before lo covers [lo, hi) iff that end reaches hi
+    auto it = std::upper_bound(al.begin(), al.end(),
+                               std::make_pair(lo, (const char *)UINTPTR_MAX));
+    const bool ok = it != al.begin() && hi <= maxend[(it - al.begin()) - 1];
     if (!ok) {
       if (msg) {
         char buf[256];

@@ -19,7 +19,7 @@ struct TsvdBlock {
   int rule;
   int status;    // out: 1 accepted
   int k;         // out: rank
-  int need;      // out: singular values of the core >= accept/2 * thr (* sigma_0)
+  int need;      // out: singular values of the core >= accept/20 * thr (* sigma_0)
   double thr;
   double err;    // out: residual estimate (Frobenius-type bound)
   double perr;   // out: residual estimate after one power step (spectral)

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
       B.err = fmin(est, pest);
     }
     int need = 0;
-    while (need < nc && sig[perm[need]] >= 0.5 * scale) need++;
+    while (need < nc && sig[perm[need]] >= 0.05 * scale) need++;
     B.need = need;
     B.status = ok ? 1 : 0;
   }

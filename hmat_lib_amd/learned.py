@@ -2,8 +2,9 @@
 
 A small MLP (hm_hip_mlp_*: FP32, nin -> 64 -> 64 -> 1, tanh) predicts, for
 an off-diagonal block, log2(w + 1) where w is the number of its singular
-values at or above accept/2 * thr * sigma_0 - the sketch width its residual
-test needs (the rank at a threshold 20x tighter) - from
+values at or above accept/20 * thr * sigma_0 (the rank at a threshold 200x
+tighter): with 8 columns of oversampling a sketch of that width leaves a
+residual below the test's accept * thr * sigma_0 - from
   * 7 geometric inputs: log2 of the block's row and column counts, its
     level, the distance between the centroids of its row and column points,
     the kernel length scale, and the bounding-box diagonals of the two point

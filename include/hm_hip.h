@@ -132,7 +132,7 @@ typedef struct hm_compress_opts {
    * width the model predicts from block_features (host, nblock x n_extra,
    * row-major, level-major blocks) and the first round's singular values
    * (hm_hip_rank_features): with y the model output and k = 2^y - 1 the
-   * predicted count of singular values at or above accept/2 * thr * sigma_0
+   * predicted count of singular values at or above accept/20 * thr * sigma_0
    * (the width the residual test needs), ceil(k (1 + margin)) + oversample
    * columns instead of doubling; the residual test still decides, and a
    * failed prediction goes on doubling. */
@@ -142,7 +142,7 @@ typedef struct hm_compress_opts {
   double margin;        /* default 0.1 */
   double *first_sigma;  /* host out (nblock x 16): first round's singular values, or NULL */
   int *need_out;        /* host out (nblock): singular values of the final core at or above
-                           accept/2 * thr * sigma_0 (the training label), or NULL */
+                           accept/20 * thr * sigma_0 (the training label), or NULL */
 } hm_compress_opts;
 
 typedef struct hm_compress_stats {
