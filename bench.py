@@ -144,7 +144,7 @@ def batch_leg(D, WL, np):
            "per_tree_loop_rows_rhs_per_s": work / t_l, "per_tree_loop_us": t_l * 1e6,
            "speedup_vs_per_tree": t_l / t_b,
            "compress_ms": tb, "per_matrix_compress_ms": tp, "compress_speedup": tp / tb,
-           "launches_per_product": len(b.variants()) + 1, "tile_variants": b.variants(),
+           "tile_variants": b.variants(),
            "rank_histogram": {str(k): int(v) for k, v in enumerate(np.bincount(ranks)) if v},
            "max_rel_diff_batched_vs_per_tree": diff,
            "max_rel_err_vs_dense_sampled": max(errs)}
@@ -272,6 +272,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, mixed-precision and learned-sketch legs")
+    ap.add_argument("--legs", default="batched,precision,learned_sketch",
+                    help="comma-separated subset of the extra legs to run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 10 device-to-device copies of this many MiB "
                          "(the known-size kernel that calibrates HBM byte counters)")
@@ -438,6 +440,8 @@ def main():
     if not args.no_extras:
         for name, leg in (("batched", batch_leg), ("precision", precision_leg),
                           ("learned_sketch", learned_leg)):
+            if name not in args.legs.split(","):
+                continue
             try:
                 extras[name] = leg(D, WL, np)
             except Exception as e:  # a failed leg is reported, the line still prints

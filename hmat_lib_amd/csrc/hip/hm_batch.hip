@@ -5,16 +5,18 @@
 // (src/dense_algebra.c:82-429).  A batch here is one launch program, built
 // once for fixed operands:
 //
-//   W       W_b = F_in^T X_src  for every block of every tree
-//   D       Y_c = D_c X_c       (D_c^T for H^T X) for every leaf of every tree
-//   level l Y_rows(b) += F_out W_b for every block of level l of every tree
+//   W       W_b = F_in^T X_src  for every block of every tree (split-K, then
+//           a fixed-order sum of the segments where K is long)
+//   apply   Y_c = op(D_c) X_c + sum over the levels of F_out(rows of c) W_b
+//           for every leaf of every tree: one chained GEMM per leaf whose K
+//           concatenates the leaf and the (level, rank) pairs
 //
-// so a product over the whole batch is 2 + max height launches of the
-// persistent batched GEMM kernel, whatever the number of trees.  Blocks of
-// one level write disjoint rows of their tree's Y, so each level is one
-// launch; the launches are ordered on the stream.  Tile shapes of every
-// launch are chosen by timing (program_tune), and the program can be
-// captured once into a HIP graph and replayed.
+// so a product over the whole batch is 2-3 launches of the persistent
+// batched GEMM kernels, whatever the number of trees or levels, and Y is
+// written once.  (A launch per level was measured first: Y is then read and
+// written once per level, 2.25 ms for the 512-tree workload.)  Tile shapes
+// of every launch are chosen by timing (program_tune), and the program can
+// be captured once into a HIP graph and replayed.
 #include <algorithm>
 #include <cstring>
 
@@ -44,10 +46,9 @@ struct BatchOperands {
 
 static int build_batch(Program &p, Arena &ar, const std::vector<hm_dtree *> &trees, int trans,
                        const std::vector<BatchOperands> &ops) {
-  std::vector<GemmJob> wj, dj;
-  int maxh = 0;
-  for (auto *t : trees) maxh = std::max(maxh, t->height);
-  std::vector<std::vector<GemmJob>> lj(maxh + 1);
+  std::vector<GemmJob> wj;
+  std::vector<GemmChain> chains;
+  std::vector<GemmLink> links;
   for (size_t i = 0; i < trees.size(); i++) {
     const hm_dtree *t = trees[i];
     const BatchOperands &o = ops[i];
@@ -63,22 +64,36 @@ static int build_batch(Program &p, Arena &ar, const std::vector<hm_dtree *> &tre
       // W_b (s x R, ld s) = F^T X[src : src + K, :]
       wj.push_back({F, o.X + src, W[g], nullptr, std::max(K, 1), o.ldx, b.s, b.s, o.R, K, 1, 0,
                     0, 1.0, 0.0});
-      const double *G = trans ? t->d_V + b.voff : t->d_U + b.uoff;
-      const int rows = trans ? b.n : b.m;
-      const int dst = trans ? b.c0 : b.r0;
-      lj[b.level].push_back({G, W[g], o.Y + dst, nullptr, std::max(rows, 1), b.s, o.ldy, rows,
-                             o.R, b.s, 1, 0, 0, 1.0, 1.0});
     }
+    // Y_c = op(D_c) X_c + sum over levels of the output factor's rows of
+    // leaf c times W_b, one chain per leaf
     for (int c = 0; c < t->nleaf; c++) {
       const int lo = t->leaf_off[c], mc = t->leaf_off[c + 1] - lo;
       if (mc <= 0) continue;
-      dj.push_back({t->d_diag + t->diag_off[c], o.X + lo, o.Y + lo, nullptr, mc, o.ldx, o.ldy,
-                    mc, o.R, mc, 1, 0, 0, 1.0, 0.0});
+      GemmChain ch;
+      ch.C = o.Y + lo;
+      ch.ldc = o.ldy;
+      ch.M = mc;
+      ch.N = o.R;
+      ch.link0 = (int)links.size();
+      links.push_back({t->d_diag + t->diag_off[c], o.X + lo, mc, o.ldx, mc, trans ? 1 : 0});
+      for (int l = 1; l <= t->height; l++) {
+        const int q = c >> (t->height - l);
+        // H X: block (l, q) has the leaf's rows (U); H^T X: block (l, q^1)
+        // has them as columns (V)
+        const int g = blk_index(l, trans ? (q ^ 1) : q);
+        const BlockMeta &b = t->blk[g];
+        if (b.s <= 0) continue;
+        const double *G = trans ? t->d_V + b.voff + (lo - b.c0) : t->d_U + b.uoff + (lo - b.r0);
+        const long long ld = trans ? b.n : b.m;
+        links.push_back({G, W[g], ld, b.s, b.s, 0});
+      }
+      ch.nlink = (int)links.size() - ch.link0;
+      chains.push_back(ch);
     }
   }
   HM_OK(gemm_add(p, ar, wj, true, false, "batch W", -1));
-  HM_OK(gemm_add(p, ar, dj, trans != 0, false, "batch diag", -1));
-  for (int l = maxh; l >= 1; l--) HM_OK(gemm_add(p, ar, lj[l], false, false, "batch level", -1));
+  HM_OK(gemm_chain_add(p, chains, links, "batch apply", -1));
   return SUCCESS;
 }
 

@@ -107,6 +107,22 @@ static constexpr int GEMM_MAX_GRID = 4096;
 int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
              const char *name, int variant = 0);
 
+// Chained GEMM: C (M x N) = sum over its links of op(A_i) B_i, the links
+// concatenated along K (B column-major; op(A) = A^T when ta).  One pass
+// over C; same tile variants as gemm_add.
+struct GemmLink {
+  const double *A, *B;
+  long long lda, ldb;
+  int K, ta;
+};
+struct GemmChain {
+  double *C;
+  long long ldc;
+  int M, N, link0, nlink;
+};
+int gemm_chain_add(Program &p, const std::vector<GemmChain> &chains,
+                   const std::vector<GemmLink> &links, const char *name, int variant = 0);
+
 // Times every variant of every tunable op (3 runs each after a warm-up)
 // and keeps the fastest.  Runs ops out of order: outputs are garbage
 // until the program is run again from the start.

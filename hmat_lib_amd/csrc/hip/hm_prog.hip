@@ -221,6 +221,90 @@ int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
   return SUCCESS;
 }
 
+template <class Tile>
+__global__ __launch_bounds__(256) void k_gemm_chain(const GemmChain *chains,
+                                                    const GemmLink *links, const int4 *tiles,
+                                                    int ntiles) {
+  __shared__ double lds[Tile::LDS_DOUBLES];
+  for (int b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const int4 t = tiles[b];
+    const GemmChain ch = chains[t.x];
+    const int i0 = t.y * Tile::TM, j0 = t.z * Tile::TN;
+    Tile tile;
+    tile.zero();
+    for (int q = ch.link0; q < ch.link0 + ch.nlink; q++) {
+      const GemmLink lk = links[q];
+      auto lb = [&](int k, int j) -> double {
+        return (k < lk.K && j0 + j < ch.N) ? lk.B[k + (long long)(j0 + j) * lk.ldb] : 0.0;
+      };
+      if (lk.ta) {
+        auto la = [&](int i, int k) -> double {
+          return (i0 + i < ch.M && k < lk.K) ? lk.A[k + (long long)(i0 + i) * lk.lda] : 0.0;
+        };
+        tile.template run<false, false>(lds, lk.K, la, lb);
+      } else {
+        auto la = [&](int i, int k) -> double {
+          return (i0 + i < ch.M && k < lk.K) ? lk.A[(i0 + i) + (long long)k * lk.lda] : 0.0;
+        };
+        tile.template run<true, false>(lds, lk.K, la, lb);
+      }
+    }
+    tile.store([&](int i, int j, double v) {
+      if (i0 + i < ch.M && j0 + j < ch.N) ch.C[(i0 + i) + (long long)(j0 + j) * ch.ldc] = v;
+    });
+    __syncthreads();
+  }
+}
+
+int gemm_chain_add(Program &p, const std::vector<GemmChain> &chains,
+                   const std::vector<GemmLink> &links, const char *name, int variant) {
+  std::vector<int> live;
+  for (size_t q = 0; q < chains.size(); q++) {
+    const GemmChain &c = chains[q];
+    if (c.M <= 0 || c.N <= 0) continue;
+    live.push_back((int)q);
+    p.touch(c.C, ((size_t)(c.N - 1) * c.ldc + c.M) * sizeof(double), name);
+    for (int i = c.link0; i < c.link0 + c.nlink; i++) {
+      const GemmLink &l = links[i];
+      if (l.K <= 0) continue;
+      const size_t ab = l.ta ? ((size_t)(c.M - 1) * l.lda + l.K) : ((size_t)(l.K - 1) * l.lda + c.M);
+      p.touch(l.A, ab * sizeof(double), name);
+      p.touch(l.B, ((size_t)(c.N - 1) * l.ldb + l.K) * sizeof(double), name);
+    }
+  }
+  if (live.empty()) return SUCCESS;
+  const size_t coff = p.push(chains), loff = p.push(links);
+  size_t toff[GEMM_VARIANTS];
+  int ntile[GEMM_VARIANTS];
+  for (int v = 0; v < GEMM_VARIANTS; v++) {
+    std::vector<int4> tiles;
+    for (int q : live) {
+      const GemmChain &c = chains[q];
+      for (int a = 0; a < (c.M + kTileTM[v] - 1) / kTileTM[v]; a++)
+        for (int b = 0; b < (c.N + kTileTN[v] - 1) / kTileTN[v]; b++)
+          tiles.push_back(make_int4(q, a, b, 0));
+    }
+    toff[v] = p.push(tiles);
+    ntile[v] = (int)tiles.size();
+  }
+  auto choice = std::make_shared<int>(variant < 0 ? 0 : std::min(variant, GEMM_VARIANTS - 1));
+  if (variant < 0) p.tunables.push_back({p.ops.size(), GEMM_VARIANTS, choice});
+  const size_t t0 = toff[0], t1 = toff[1], t2 = toff[2];
+  const int n0 = ntile[0], n1 = ntile[1], n2 = ntile[2];
+  p.add(name, [=](hipStream_t s, const unsigned char *b) {
+    const GemmChain *C = reinterpret_cast<const GemmChain *>(b + coff);
+    const GemmLink *L = reinterpret_cast<const GemmLink *>(b + loff);
+    const int v = *choice;
+    const int n = v == 0 ? n0 : (v == 1 ? n1 : n2);
+    const int4 *T = reinterpret_cast<const int4 *>(b + (v == 0 ? t0 : (v == 1 ? t1 : t2)));
+    const unsigned grid = (unsigned)std::min(n, GEMM_MAX_GRID);
+    if (v == 0) hipLaunchKernelGGL(k_gemm_chain<GTile0>, dim3(grid), dim3(256), 0, s, C, L, T, n);
+    if (v == 1) hipLaunchKernelGGL(k_gemm_chain<GTile1>, dim3(grid), dim3(256), 0, s, C, L, T, n);
+    if (v == 2) hipLaunchKernelGGL(k_gemm_chain<GTile2>, dim3(grid), dim3(256), 0, s, C, L, T, n);
+  });
+  return SUCCESS;
+}
+
 int program_tune(Program &p, hipStream_t s) {
   if (p.tunables.empty()) return SUCCESS;
   hipEvent_t e0, e1;
