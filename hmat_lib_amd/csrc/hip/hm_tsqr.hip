@@ -195,6 +195,42 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down(const QrItem *items) {
 static constexpr int TQ_CR = 128;
 static constexpr int TQ_FAST_COLS = 64;
 
+// Applies H = I - tau v v^T (v = [1; vk[k+1..rows)]) to the LDS columns
+// jf, jf + 4, ... < jend of M (column stride TQ_CR, rows <= TQ_CR <= 128):
+// each lane holds rows k + lane and k + lane + 64; four columns per pass
+// so that their wavefront reductions overlap.
+__device__ __forceinline__ void lds_reflect(double *M, const double *vk, int k, int rows,
+                                            double tau, int jf, int jend, int lane) {
+  const int i0 = k + lane, i1 = i0 + 64;
+  const bool h0 = i0 < rows, h1 = i1 < rows;
+  const double v0 = h0 ? (i0 == k ? 1.0 : vk[i0]) : 0.0;
+  const double v1 = h1 ? vk[i1] : 0.0;
+  for (int j = jf; j < jend; j += 16) {
+    double d[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int jj = j + 4 * t;
+      double x = 0.0;
+      if (jj < jend) {
+        const double *mj = M + jj * TQ_CR;
+        x = (h0 ? v0 * mj[i0] : 0.0) + (h1 ? v1 * mj[i1] : 0.0);
+      }
+      d[t] = x;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) d[t] = tau * wave_sum(d[t]);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int jj = j + 4 * t;
+      if (jj < jend) {
+        double *mj = M + jj * TQ_CR;
+        if (h0) mj[i0] -= d[t] * v0;
+        if (h1) mj[i1] -= d[t] * v1;
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
   __shared__ double Y[TQ_CR * TQ_FAST_COLS];
   __shared__ double tauS[TQ_FAST_COLS];
@@ -233,13 +269,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
     __syncthreads();
     const double tau = tauS[k];
     if (tau == 0.0) continue;
-    for (int j = k + 1 + ((wave - (k + 1)) & 3); j < nc; j += 4) {
-      double *yj = Y + j * TQ_CR;
-      double d = 0.0;
-      for (int i = k + lane; i < rows; i += 64) d += (i == k ? 1.0 : yk[i]) * yj[i];
-      d = tau * wave_sum(d);
-      for (int i = k + lane; i < rows; i += 64) yj[i] -= d * (i == k ? 1.0 : yk[i]);
-    }
+    lds_reflect(Y, yk, k, rows, tau, k + 1 + ((wave - (k + 1)) & 3), nc, lane);
   }
   __syncthreads();
   for (int k = tid; k < nc; k += TQ_NT) it.tau[k] = k < nref ? tauS[k] : 0.0;
@@ -289,14 +319,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
   for (int k = nref - 1; k >= 0; k--) {
     const double tau = tauS[k];
     if (tau == 0.0) continue;
-    const double *vk = V + k * TQ_CR;
-    for (int j = wave; j < nq; j += 4) {
-      double *qj = Qs + j * TQ_CR;
-      double d = 0.0;
-      for (int i = k + lane; i < rows; i += 64) d += (i == k ? 1.0 : vk[i]) * qj[i];
-      d = tau * wave_sum(d);
-      for (int i = k + lane; i < rows; i += 64) qj[i] -= d * (i == k ? 1.0 : vk[i]);
-    }
+    lds_reflect(Qs, V + k * TQ_CR, k, rows, tau, wave, nq, lane);
   }
   __syncthreads();
   double *Q = it.qout + it.r0;
