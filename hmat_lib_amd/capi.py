@@ -142,6 +142,8 @@ def _declare_hmat_api(lib, test_abi=False):
     lib.compute_multiply_hodlr_dense_workspace.restype = c_int
     lib.get_highest_s.argtypes = [PTree]
     lib.get_highest_s.restype = c_int
+    lib.multiply_hodlr_hodlr.argtypes = [PTree, PTree, PTree, c_double, PInt]
+    lib.multiply_hodlr_hodlr.restype = c_int
 
 
 def _declare_hip_api(lib):
@@ -173,6 +175,7 @@ def _declare_hip_api(lib):
                                        V, c_long, V], c_int),
         "hm_hip_recompress_batch": ([c_int, PInt, PInt, PInt, POINTER(V), POINTER(V),
                                      c_double, c_int, PInt, V], c_int),
+        "hm_hip_multiply_hodlr_hodlr": ([V, V, c_double, POINTER(V), V], c_int),
         "hm_hip_selftest_mfma": ([PDouble], c_int),
         "hm_hip_selftest_mfma_bf16": ([POINTER(C.c_float), POINTER(C.c_float),
                                        POINTER(C.c_float)], c_int),
@@ -200,6 +203,8 @@ def _declare_hip_api(lib):
         "hm_hip_tree_set_precision": ([V, c_int, V], c_int),
         "hm_hip_kernel_timing_dump": ([C.c_char_p, c_int], c_int),
         "hm_hip_audit_qr": ([c_int, POINTER(c_int), POINTER(c_int), c_int, C.c_char_p,
+                             c_int], c_int),
+        "hm_hip_audit_hh": ([c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int), C.c_char_p,
                              c_int], c_int),
         "hm_hip_matmat_variant": ([V, c_int, c_int], c_int),
         "hm_hip_kernel_timing": ([c_int], c_int),

@@ -44,54 +44,64 @@ struct BatchOperands {
   long ldy;
 };
 
+void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, const double *X,
+                     long long ldx, int R, double *Y, long long ldy, std::vector<GemmJob> &wj,
+                     std::vector<GemmChain> &chains, std::vector<GemmLink> &links) {
+  if (R <= 0) return;
+  const int h = H->height, span = 1 << (h - l);
+  const int r0 = H->leaf_off[q * span];
+  // W_b for the blocks of the subtree (levels l + 1 .. h)
+  std::vector<double *> W(H->nblock, nullptr);
+  for (int l2 = l + 1; l2 <= h; l2++) {
+    for (int j = q << (l2 - l); j < (q + 1) << (l2 - l); j++) {
+      const int g = blk_index(l2, j);
+      const BlockMeta &b = H->blk[g];
+      if (b.s <= 0) continue;
+      W[g] = ar.take<double>((size_t)b.s * R);
+      const double *F = trans ? H->d_U + b.uoff : H->d_V + b.voff;
+      const int K = trans ? b.m : b.n;
+      const int src = (trans ? b.r0 : b.c0) - r0;
+      // W_b (s x R, ld s) = F^T X[src : src + K, :]
+      wj.push_back({F, X + src, W[g], nullptr, std::max(K, 1), ldx, b.s, b.s, R, K, 1, 0, 0,
+                    1.0, 0.0});
+    }
+  }
+  // Y_c = op(D_c) X_c + sum over levels of the output factor's rows of
+  // leaf c times W_b, one chain per leaf
+  for (int c = q * span; c < (q + 1) * span; c++) {
+    const int lo = H->leaf_off[c], mc = H->leaf_off[c + 1] - lo;
+    if (mc <= 0) continue;
+    GemmChain ch;
+    ch.C = Y + (lo - r0);
+    ch.ldc = ldy;
+    ch.M = mc;
+    ch.N = R;
+    ch.link0 = (int)links.size();
+    links.push_back({H->d_diag + H->diag_off[c], X + (lo - r0), mc, ldx, mc, trans ? 1 : 0});
+    for (int l2 = l + 1; l2 <= h; l2++) {
+      const int q2 = c >> (h - l2);
+      // H X: block (l2, q2) has the leaf's rows (U); H^T X: block
+      // (l2, q2^1) has them as columns (V)
+      const int g = blk_index(l2, trans ? (q2 ^ 1) : q2);
+      const BlockMeta &b = H->blk[g];
+      if (b.s <= 0) continue;
+      const double *G = trans ? H->d_V + b.voff + (lo - b.c0) : H->d_U + b.uoff + (lo - b.r0);
+      const long long ld = trans ? b.n : b.m;
+      links.push_back({G, W[g], ld, b.s, b.s, 0});
+    }
+    ch.nlink = (int)links.size() - ch.link0;
+    chains.push_back(ch);
+  }
+}
+
 static int build_batch(Program &p, Arena &ar, const std::vector<hm_dtree *> &trees, int trans,
                        const std::vector<BatchOperands> &ops) {
   std::vector<GemmJob> wj;
   std::vector<GemmChain> chains;
   std::vector<GemmLink> links;
-  for (size_t i = 0; i < trees.size(); i++) {
-    const hm_dtree *t = trees[i];
-    const BatchOperands &o = ops[i];
-    if (o.R <= 0) continue;
-    std::vector<double *> W(t->nblock, nullptr);
-    for (int g = 0; g < t->nblock; g++) {
-      const BlockMeta &b = t->blk[g];
-      if (b.s <= 0) continue;
-      W[g] = ar.take<double>((size_t)b.s * o.R);
-      const double *F = trans ? t->d_U + b.uoff : t->d_V + b.voff;
-      const int K = trans ? b.m : b.n;
-      const int src = trans ? b.r0 : b.c0;
-      // W_b (s x R, ld s) = F^T X[src : src + K, :]
-      wj.push_back({F, o.X + src, W[g], nullptr, std::max(K, 1), o.ldx, b.s, b.s, o.R, K, 1, 0,
-                    0, 1.0, 0.0});
-    }
-    // Y_c = op(D_c) X_c + sum over levels of the output factor's rows of
-    // leaf c times W_b, one chain per leaf
-    for (int c = 0; c < t->nleaf; c++) {
-      const int lo = t->leaf_off[c], mc = t->leaf_off[c + 1] - lo;
-      if (mc <= 0) continue;
-      GemmChain ch;
-      ch.C = o.Y + lo;
-      ch.ldc = o.ldy;
-      ch.M = mc;
-      ch.N = o.R;
-      ch.link0 = (int)links.size();
-      links.push_back({t->d_diag + t->diag_off[c], o.X + lo, mc, o.ldx, mc, trans ? 1 : 0});
-      for (int l = 1; l <= t->height; l++) {
-        const int q = c >> (t->height - l);
-        // H X: block (l, q) has the leaf's rows (U); H^T X: block (l, q^1)
-        // has them as columns (V)
-        const int g = blk_index(l, trans ? (q ^ 1) : q);
-        const BlockMeta &b = t->blk[g];
-        if (b.s <= 0) continue;
-        const double *G = trans ? t->d_V + b.voff + (lo - b.c0) : t->d_U + b.uoff + (lo - b.r0);
-        const long long ld = trans ? b.n : b.m;
-        links.push_back({G, W[g], ld, b.s, b.s, 0});
-      }
-      ch.nlink = (int)links.size() - ch.link0;
-      chains.push_back(ch);
-    }
-  }
+  for (size_t i = 0; i < trees.size(); i++)
+    subtree_product(ar, trees[i], 0, 0, trans, ops[i].X, ops[i].ldx, ops[i].R, ops[i].Y,
+                    ops[i].ldy, wj, chains, links);
   HM_OK(gemm_add(p, ar, wj, true, false, "batch W", -1));
   HM_OK(gemm_chain_add(p, chains, links, "batch apply", -1));
   return SUCCESS;

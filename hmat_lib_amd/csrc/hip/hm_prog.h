@@ -153,4 +153,17 @@ struct ColNorm {
 };
 int colnorm_add(Program &p, const std::vector<ColNorm> &items);
 
+// ------------------------------------------------ tree products as items
+// Y = op(H_qq) X for the diagonal subtree of H at node (l, q) (l = 0: the
+// whole matrix; l = height: one diagonal leaf).  X and Y (R columns,
+// leading dims ldx / ldy) start at the node's first row.  Appends one
+// W_b = F_in^T X[rows of b's input node] job (for gemm_add with ta = 1,
+// W in `ar` scratch) per off-diagonal block of the subtree, and one chained
+// GEMM per leaf c: Y_c = op(D_c) X_c + sum over the levels below l of
+// F_out(rows of c) W_b, the diagonal leaf and the blocks concatenated
+// along K.  F_in / F_out are V / U for H X and U / V for H^T X.
+void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, const double *X,
+                     long long ldx, int R, double *Y, long long ldy, std::vector<GemmJob> &wj,
+                     std::vector<GemmChain> &chains, std::vector<GemmLink> &links);
+
 }  // namespace hm

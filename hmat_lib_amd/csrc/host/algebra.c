@@ -248,3 +248,38 @@ HM_EXPORT int recompress_large_s(struct NodeOffDiagonal *node, const int m_small
   (void)m_smaller;
   return recompress_host(node, svd_threshold, 1, ierr);
 }
+
+/* Reference: src/hodlr_algebra.c:1452-1553.  Both inputs come from the side
+ * table (uploaded on first use), the product is formed and recompressed on
+ * the device, then copied into malloc'd arrays of `out` (which must hold no
+ * data, as in the reference) and kept in the side table for later products.
+ * Trees of different heights or leaf sizes give INPUT_ERROR (the reference
+ * leaves that undefined).  Unlike the reference, which never writes them,
+ * the internal-node sizes of `out` are set. */
+HM_EXPORT int multiply_hodlr_hodlr(const struct TreeHODLR *hodlr1,
+                                   const struct TreeHODLR *hodlr2,
+                                   struct TreeHODLR *out,
+                                   const double svd_threshold, int *ierr) {
+  if (hodlr1 == NULL || hodlr2 == NULL || out == NULL) {
+    *ierr = INPUT_ERROR;
+    return 0;
+  }
+  *ierr = SUCCESS;
+  if (hodlr1->height != hodlr2->height || out->height != hodlr1->height) {
+    *ierr = INPUT_ERROR;
+    return 0;
+  }
+  int rc = SUCCESS;
+  hm_dtree *da = hm_side_get(hodlr1->root, hodlr1->height, &rc);
+  hm_dtree *db = da ? hm_side_get(hodlr2->root, hodlr2->height, &rc) : NULL;
+  hm_dtree *dc = NULL;
+  if (da && db) rc = hm_hip_multiply_hodlr_hodlr(da, db, svd_threshold, &dc, NULL);
+  if (rc == SUCCESS && dc) rc = hm_hip_tree_download(dc, out, malloc, NULL);
+  if (rc != SUCCESS) {
+    if (dc) hm_hip_tree_destroy(dc);
+    *ierr = hm_report(rc, "multiply_hodlr_hodlr");
+    return 0;
+  }
+  hm_side_put(out->root, out->height, dc);
+  return 0;
+}

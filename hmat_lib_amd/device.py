@@ -143,6 +143,13 @@ class DeviceTree:
         capi.check(capi.hip().hm_hip_matmat(self.h, trans, R, dX.ptr, xs[0], xs[1],
                                             dY.ptr, ys[0], ys[1], None), "matmat")
 
+    def multiply(self, other: "DeviceTree", thr: float) -> "DeviceTree":
+        """self @ other as a new recompressed device tree (H x H)."""
+        out = C.c_void_p()
+        capi.check(capi.hip().hm_hip_multiply_hodlr_hodlr(self.h, other.h, thr, C.byref(out),
+                                                          None), "multiply_hodlr_hodlr")
+        return DeviceTree(out)
+
     def set_precision(self, slices: int):
         """0: FP64 products; 1..4: BF16-sliced off-diagonal products."""
         capi.check(capi.hip().hm_hip_tree_set_precision(self.h, slices, None), "set_precision")

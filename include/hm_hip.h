@@ -183,6 +183,19 @@ HM_HIP_API int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
                             double *const *d_v, double thr, int method,
                             int *k_out, void *stream);
 
+/* ---- HODLR x HODLR ------------------------------------------------------
+ * *out = a b for two device trees of the same height and leaf sizes (ranks
+ * may differ), all on the device.  Every block of the product is formed
+ * from a and b (diagonal leaves: a_cc b_cc plus the low-rank terms of every
+ * level; off-diagonal blocks: the ancestor terms, a_qq U_b and U_a,
+ * b_pp^T V_a, in the reference's column order) and each off-diagonal block
+ * is recompressed as hm_hip_recompress_batch does with method -1
+ * (absolute rule: keep sigma_j > thr, at least 1).  Fails with INPUT_ERROR
+ * when a block's uncompressed rank exceeds the recompression's limit
+ * (1024).  Replaces src/hodlr_algebra.c:1452-1553 (multiply_hodlr_hodlr). */
+HM_HIP_API int hm_hip_multiply_hodlr_hodlr(const hm_dtree *a, const hm_dtree *b, double thr,
+                                           hm_dtree **out, void *stream);
+
 /* ---- learned rank predictor --------------------------------------------
  * MLP nin -> nhidden -> nhidden -> 1 (tanh, FP32), Glorot-initialised from
  * seed; trained and run by the library's own kernels.  Inputs are row-major
@@ -256,6 +269,14 @@ HM_HIP_API int hm_hip_selftest_mfma_bf16(const float *hA, const float *hB, float
  * receives a summary or the first violation.  Returns 0 when clean. */
 HM_HIP_API int hm_hip_audit_qr(int njobs, const int *m, const int *L, int with_r,
                                char *msg, int msglen);
+
+/* Host-only check of the HODLR x HODLR launch program: built for two trees
+ * of the given leaf sizes (2^height) and block ranks (level-major, block
+ * (l, j) at 2^l - 2 + j) on fake buffers, every range its work items read
+ * or write is checked against the operands and its scratch.  Returns 0
+ * when clean; msg receives a summary or the first violation. */
+HM_HIP_API int hm_hip_audit_hh(int height, const int *leaves, const int *ranks_a,
+                               const int *ranks_b, char *msg, int msglen);
 
 /* ---- kernel timing (HIP events around tracked kernels) ----------------- */
 HM_HIP_API int hm_hip_kernel_timing(int enable);

@@ -1,16 +1,17 @@
 /*
- * Recompression of a low-rank off-diagonal node.
+ * HODLR x HODLR product and recompression of a low-rank off-diagonal node.
  *
- * The reference keeps its two recompression routines static
- * (src/hodlr_algebra.c:97-211 `recompress`, QR based, used when
- * s < min(m, n); src/hodlr_algebra.c:280-343 `recompress_large_s`, dense
- * SVD of U*V^T, used otherwise) and reaches them only through
- * multiply_hodlr_hodlr (src/hodlr_algebra.c:835-845, 1116-1126).  This
- * header exports them directly so that callers, and the H x H product when
- * it is added, can recompress on the GPU.  Both replace u, v and s of the
- * node in place (u and v are freed and re-malloc'd) and use the
- * reference's absolute truncation rule: keep sigma_i while
- * sigma_i > svd_threshold, at least one column.
+ * multiply_hodlr_hodlr is the reference's public entry point
+ * (include/hmat_lib/hodlr_algebra.h:4-10 of hmat_lib).  The reference keeps
+ * its two recompression routines static (src/hodlr_algebra.c:97-211
+ * `recompress`, QR based, used when s < min(m, n); src/hodlr_algebra.c:
+ * 280-343 `recompress_large_s`, dense SVD of U*V^T, used otherwise) and
+ * reaches them only through multiply_hodlr_hodlr (src/hodlr_algebra.c:
+ * 835-845, 1116-1126).  This header exports them directly so that callers
+ * can recompress on the GPU.  Both replace u, v and s of the node in place
+ * (u and v are freed and re-malloc'd) and use the reference's absolute
+ * truncation rule: keep sigma_i while sigma_i > svd_threshold, at least one
+ * column.
  *
  * Return value: 0, or a negative LAPACK-style argument index on bad input.
  */
@@ -22,6 +23,17 @@ extern "C" {
 #endif
 
 struct NodeOffDiagonal;
+struct TreeHODLR;
+
+/* out = hodlr1 @ hodlr2 (same height and block sizes).  `out` must be an
+ * allocated tree of that height holding no data; its diagonal and
+ * off-diagonal arrays are malloc'd (free with free_tree_data /
+ * free_tree_hodlr).  Every off-diagonal block is recompressed with the
+ * absolute rule above.  Replaces src/hodlr_algebra.c:1452-1553. */
+int multiply_hodlr_hodlr(const struct TreeHODLR *hodlr1,
+                         const struct TreeHODLR *hodlr2,
+                         struct TreeHODLR *out, const double svd_threshold,
+                         int *ierr);
 
 /* QR method: U = Q_U R_U, V = Q_V R_V, SVD of R_U R_V^T (s x s).
  * Requires node->s < min(node->m, node->n).

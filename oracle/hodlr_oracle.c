@@ -30,6 +30,9 @@
  *   recompression        12-digit golden values of
  *                        test_hodlr_hodlr_algebra.c:1643-1669 and the zero
  *                        cases 1575-1630; reconstruction invariants
+ *   H x H product        the known answers of test_hodlr_hodlr_algebra.c:
+ *                        100-339, the dense product of random trees (exact
+ *                        algebra), bitwise agreement with oracle/_ref
  */
 #include <math.h>
 #include <stdint.h>
@@ -835,6 +838,222 @@ ORC_API int orc_recompress_large_s(struct NodeOffDiagonal *nd, int m_smaller,
   }
 done:
   free(sig); free(dd); free(uu); free(vt);
+  return info;
+}
+
+/* ------------------------------------------------------------------------
+ * HODLR x HODLR product, multiply_hodlr_hodlr (src/hodlr_algebra.c:1452-1553).
+ * Every block of C = A B is formed from A and B alone:
+ *   diagonal leaf c (compute_diagonal 1267-1352 with
+ *   add_off_diagonal_contribution 1189-1222): C_cc = A_cc B_cc (beta 0),
+ *   then for each level from the leaf's parent up to the root, with the A
+ *   block holding c's rows (children[1] when the child containing c is a
+ *   left child, else children[2]) and the B block holding c's columns (the
+ *   other one): W1 = V1^T U2, W2 = W1 V2[c]^T, C_cc += U1[c] W2;
+ *   off-diagonal pair of internal node P (compute_inner_off_diagonal
+ *   780-869, compute_other_off_diagonal 1057-1127): columns
+ *     for each ancestor of P from its parent up to the root (467-545):
+ *       U: U1[rows of the block], V: V2[columns of the block] (V1^T U2)^T
+ *     then (670-705, 930-957): U: A_qq U_B, V: V_B  and  U: U_A, V: B_pp^T V_A
+ *   (q = the block's row child, p = its column child; A_qq B_pp are the
+ *   diagonal leaves or subtrees, multiplied as multiply_internal_node_dense
+ *   and its transpose do), then recompress when s < min(m, n), else
+ *   recompress_large_s, with the absolute threshold (835-845, 1116-1126).
+ * `out` must be an allocated tree of the same height and hold no data.
+ * Unlike the reference, which leaves them 0, the internal-node sizes of
+ * `out` are set.  Returns the LAPACK info of the first failing
+ * recompression, else 0.
+ * ---------------------------------------------------------------------- */
+static void orc_copy_cols(int rows, int cols, const double *src, int lds,
+                          double *dst, int ldd) {
+  for (int j = 0; j < cols; j++)
+    memcpy(dst + (size_t)j * ldd, src + (size_t)j * lds, (size_t)rows * sizeof(double));
+}
+
+/* rows x rows diagonal piece: leaf (height 0) or internal subtree */
+static void orc_sub_product(const struct HODLRInternalNode *parent, int child,
+                            int sub_height, int trans, const double *x, int r,
+                            int ldx, double *y, int ldy) {
+  const double one = 1.0, zero = 0.0;
+  if (sub_height == 0) {
+    const struct NodeDiagonal *dg = &parent->children[child].leaf->data.diagonal;
+    int m = dg->m;
+    dgemm_(trans ? "T" : "N", "N", &m, &r, &m, &one, dg->data, &m, x, &ldx,
+           &zero, y, &ldy);
+  } else {
+    orc_multiply_internal_node_dense(parent->children[child].internal,
+                                     sub_height, trans, x, r, ldx, y, ldy);
+  }
+}
+
+/* Size of child 0 (left) of an internal node, from its top-right block
+ * (set in every tree, also in product trees whose internal sizes are 0). */
+static int orc_left_m(const struct HODLRInternalNode *p) {
+  return p->children[1].leaf->data.off_diagonal.m;
+}
+
+static int orc_hh_pair(int height, int depth, long pos,
+                       const struct HODLRInternalNode *pa,
+                       const struct HODLRInternalNode *pb,
+                       struct HODLRInternalNode *po, double thr, int *ierr) {
+  const double one = 1.0, zero = 0.0;
+  const int sub_h = height - depth - 1;
+  const int m0 = orc_left_m(pa), m3 = pa->children[1].leaf->data.off_diagonal.n;
+  struct NodeOffDiagonal *tr = &po->children[1].leaf->data.off_diagonal;
+  struct NodeOffDiagonal *bl = &po->children[2].leaf->data.off_diagonal;
+  const struct NodeOffDiagonal *atr = &pa->children[1].leaf->data.off_diagonal;
+  const struct NodeOffDiagonal *abl = &pa->children[2].leaf->data.off_diagonal;
+  const struct NodeOffDiagonal *btr = &pb->children[1].leaf->data.off_diagonal;
+  const struct NodeOffDiagonal *bbl = &pb->children[2].leaf->data.off_diagonal;
+  /* ranks of the A blocks on the way to the root */
+  int s_anc = 0;
+  {
+    const struct HODLRInternalNode *q = pa;
+    long ps = pos;
+    while (q->parent) {
+      s_anc += q->parent->children[(ps % 2 == 0) ? 1 : 2].leaf->data.off_diagonal.s;
+      q = q->parent;
+      ps /= 2;
+    }
+  }
+  tr->m = m0; tr->n = m3; tr->s = s_anc + atr->s + btr->s;
+  bl->m = m3; bl->n = m0; bl->s = s_anc + abl->s + bbl->s;
+  tr->u = malloc((size_t)tr->m * tr->s * sizeof(double) + 8);
+  tr->v = malloc((size_t)tr->n * tr->s * sizeof(double) + 8);
+  bl->u = malloc((size_t)bl->m * bl->s * sizeof(double) + 8);
+  bl->v = malloc((size_t)bl->n * bl->s * sizeof(double) + 8);
+  if (!tr->u || !tr->v || !bl->u || !bl->v) {
+    *ierr = ALLOCATION_FAILURE;
+    return 0;
+  }
+  int smax = 1;
+  {
+    const struct HODLRInternalNode *qa = pa, *qb = pb;
+    long ps = pos;
+    while (qa->parent) {
+      const int w = (ps % 2 == 0) ? 1 : 2;
+      const int s1 = qa->parent->children[w].leaf->data.off_diagonal.s;
+      const int s2 = qb->parent->children[3 - w].leaf->data.off_diagonal.s;
+      if (s1 * s2 > smax) smax = s1 * s2;
+      qa = qa->parent; qb = qb->parent; ps /= 2;
+    }
+  }
+  double *w12 = malloc((size_t)smax * sizeof(double));
+  if (!w12) { *ierr = ALLOCATION_FAILURE; return 0; }
+  /* ancestor terms; off = row offset of P within the ancestor's child */
+  size_t ctr_u = 0, ctr_v = 0, cbl_u = 0, cbl_v = 0;
+  {
+    const struct HODLRInternalNode *qa = pa, *qb = pb;
+    long ps = pos;
+    int off = 0;
+    while (qa->parent) {
+      const int w = (ps % 2 == 0) ? 1 : 2;
+      const struct NodeOffDiagonal *l1 = &qa->parent->children[w].leaf->data.off_diagonal;
+      const struct NodeOffDiagonal *l2 = &qb->parent->children[3 - w].leaf->data.off_diagonal;
+      int s1 = l1->s, s2 = l2->s, k1 = l1->n, ld2 = l1->m;
+      dgemm_("T", "N", &s1, &s2, &k1, &one, l1->v, &k1, l2->u, &k1, &zero, w12, &s1);
+      int nt = tr->n, nb = bl->n;
+      dgemm_("N", "T", &nt, &s1, &s2, &one, l2->v + off + m0, &ld2, w12, &s1, &zero,
+             tr->v + ctr_v, &nt);
+      orc_copy_cols(tr->m, s1, l1->u + off, ld2, tr->u + ctr_u, tr->m);
+      dgemm_("N", "T", &nb, &s1, &s2, &one, l2->v + off, &ld2, w12, &s1, &zero,
+             bl->v + cbl_v, &nb);
+      orc_copy_cols(bl->m, s1, l1->u + off + m0, ld2, bl->u + cbl_u, bl->m);
+      ctr_u += (size_t)tr->m * s1; ctr_v += (size_t)tr->n * s1;
+      cbl_u += (size_t)bl->m * s1; cbl_v += (size_t)bl->n * s1;
+      if (ps % 2 == 1) off += orc_left_m(qa->parent);
+      qa = qa->parent; qb = qb->parent; ps /= 2;
+    }
+  }
+  free(w12);
+  /* A_00 U_B(tr) | V_B(tr), then U_A(tr) | B_33^T V_A(tr) */
+  orc_sub_product(pa, 0, sub_h, 0, btr->u, btr->s, btr->m, tr->u + ctr_u, tr->m);
+  memcpy(tr->v + ctr_v, btr->v, (size_t)btr->n * btr->s * sizeof(double));
+  ctr_u += (size_t)tr->m * btr->s; ctr_v += (size_t)tr->n * btr->s;
+  memcpy(tr->u + ctr_u, atr->u, (size_t)atr->m * atr->s * sizeof(double));
+  orc_sub_product(pb, 3, sub_h, 1, atr->v, atr->s, atr->n, tr->v + ctr_v, tr->n);
+  /* A_33 U_B(bl) | V_B(bl), then U_A(bl) | B_00^T V_A(bl) */
+  orc_sub_product(pa, 3, sub_h, 0, bbl->u, bbl->s, bbl->m, bl->u + cbl_u, bl->m);
+  memcpy(bl->v + cbl_v, bbl->v, (size_t)bbl->n * bbl->s * sizeof(double));
+  cbl_u += (size_t)bl->m * bbl->s; cbl_v += (size_t)bl->n * bbl->s;
+  memcpy(bl->u + cbl_u, abl->u, (size_t)abl->m * abl->s * sizeof(double));
+  orc_sub_product(pb, 0, sub_h, 1, abl->v, abl->s, abl->n, bl->v + cbl_v, bl->n);
+  const int mn = m0 < m3 ? m0 : m3;
+  int info = tr->s < mn ? orc_recompress(tr, mn, thr, ierr)
+                        : orc_recompress_large_s(tr, mn, thr, ierr);
+  if (*ierr != SUCCESS || info != 0) return info;
+  return bl->s < mn ? orc_recompress(bl, mn, thr, ierr)
+                    : orc_recompress_large_s(bl, mn, thr, ierr);
+}
+
+ORC_API int orc_multiply_hodlr_hodlr(const struct TreeHODLR *a,
+                                     const struct TreeHODLR *b,
+                                     struct TreeHODLR *out, double thr,
+                                     int *ierr) {
+  if (!a || !b || !out) {
+    *ierr = INPUT_ERROR;
+    return 0;
+  }
+  *ierr = SUCCESS;
+  const int h = a->height;
+  const double one = 1.0, zero = 0.0;
+  struct HODLRInternalNode **la = orc_levels(a->root, h);
+  struct HODLRInternalNode **lb = orc_levels(b->root, h);
+  struct HODLRInternalNode **lo = orc_levels(out->root, h);
+  const long total = (1L << h) - 1;
+  for (long k = 0; k < total; k++)
+    lo[k]->m = orc_left_m(la[k]) + la[k]->children[1].leaf->data.off_diagonal.n;
+  /* diagonal leaves */
+  int smax = orc_max_rank(a->root, h), smax2 = orc_max_rank(b->root, h), mmax = 0;
+  for (long i = 0; i < (1L << h); i++)
+    if (a->innermost_leaves[i]->data.diagonal.m > mmax)
+      mmax = a->innermost_leaves[i]->data.diagonal.m;
+  double *w1 = malloc((size_t)(smax > 0 ? smax : 1) * (smax2 > 0 ? smax2 : 1) * sizeof(double));
+  double *w2 = malloc((size_t)(smax > 0 ? smax : 1) * (mmax > 0 ? mmax : 1) * sizeof(double));
+  int info = 0;
+  if (!w1 || !w2) {
+    *ierr = ALLOCATION_FAILURE;
+    goto done;
+  }
+  for (long c = 0; c < (1L << h); c++) {
+    const struct HODLRLeafNode *da = a->innermost_leaves[c];
+    const struct HODLRLeafNode *db = b->innermost_leaves[c];
+    struct NodeDiagonal *dc = &out->innermost_leaves[c]->data.diagonal;
+    int m = da->data.diagonal.m;
+    dc->m = m;
+    dc->data = malloc((size_t)m * m * sizeof(double) + 8);
+    if (!dc->data) {
+      *ierr = ALLOCATION_FAILURE;
+      goto done;
+    }
+    dgemm_("N", "N", &m, &m, &m, &one, da->data.diagonal.data, &m,
+           db->data.diagonal.data, &m, &zero, dc->data, &m);
+    const struct HODLRInternalNode *qa = da->parent, *qb = db->parent;
+    long ps = c;
+    int off = 0;
+    while (qa) {
+      const int w = (ps % 2 == 0) ? 1 : 2;
+      const struct NodeOffDiagonal *l1 = &qa->children[w].leaf->data.off_diagonal;
+      const struct NodeOffDiagonal *l2 = &qb->children[3 - w].leaf->data.off_diagonal;
+      int s1 = l1->s, s2 = l2->s, k1 = l1->n, ldu = l1->m, ldv = l2->n;
+      dgemm_("T", "N", &s1, &s2, &k1, &one, l1->v, &k1, l2->u, &k1, &zero, w1, &s1);
+      dgemm_("N", "T", &s1, &m, &s2, &one, w1, &s1, l2->v + off, &ldv, &zero, w2, &s1);
+      dgemm_("N", "N", &m, &m, &s1, &one, l1->u + off, &ldu, w2, &s1, &one, dc->data, &m);
+      if (ps % 2 == 1) off += orc_left_m(qa);
+      qa = qa->parent; qb = qb->parent; ps /= 2;
+    }
+  }
+  /* off-diagonal pairs, innermost internal nodes first (as the reference) */
+  for (int d = h - 1; d >= 0 && *ierr == SUCCESS && info == 0; d--) {
+    for (long i = 0; i < (1L << d); i++) {
+      info = orc_hh_pair(h, d, i, ORC_LEVEL(la, d)[i], ORC_LEVEL(lb, d)[i],
+                         ORC_LEVEL(lo, d)[i], thr, ierr);
+      if (*ierr != SUCCESS || info != 0) break;
+    }
+  }
+done:
+  free(w1); free(w2);
+  free(la); free(lb); free(lo);
   return info;
 }
 

@@ -69,6 +69,8 @@ def oracle():
         lib.orc_recompress.restype = c_int
         lib.orc_recompress_large_s.argtypes = rec
         lib.orc_recompress_large_s.restype = c_int
+        lib.orc_multiply_hodlr_hodlr.argtypes = [PTree, PTree, PTree, c_double, PInt]
+        lib.orc_multiply_hodlr_hodlr.restype = c_int
         lib.orc_fill_kernel_block.argtypes = [c_int, c_int, PDouble, c_double, c_double,
                                               C.c_long, C.c_long, C.c_long, C.c_long,
                                               PDouble, C.c_long]

@@ -32,6 +32,46 @@ def laplacian_corners(m, symmetric=True):
     return a
 
 
+def laplacian_converse(m):
+    """-1 on the diagonal, 2 next to it (common_data.c:44-57)."""
+    a = -1.0 * np.eye(m)
+    i = np.arange(m - 1)
+    a[i, i + 1] = 2.0
+    a[i + 1, i] = 2.0
+    return np.asfortranarray(a)
+
+
+def banded(m, diag, first, second, corner):
+    """Pentadiagonal Toeplitz with `corner` at both diagonal ends (the
+    expected products of test_hodlr_hodlr_algebra.c:57-97)."""
+    a = diag * np.eye(m)
+    for k, v in ((1, first), (2, second)):
+        i = np.arange(m - k)
+        a[i, i + k] = v
+        a[i + k, i] = v
+    a[0, 0] = a[m - 1, m - 1] = corner
+    return np.asfortranarray(a)
+
+
+def hh_known_answers(m=21):
+    """The H x H cases of test_hodlr_hodlr_algebra.c:100-295 as
+    (name, A, B, expected, same_tree): Laplacian x converse (plain, 0.5 in
+    both corners, 0.5 in the bottom-left corner), identity x {I, 0, L}, and
+    each of I, 0, L times itself."""
+    L, LC, I, Z = laplacian(m), laplacian_converse(m), np.eye(m), np.zeros((m, m))
+    lxc = banded(m, -6.0, 5.0, -2.0, -4.0)      # fill_laplacian_x_converse
+    lxl = banded(m, 6.0, -4.0, 1.0, 5.0)        # fill_laplacian_x_laplacian
+    ls_exp = lxc.copy()
+    ls_exp[m - 1, 0], ls_exp[m - 1, 1], ls_exp[0, m - 2], ls_exp[0, m - 1] = -0.5, 1.0, 1.0, -0.5
+    la_exp = lxc.copy()
+    la_exp[m - 1, 0], la_exp[m - 1, 1] = -0.5, 1.0
+    return [("LxLC", L, LC, lxc, False),
+            ("L0.5SxLC", laplacian_corners(m, True), LC, ls_exp, False),
+            ("L0.5AxLC", laplacian_corners(m, False), LC, la_exp, False),
+            ("IxI", I, I, I, False), ("Ix0", I, Z, Z, False), ("IxL", I, L, L, False),
+            ("I^2", I, I, I, True), ("0^2", Z, Z, Z, True), ("L^2", L, L, lxl, True)]
+
+
 def glibc_decay_matrices(m, scale, sorted_first, count_sorted, count_unsorted):
     """The decay matrices of test_constructors.c:1047-1105: srand(42), then
     `count_sorted` sorted and `count_unsorted` unsorted draws of
@@ -66,6 +106,16 @@ def oracle_compress(a, height, thr, ms=None):
 
 def oracle_free(t):
     O.oracle().orc_free_tree(t)
+
+
+def oracle_hh(ta, tb, thr):
+    """Oracle product tree ta @ tb (a new oracle-allocated tree)."""
+    lib = O.oracle()
+    ierr = C.c_int(0)
+    out = lib.orc_allocate_tree(ta.contents.height, C.byref(ierr))
+    info = lib.orc_multiply_hodlr_hodlr(ta, tb, out, thr, C.byref(ierr))
+    assert ierr.value == 0 and info == 0, (ierr.value, info)
+    return out
 
 
 def oracle_matmat(t, x, mode="hd"):

@@ -243,3 +243,49 @@ def test_qr_program_stays_in_bounds(shapes, with_r):
     msg = C.create_string_buffer(256)
     rc = capi.hip().hm_hip_audit_qr(len(shapes), m, L, with_r, msg, 256)
     assert rc == 0, msg.value.decode()
+
+
+HH_CASES = [
+    (1, [10, 11], None),
+    (3, [7, 12, 5, 9, 13, 8, 6, 10], None),
+    (4, [6, 0, 9, 1, 14, 3, 8, 8, 2, 11, 5, 7, 13, 4, 9, 1], None),  # empty and 1-row leaves
+    (5, [64] * 32, "wide"),  # ranks up to the leaf size: dense recompression
+]
+
+
+@pytest.mark.parametrize("height,leaves,kind", HH_CASES, ids=["h1", "h3", "ragged", "wide"])
+def test_hh_program_stays_in_bounds(height, leaves, kind):
+    """Every work item of the HODLR x HODLR launch program (T cores,
+    subtree W pieces, leaf and ancestor GEMMs, chained GEMMs, copies) reads
+    and writes only inside the two trees, the product's diagonal, the
+    pre-recompression factor pools and its scratch; checked on the host."""
+    import ctypes as C
+    import numpy as np
+    from hmat_lib_amd import capi
+    rng = np.random.default_rng(height)
+    nb = (1 << (height + 1)) - 2
+    hi = 64 if kind == "wide" else 6
+    lv = (C.c_int * len(leaves))(*leaves)
+    ra = (C.c_int * nb)(*[int(x) for x in rng.integers(1, hi, size=nb)])
+    rb = (C.c_int * nb)(*[int(x) for x in rng.integers(1, hi, size=nb)])
+    msg = C.create_string_buffer(256)
+    rc = capi.hip().hm_hip_audit_hh(height, lv, ra, rb, msg, 256)
+    assert rc == 0, msg.value.decode()
+
+
+def test_hh_dropin_argument_errors():
+    """multiply_hodlr_hodlr: NULL operands and trees of different heights
+    give INPUT_ERROR before any device work."""
+    import ctypes as C
+    from hmat_lib_amd import capi
+    lib = capi.hmat()
+    ierr = C.c_int(0)
+    t2 = lib.allocate_tree(2, C.byref(ierr))
+    t3 = lib.allocate_tree(3, C.byref(ierr))
+    assert lib.multiply_hodlr_hodlr(t2, t3, t2, 1e-8, C.byref(ierr)) == 0
+    assert ierr.value == capi.INPUT_ERROR
+    ierr.value = 0
+    lib.multiply_hodlr_hodlr(None, t2, t2, 1e-8, C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+    lib.free_tree_hodlr(C.byref(t2))
+    lib.free_tree_hodlr(C.byref(t3))

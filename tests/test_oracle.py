@@ -342,3 +342,80 @@ def test_rank_features_library_matches_oracle():
         out.ctypes.data_as(C.POINTER(C.c_float)))
     assert rc == 0
     np.testing.assert_allclose(out, M.rank_features(extra, sig, 1e-8), rtol=1e-6, atol=1e-7)
+
+
+# ------------------------------------------------------- H x H oracle
+@pytest.mark.parametrize("height", [1, 2, 3])
+def test_hh_oracle_known_answers(height):
+    """The reference's H x H known answers (test_hodlr_hodlr_algebra.c:
+    100-339, DELTA 1e-10 on the reconstruction) hold on the oracle; the
+    non-symmetric corner cases catch a transposed factor or a block taken
+    from the wrong side."""
+    for name, a, b, exp, same in H.hh_known_answers():
+        np.testing.assert_allclose(a @ b, exp, atol=0, err_msg=name)  # fixture transcription
+        ta = H.oracle_compress(a, height, 1e-8)
+        tb = ta if same else H.oracle_compress(b, height, 1e-8)
+        tc = H.oracle_hh(ta, tb, 1e-8)
+        np.testing.assert_allclose(T.flatten(tc).to_dense(), exp, atol=1e-10, err_msg=name)
+        assert tc.contents.root.contents.m == a.shape[0]
+        for t in {id(x): x for x in (ta, tb, tc)}.values():
+            H.oracle_free(t)
+
+
+def test_hh_oracle_exact_algebra_random_trees():
+    """Random trees (uneven leaves, ranks 1-7, some blocks with s_tot >=
+    min(m, n) so both recompression methods run): with a threshold far
+    below the singular values kept, the product tree equals the dense
+    product of the two dense matrices (a dropped term, a wrong offset or a
+    wrong ancestor fails this)."""
+    rng = np.random.default_rng(11)
+    lib = capi.hmat()
+    for height, leaves in ((2, [9, 14, 6, 11]), (3, [7, 12, 5, 9, 13, 8, 6, 10])):
+        keep = T.Keep()
+        fa = H.random_flat_tree(rng, leaves, lambda l, j: int(rng.integers(1, 5)))
+        fb = H.random_flat_tree(rng, leaves, lambda l, j: int(rng.integers(1, 8)))
+        ta, tb = T.fill_tree(lib, fa, keep), T.fill_tree(lib, fb, keep)
+        tc = H.oracle_hh(ta, tb, 1e-12)
+        exp = fa.to_dense() @ fb.to_dense()
+        assert H.rel_err(T.flatten(tc).to_dense(), exp) < 1e-13
+        H.oracle_free(tc)
+        T.detach_and_free(lib, ta)
+        T.detach_and_free(lib, tb)
+
+
+@needs_ref
+def test_hh_oracle_matches_reference_bitwise():
+    """multiply_hodlr_hodlr of the reference built from its sources and the
+    restatement agree bit for bit (same LAPACK, same call sequence), on the
+    glibc decay matrices (m = 155, heights 1-5, halves and custom leaves)
+    and on a tree multiplied by itself."""
+    from tests.test_gpu_compress import MS155
+    mats = H.glibc_decay_matrices(155, 1.0, True, 6, 6)
+    rl = O.ref()
+    k = 0
+    for height in range(1, 6):
+        for custom in (False, True):
+            a, b = mats[k % 12][1], mats[(k + 5) % 12][1]
+            k += 1
+            ms = MS155[height] if custom else None
+            to_a, to_b = H.oracle_compress(a, height, 1e-8, ms), H.oracle_compress(b, height, 1e-8, ms)
+            tr_a, tr_b = _ref_compress(a, height, 1e-8, ms), _ref_compress(b, height, 1e-8, ms)
+            for same in (False, True):
+                ierr = C.c_int(0)
+                tr_c = rl.allocate_tree_monolithic(height, C.byref(ierr))
+                info = rl.multiply_hodlr_hodlr(tr_a, tr_a if same else tr_b, tr_c, 1e-8,
+                                               C.byref(ierr))
+                assert ierr.value == 0 and info == 0
+                to_c = H.oracle_hh(to_a, to_a if same else to_b, 1e-8)
+                fo, fr = T.flatten(to_c), T.flatten(tr_c)
+                for i in range(len(fo.diag)):
+                    np.testing.assert_array_equal(fo.diag[i], fr.diag[i])
+                for key in fr.blocks:
+                    np.testing.assert_array_equal(fo.blocks[key][0], fr.blocks[key][0])
+                    np.testing.assert_array_equal(fo.blocks[key][1], fr.blocks[key][1])
+                H.oracle_free(to_c)
+                rl.free_tree_hodlr(C.byref(tr_c))
+            for t in (to_a, to_b):
+                H.oracle_free(t)
+            for t in (tr_a, tr_b):
+                rl.free_tree_hodlr(C.byref(t))
