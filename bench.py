@@ -13,12 +13,12 @@ region.  cpu_baseline: the reference hmat_lib's multiply_hodlr_dense
 (built from its sources, MKL LAPACK/BLAS) on the same tree on the host
 cores, over a bounded number of calls.
 
-"additions" reports the three extensions on their own workloads (each
-measured here, none part of `value`): the batched product and batched
-compression of 512 uneven trees vs per-tree calls, the BF16-sliced
-off-diagonal products (1..4 slices) vs FP64 on the 2-D inverse-distance
-kernel, and the learned second sketch width vs the plain doubling on
-Matern-3/2 matrices.
+"additions" reports the extensions on their own workloads (each measured
+here, none part of `value`): the batched product and batched compression
+of 512 uneven trees vs per-tree calls, the BF16-sliced off-diagonal
+products (1..4 slices) vs FP64 on the 2-D inverse-distance kernel, the
+learned second sketch width vs the plain doubling on Matern-3/2 matrices,
+and the HODLR x HODLR product vs the reference's on the same trees.
 """
 import argparse
 import ctypes as C
@@ -262,6 +262,105 @@ def learned_leg(D, WL, np):
             "cases": cases}
 
 
+def hh_leg(D, WL, np, dt_main, w, cores, cpu_seconds):
+    """HODLR x HODLR products (multiply_hodlr_hodlr) on the device vs the
+    reference's multiply_hodlr_hodlr (built from its sources, MKL, host
+    cores) on the same two trees: the headline exp1d_64k tree times itself,
+    and a Matern-3/2 pair (N=16384, height 7, ell 0.1 and 0.2, compressed
+    at thr 1e-8).  The product's threshold is absolute (the reference's
+    rule); it is set to 1e-9 |A B X| / |X| (X random), since blocks of A B
+    reach sigma_0 ~ 1e5-1e7 and a fixed 1e-8 would cut inside the FP64
+    rounding floor.  Accuracy: (A B) X vs A (B X) through the device
+    products."""
+    from hmat_lib_amd import capi
+    cases = []
+
+    def one(name, ta, tb, n, h):
+        rng = np.random.default_rng(3)
+        x = np.asfortranarray(rng.standard_normal((n, 8)))
+        dx, d1, d2, d3 = (D.DeviceArray.from_numpy(x), D.DeviceArray((n, 8)),
+                          D.DeviceArray((n, 8)), D.DeviceArray((n, 8)))
+        tb.matmat(dx, d2)
+        ta.matmat(d2, d3)
+        y3 = d3.numpy()
+        thr = 1e-9 * float(np.linalg.norm(y3) / np.linalg.norm(x))
+        tc = ta.multiply(tb, thr)  # warm-up
+        tc.free()
+        D.synchronize()
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            tc = ta.multiply(tb, thr)
+            D.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+            if len(times) < 3:
+                tc.free()
+        tc.matmat(dx, d1)
+        y1 = d1.numpy()
+        err = float(np.linalg.norm(y1 - y3) / np.linalg.norm(y3))
+        rc = tc.ranks()
+        res = {"workload": name, "svd_threshold": thr, "device_ms": min(times),
+               "device_ms_runs": times,
+               "max_rank_in": int(max(ta.ranks().max(), tb.ranks().max())),
+               "max_rank_out": int(rc.max()), "rel_err_AB_X_vs_A_BX": err}
+        try:
+            from oracle import oracle as O
+            if O.have_ref():
+                lib, kind = O.ref(), "reference"
+                ierr = C.c_int(0)
+                ha = lib.allocate_tree_monolithic(h, C.byref(ierr))
+                ta.download(None, ha)
+                hb = ha
+                if tb is not ta:
+                    hb = lib.allocate_tree_monolithic(h, C.byref(ierr))
+                    tb.download(None, hb)
+                calls, t_cpu, hc = 0, 0.0, None
+                while calls < 1 or (t_cpu < cpu_seconds and calls < 20):
+                    if hc is not None:
+                        lib.free_tree_hodlr(C.byref(hc))
+                    hc = lib.allocate_tree_monolithic(h, C.byref(ierr))
+                    t0 = time.perf_counter()
+                    lib.multiply_hodlr_hodlr(ha, hb, hc, thr, C.byref(ierr))
+                    t_cpu += time.perf_counter() - t0
+                    calls += 1
+                from hmat_lib_amd import tree as T
+                fr = T.flatten(hc)
+                keys = sorted(fr.blocks, key=lambda k: (1 << k[0]) - 2 + k[1])
+                rr = np.array([fr.blocks[k][0].shape[1] for k in keys])
+                res["cpu_baseline"] = {"ms": t_cpu / calls * 1e3, "cores": cores, "kind": kind,
+                                       "sample": f"{calls} calls of the reference "
+                                                 "multiply_hodlr_hodlr on the same trees, "
+                                                 f"MKL, {cores} threads"}
+                res["speedup_vs_cpu"] = res["cpu_baseline"]["ms"] / res["device_ms"]
+                res["rank_mismatches_vs_reference"] = int((rr != rc).sum())
+                res["max_rank_diff_vs_reference"] = int(np.abs(rr - rc).max())
+                lib.free_tree_hodlr(C.byref(hc))
+                if hb is not ha:
+                    lib.free_tree_hodlr(C.byref(hb))
+                lib.free_tree_hodlr(C.byref(ha))
+        except Exception as e:  # the baseline is reported, never required
+            log(f"hh cpu baseline skipped: {e}")
+        tc.free()
+        log(f"hh {name}: device {res['device_ms']:.1f} ms, err {err:.1e}, "
+            f"cpu {res.get('cpu_baseline', {}).get('ms', float('nan')):.0f} ms")
+        cases.append(res)
+
+    one(f"{w.name} tree times itself (N={w.n}, height {w.height})",
+        dt_main, dt_main, w.n, w.height)
+    n, h = 16384, 7
+    pts = WL.points_2d_morton(4321, n)
+    trees = []
+    for ell in (0.1, 0.2):
+        da = D.fill_kernel_matrix(2, pts, ell, 0.0)
+        trees.append(D.DeviceTree.compress(da, n, h, 1e-8))
+        da.free()
+    one("Matern-3/2 N=16384 Morton U[0,1]^2, ell 0.1 times ell 0.2, height 7",
+        trees[0], trees[1], n, h)
+    for t in trees:
+        t.free()
+    return {"cases": cases}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,7 +371,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, mixed-precision and learned-sketch legs")
-    ap.add_argument("--legs", default="batched,precision,learned_sketch",
+    ap.add_argument("--legs", default="batched,precision,learned_sketch,hh",
                     help="comma-separated subset of the extra legs to run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 10 device-to-device copies of this many MiB "
@@ -435,8 +534,14 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {e}")
 
-    dt.free()
     extras = {}
+    if not args.no_extras and "hh" in args.legs.split(","):
+        try:
+            extras["hh"] = hh_leg(D, WL, np, dt, w, cores, args.cpu_seconds)
+        except Exception as e:  # a failed leg is reported, the line still prints
+            log(f"hh leg failed: {e}")
+            extras["hh"] = {"error": str(e)}
+    dt.free()
     if not args.no_extras:
         for name, leg in (("batched", batch_leg), ("precision", precision_leg),
                           ("learned_sketch", learned_leg)):

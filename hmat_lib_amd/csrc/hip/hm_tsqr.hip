@@ -495,7 +495,7 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
   }
   // per-job temporaries: C1 (when R is not wanted) and C2 (L x bw), R1, R2
   struct Tmp {
-    double *c1, *c2, *r1, *r2;
+    double *c1, *c2, *c3, *r1, *r2;
   };
   std::vector<Tmp> tmp(jobs.size());
   std::vector<Copy2D> zero;
@@ -504,6 +504,7 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
     if (J.L <= 0) continue;
     tmp[q].c1 = J.R ? nullptr : ar.take<double>((size_t)J.L * bw);
     tmp[q].c2 = ar.take<double>((size_t)J.L * bw);
+    tmp[q].c3 = ar.take<double>((size_t)J.L * bw);
     tmp[q].r1 = ar.take<double>((size_t)bw * bw);
     tmp[q].r2 = ar.take<double>((size_t)bw * bw);
     if (J.R) zero.push_back({nullptr, J.R, 0, J.ldr, J.L, J.L, 0, 0});
@@ -511,7 +512,7 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
   HM_OK(copy2d_add(prog, zero));
   for (int p = 0; p < maxp; p++) {
     const int c0 = p * bw;
-    std::vector<GemmJob> g1, g2, g3, g4, g5, g6;
+    std::vector<GemmJob> g1, g2, g3, g4, g3b, g4b, g5, g5b, g6;
     std::vector<TsqrPanel> t1, t2;
     std::vector<Copy2D> cp;
     for (size_t q = 0; q < jobs.size(); q++) {
@@ -545,9 +546,16 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
       }
       t1.push_back(P);
       if (p > 0) {
-        // reorthogonalise: C2 = Q_prev^T Q_p ; Q_p -= Q_prev C2 ; TSQR again
+        // reorthogonalise: C2 = Q_prev^T Q_p ; Q_p -= Q_prev C2, twice
+        // (C3 the second time), then TSQR again.  When the panel lies in the
+        // span of the previous ones to rounding (a numerically rank-deficient
+        // Y), the first projection leaves |Q_p| tiny and a single second
+        // projection is not enough: the TSQR would magnify what is left in
+        // that span and Q would lose orthogonality.
         g3.push_back({J.Q, Qp, tmp[q].c2, nullptr, J.ldq, J.ldq, J.L, c0, w, J.m, 1, 0, 0, 1.0, 0.0});
         g4.push_back({J.Q, tmp[q].c2, Qp, nullptr, J.ldq, J.L, J.ldq, J.m, w, c0, 1, 0, 0, -1.0, 1.0});
+        g3b.push_back({J.Q, Qp, tmp[q].c3, nullptr, J.ldq, J.ldq, J.L, c0, w, J.m, 1, 0, 0, 1.0, 0.0});
+        g4b.push_back({J.Q, tmp[q].c3, Qp, nullptr, J.ldq, J.L, J.ldq, J.m, w, c0, 1, 0, 0, -1.0, 1.0});
         TsqrPanel P2;
         memset(&P2, 0, sizeof(P2));
         P2.A = Qp;
@@ -562,8 +570,10 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
         t2.push_back(P2);
         cp.push_back({Yp, Qp, J.ldy, J.ldq, J.m, w, 0, 0});
         if (J.R) {
-          // R[:c0, p] = C1 + C2 R1 ; R[p, p] = R2 R1
+          // R[:c0, p] = C1 + (C2 + C3) R1 ; R[p, p] = R2 R1
           g5.push_back({tmp[q].c2, tmp[q].r1, J.R + (long long)c0 * J.ldr, nullptr, J.L, bw,
+                        J.ldr, c0, w, w, 1, 0, 0, 1.0, 1.0});
+          g5b.push_back({tmp[q].c3, tmp[q].r1, J.R + (long long)c0 * J.ldr, nullptr, J.L, bw,
                         J.ldr, c0, w, w, 1, 0, 0, 1.0, 1.0});
           g6.push_back({tmp[q].r2, tmp[q].r1, J.R + c0 + (long long)c0 * J.ldr, nullptr, bw, bw,
                         J.ldr, w, w, w, 1, 0, 0, 1.0, 0.0});
@@ -575,9 +585,12 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
     HM_OK(tsqr_add(prog, ar, t1));
     HM_OK(gemm_add(prog, ar, g3, true, false, "qr proj2"));
     HM_OK(gemm_add(prog, ar, g4, false, false, "qr upd2"));
+    HM_OK(gemm_add(prog, ar, g3b, true, false, "qr proj3"));
+    HM_OK(gemm_add(prog, ar, g4b, false, false, "qr upd3"));
     HM_OK(tsqr_add(prog, ar, t2));
     HM_OK(copy2d_add(prog, cp));
     HM_OK(gemm_add(prog, ar, g5, false, false, "qr r12"));
+    HM_OK(gemm_add(prog, ar, g5b, false, false, "qr r12"));  // same output: own launch
     HM_OK(gemm_add(prog, ar, g6, false, false, "qr r22"));
   }
   return SUCCESS;

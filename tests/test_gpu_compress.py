@@ -294,3 +294,28 @@ def test_wide_core_known_spectrum(gpu):
     lib.free_tree_hodlr(C.byref(t))
     dt.free()
     da.free()
+
+
+def test_full_width_sketch_rank_deficient(gpu):
+    """Blocks (512 x 512) of exact rank 262 whose singular values all stay
+    above the threshold (1 .. 1.26e-8): a 256-column sketch fails the
+    residual test, the doubling reaches the full block width (no test
+    columns left), and that sketch is numerically rank-deficient (rank 262
+    of 512), which the panel QR must still orthonormalise."""
+    from hmat_lib_amd import device as D
+    _say("case: full-width rank-deficient sketch n=1024 h=1")
+    rng = np.random.default_rng(44)
+    n, r = 1024, 262
+    q1, _ = np.linalg.qr(rng.standard_normal((512, r)))
+    q2, _ = np.linalg.qr(rng.standard_normal((512, r)))
+    blk = (q1 * np.logspace(0, -7.9, r)) @ q2.T
+    a = np.asfortranarray(np.block([[np.eye(512), blk], [blk.T, np.eye(512)]]))
+    da = D.DeviceArray.from_numpy(a)
+    t = D.DeviceTree.compress(da, n, 1, 1e-8)
+    assert t.stats.max_sketch == 512
+    x = np.asfortranarray(rng.standard_normal((n, 4)))
+    dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((n, 4))
+    t.matmat(dx, dy)
+    assert H.rel_err(dy.numpy(), a @ x) < 1e-8
+    assert all(int(k) == r for k in t.ranks()), t.ranks()
+    t.free()

@@ -139,7 +139,10 @@ def test_kernel_products(gpu, kind, n, height, ell, thr):
     inverse distance: ranks to ~40, QR recompression of s_tot up to a few
     hundred): A B through the device API, its products vs A (B X) of the
     dense matrices, and its ranks vs the oracle's product of the same two
-    trees (downloaded)."""
+    trees (downloaded).  The product's threshold is absolute (the
+    reference's rule), so it is set to 1e-9 |A B X| / |X|: blocks of A B
+    reach sigma_0 ~ 1e7 here, and an absolute 1e-8 would cut inside the
+    FP64 rounding floor, where any two SVDs disagree on the rank."""
     from hmat_lib_amd import device as D
     _say(f"case: hh kernel kind={kind} n={n} h={height}")
     if kind == 0:
@@ -153,20 +156,21 @@ def test_kernel_products(gpu, kind, n, height, ell, thr):
     a, b = daa.numpy(), dbb.numpy()
     ta = D.DeviceTree.compress(daa, n, height, thr)
     tb = D.DeviceTree.compress(dbb, n, height, thr)
-    tc = ta.multiply(tb, thr)
     x = np.asfortranarray(np.random.default_rng(2).standard_normal((n, 6)))
+    ref = a @ (b @ x)
+    thr_ab = 1e-9 * np.linalg.norm(ref) / np.linalg.norm(x)
+    tc = ta.multiply(tb, thr_ab)
     dx = D.DeviceArray.from_numpy(x)
     dy = D.DeviceArray((n, 6))
     tc.matmat(dx, dy)
-    ref = a @ (b @ x)
-    assert H.rel_err(dy.numpy(), ref) < 1e-8
+    assert H.rel_err(dy.numpy(), ref) < 1e-7
     lib = capi.hmat()
     ierr = C.c_int(0)
     ha = lib.allocate_tree_monolithic(height, C.byref(ierr))
     hb = lib.allocate_tree_monolithic(height, C.byref(ierr))
     ta.download(lib, ha)
     tb.download(lib, hb)
-    oc = H.oracle_hh(ha, hb, thr)
+    oc = H.oracle_hh(ha, hb, thr_ab)
     ro = T.flatten(oc).ranks()
     rc = tc.ranks()
     keys = sorted(ro, key=lambda k: (1 << k[0]) - 2 + k[1])

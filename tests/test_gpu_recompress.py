@@ -153,3 +153,29 @@ def test_random_wide_vs_oracle(gpu, seed):
         assert nd.s == k
         uo, vo = _read(nd)
         np.testing.assert_allclose(nu[i] @ nv[i].T, uo @ vo.T, atol=1e-9)
+
+
+@pytest.mark.parametrize("m,n", [(384, 384), (512, 512), (1000, 384), (384, 1000)])
+def test_rank_deficient_full_width(gpu, m, n):
+    """Nodes whose product U V^T has numerical rank 100 far below s >=
+    min(m, n) (the sums of terms the H x H product recompresses): the
+    dense method orthonormalises a full-width, numerically rank-deficient
+    sketch, and the QR method (s = min(m, n) - 1) rank-deficient factors.
+    Absolute rule, singular values 1e3 .. 1e-11; the cut sits between
+    1.3e-8 and 9.3e-9.  A panel QR that loses orthogonality on such input
+    gives O(1) errors here with the right rank."""
+    from hmat_lib_amd import device as D
+    rng = np.random.default_rng(m + 7 * n)
+    r = 100
+    qa, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    qb, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    sig = np.logspace(3, -11, r)
+    core = qa * sig @ qb.T
+    for method, s in ((1, min(m, n) + 20), (0, min(m, n) - 1)):
+        print(f"case: rank-deficient m={m} n={n} s={s} method={method}", flush=True)
+        mix = rng.standard_normal((r, s))
+        u = np.asfortranarray(qa * sig @ mix)
+        v = np.asfortranarray(qb @ np.linalg.pinv(mix).T)
+        nu, nv = D.recompress_batch([u], [v], 1e-8, method=method)
+        assert nu[0].shape[1] == 78, (method, nu[0].shape)
+        assert np.linalg.norm(nu[0] @ nv[0].T - core, 2) < 2e-11 * np.linalg.norm(core, 2)
