@@ -543,23 +543,6 @@ __global__ __launch_bounds__(256) void k_copy_list(const CopyItem *items) {
     it.dst[e] = it.src[e];
 }
 
-// M = R_U R_V^T for recompression; R_U at RU + moff, R_V at RV + moff
-// (both s x s upper triangular, ld s) from the two QRs.
-__global__ __launch_bounds__(256) void k_rmul(TsvdBlock *blocks, const int *list,
-                                              TsvdBufs bf, const double *RU,
-                                              const double *RV) {
-  const TsvdBlock B = blocks[list[blockIdx.x]];
-  const int s = B.ell;
-  const double *ru = RU + B.moff, *rv = RV + B.moff;
-  double *M = bf.M + B.moff;
-  for (int e = threadIdx.x; e < s * s; e += 256) {
-    const int i = e % s, j = e / s;
-    double acc = 0.0;
-    for (int t = max(i, j); t < s; t++) acc += ru[i + (long long)t * s] * rv[j + (long long)t * s];
-    M[i + (long long)j * s] = acc;
-  }
-}
-
 // D = U V^T (m x n, ld m) for the dense recompression method.
 typedef MfmaTile<2, 2, 2, 2, 16> DenseTile;
 struct OuterJob {
@@ -1003,9 +986,7 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
   std::vector<int> list(nn);
   for (int i = 0; i < nn; i++) list[i] = i;
   TsvdBlock *d_blocks;
-  int *d_list;
   HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
-  HM_OK(to_device(&d_list, list, s)); mem.keep(d_list);
   Timed t0 = timing_begin("recompress_core", s);
   {
     std::vector<QrJob> qj;
@@ -1014,12 +995,22 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
       qj.push_back({bf.Y + b.yoff, b.m, b.m, b.ell, bf.Q + b.qoff, b.m, RU + b.moff, b.ell});
       qj.push_back({bf.Bt + b.boff, b.n, b.n, b.ell, bf.Q2 + b.boff, b.n, RV + b.moff, b.ell});
     }
+    // M = R_U R_V^T (both upper triangular; qr_add zeroes R first)
+    std::vector<GemmJob> mj;
+    for (int i = 0; i < nn; i++) {
+      const TsvdBlock &b = blocks[i];
+      mj.push_back({RU + b.moff, RV + b.moff, bf.M + b.moff, nullptr, b.ell, b.ell, b.ell, b.ell,
+                    b.ell, b.ell, 1, 0, 0, 1.0, 0.0});
+    }
     std::vector<MemRange> inputs = {{bf.Y, (size_t)ytot * 8 + 16}, {bf.Q, (size_t)ytot * 8 + 16},
                                     {bf.Bt, (size_t)btot * 8 + 16}, {bf.Q2, (size_t)btot * 8 + 16},
-                                    {RU, (size_t)mtot * 8 + 16}, {RV, (size_t)mtot * 8 + 16}};
-    HM_OK(program_exec([&](Program &pg, Arena &ar) { return qr_add(pg, ar, qj); }, inputs, s));
+                                    {RU, (size_t)mtot * 8 + 16}, {RV, (size_t)mtot * 8 + 16},
+                                    {bf.M, (size_t)mtot * 8 + 16}};
+    HM_OK(program_exec([&](Program &pg, Arena &ar) {
+      HM_OK(qr_add(pg, ar, qj));
+      return gemm_add(pg, ar, mj, false, true, "recompress core");
+    }, inputs, s));
   }
-  hipLaunchKernelGGL(k_rmul, dim3(nn), dim3(256), 0, s, d_blocks, d_list, bf, RU, RV);
   HM_OK(svd_cores(d_blocks, blocks, list, bf, 0.0, s));
   timing_end(t0, s);
   HM_LAUNCH_CHECK();
