@@ -301,6 +301,10 @@ static int to_device(T **d, const std::vector<T> &h, hipStream_t s) {
 // sweep rotates anything marks its matrix as not converged.
 static constexpr int BJ_W = 32;
 static constexpr int BJ_SCALAR_MAX = 64;
+// inner cyclic sweeps per pair visit: the outer sweeps revisit every pair
+// until none rotates, so a partly diagonalised pair only costs another
+// visit, while each inner sweep costs 189 barriers
+static constexpr int BJ_INNER = 2;
 typedef MfmaTile<2, 2, 2, 2, 16> BjTile;  // 64 x 64
 
 __global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list, TsvdBufs bf) {
@@ -315,9 +319,13 @@ __global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list,
 
 __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4 *items,
                                                    TsvdBufs bf, int *flags) {
-  __shared__ double lds[BjTile::LDS_DOUBLES];
+  // the MFMA staging area lives inside G: the Gram tile is stored to G only
+  // after its last K-chunk, and G is dead while J is applied; 67 KB of LDS
+  // per workgroup lets two share a CU
   __shared__ double G[64][65];
   __shared__ double J[64][65];
+  static_assert(BjTile::LDS_DOUBLES <= 64 * 65, "staging fits in G");
+  double *lds = &G[0][0];
   __shared__ double cs[32][2];
   __shared__ int pq[32][2];
   __shared__ int rot;
@@ -349,7 +357,7 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
   __syncthreads();
   const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
   // (2) cyclic Jacobi on G (round robin over 64 indices, 32 disjoint pairs)
-  for (int sweep = 0; sweep < 4; sweep++) {
+  for (int sweep = 0; sweep < BJ_INNER; sweep++) {
     __syncthreads();
     int any = 0;
     for (int r = 0; r < 63; r++) {
