@@ -303,8 +303,9 @@ static constexpr int BJ_W = 32;
 static constexpr int BJ_SCALAR_MAX = 64;
 // inner cyclic sweeps per pair visit: the outer sweeps revisit every pair
 // until none rotates, so a partly diagonalised pair only costs another
-// visit, while each inner sweep costs 189 barriers
-static constexpr int BJ_INNER = 2;
+// visit, while each inner sweep costs 189 barriers (one sweep measured
+// faster than two or four)
+static constexpr int BJ_INNER = 1;
 typedef MfmaTile<2, 2, 2, 2, 16> BjTile;  // 64 x 64
 
 __global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list, TsvdBufs bf) {
