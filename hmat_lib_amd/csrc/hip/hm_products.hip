@@ -508,26 +508,31 @@ static void launch_apply(const ProdArgs &a, bool nfast, bool fast, int rtiles, i
                          int grid, hipStream_t s) {
   if (fast) {
     if (nfast)
-      hipLaunchKernelGGL((k_mm_apply<true, WM, FM, FN, KC, true>), dim3(grid), dim3(256), 0,
-                         s, a, rtiles, ntiles);
+      hipLaunchKernelGGL((k_mm_apply<true, WM, FM, FN, KC, true>), dim3(grid), dim3(256),
+                         0, s, a, rtiles, ntiles);
     else
-      hipLaunchKernelGGL((k_mm_apply<false, WM, FM, FN, KC, true>), dim3(grid), dim3(256), 0,
-                         s, a, rtiles, ntiles);
+      hipLaunchKernelGGL((k_mm_apply<false, WM, FM, FN, KC, true>), dim3(grid), dim3(256),
+                         0, s, a, rtiles, ntiles);
   } else {
     if (nfast)
-      hipLaunchKernelGGL((k_mm_apply<true, WM, FM, FN, KC, false>), dim3(grid), dim3(256), 0,
-                         s, a, rtiles, ntiles);
-    else
-      hipLaunchKernelGGL((k_mm_apply<false, WM, FM, FN, KC, false>), dim3(grid), dim3(256),
+      hipLaunchKernelGGL((k_mm_apply<true, WM, FM, FN, KC, false>), dim3(grid), dim3(256),
                          0, s, a, rtiles, ntiles);
+    else
+      hipLaunchKernelGGL((k_mm_apply<false, WM, FM, FN, KC, false>), dim3(grid),
+                         dim3(256), 0, s, a, rtiles, ntiles);
   }
 }
 
+// Measured at N = 65536, leaf 256, R = 64: 128 x 64 with four 32 x 64 wave
+// tiles (60.6 us) and 64 x 64 (62.9 us) lead; smaller tiles (more waves per
+// SIMD), one 64 x 64 block per wave (one wave per SIMD), K-chunks of 32 at
+// 128 x 64 and non-temporal loads of D were all slower.
 static const ApplyVariant kVariants[] = {
     {64, 64, 16, launch_apply<2, 2, 2, 16>},   // 64 x 64, KC 16
     {64, 64, 32, launch_apply<2, 2, 2, 32>},   // 64 x 64, KC 32
     {128, 64, 32, launch_apply<2, 4, 2, 32>},  // 128 x 64, KC 32
     {128, 32, 32, launch_apply<4, 2, 2, 32>},  // 128 x 32, KC 32
+    {128, 64, 16, launch_apply<4, 2, 4, 16>},  // 128 x 64, KC 16, 32 x 64 per wave
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 

@@ -108,6 +108,36 @@ def test_products_vs_dense(gpu, monkeypatch, name, leaves, ranks, R, wpath):
         T.detach_and_free(lib, t)
 
 
+@pytest.mark.parametrize("variant", range(5))
+def test_every_apply_tile_variant(gpu, monkeypatch, variant):
+    """Each tile shape of the apply kernel (forced instead of tuned), on a
+    uniform tree (bounds-check-free path) and a ragged one, H X and H^T X,
+    R = 64 and 100."""
+    from hmat_lib_amd import device as D
+    monkeypatch.setenv("HM_APPLY_VARIANT", str(variant))
+    for name, leaves, ranks in (CASES[1], CASES[2], CASES[4]):
+        for R in (64, 100):
+            _say(f"case: apply variant {variant} {name} R={R}")
+            rng = np.random.default_rng(variant * 7 + R)
+            ft = H.random_flat_tree(rng, leaves, ranks)
+            lib = capi.hmat()
+            keep = T.Keep()
+            t = T.fill_tree(lib, ft, keep)
+            try:
+                dense = ft.to_dense()
+                x = np.asfortranarray(rng.standard_normal((ft.n, R)))
+                dt = D.DeviceTree.upload(t.contents.root, ft.height)
+                dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((ft.n, R))
+                dt.matmat(dx, dy)
+                assert capi.hip().hm_hip_matmat_variant(dt.h, 0, R) == variant
+                assert H.rel_err(dy.numpy(), dense @ x) < TOL
+                dt.matmat(dx, dy, trans=1)
+                assert H.rel_err(dy.numpy(), dense.T @ x) < TOL
+                dt.free()
+            finally:
+                T.detach_and_free(lib, t)
+
+
 def test_dropin_api_vs_oracle(gpu):
     """multiply_* of libhmat_lib vs the oracle's restatement, same tree."""
     _say("case: drop-in API vs oracle")
