@@ -176,6 +176,8 @@ def _declare_hip_api(lib):
         "hm_hip_recompress_batch": ([c_int, PInt, PInt, PInt, POINTER(V), POINTER(V),
                                      c_double, c_int, PInt, V], c_int),
         "hm_hip_multiply_hodlr_hodlr": ([V, V, c_double, POINTER(V), V], c_int),
+        "hm_hip_tree_save": ([V, C.c_char_p, V], c_int),
+        "hm_hip_tree_load": ([C.c_char_p, POINTER(V), V], c_int),
         "hm_hip_selftest_mfma": ([PDouble], c_int),
         "hm_hip_selftest_mfma_bf16": ([POINTER(C.c_float), POINTER(C.c_float),
                                        POINTER(C.c_float)], c_int),
@@ -254,6 +256,12 @@ def hmat(test_abi=False):
         lib.hmat_amd_attach.restype = None
         lib.hmat_amd_cache_size.argtypes = []
         lib.hmat_amd_cache_size.restype = c_int
+        lib.hmat_amd_save_tree.argtypes = [PTree, C.c_char_p]
+        lib.hmat_amd_save_tree.restype = c_int
+        lib.hmat_amd_load_tree.argtypes = [C.c_char_p, PInt]
+        lib.hmat_amd_load_tree.restype = PTree
+        lib.hmat_amd_read_dense_text.argtypes = [C.c_char_p, PInt, PInt]
+        lib.hmat_amd_read_dense_text.restype = PDouble
         recomp = [C.POINTER(NodeOffDiagonal), c_int, c_double, PInt]
         lib.recompress.argtypes = recomp
         lib.recompress.restype = c_int

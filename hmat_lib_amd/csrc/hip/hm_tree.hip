@@ -430,3 +430,184 @@ int hm_hip_tree_ranks(const hm_dtree *t, int *ranks) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------- binary files
+// The layout of hmat_amd_save_tree (csrc/host/serialize.c): header, leaf
+// sizes, ranks in block order, padding to 8 bytes, then the diagonal, U and
+// V pools exactly as they lie on the device.
+namespace hm {
+
+static const char kMagic[8] = {'H', 'O', 'D', 'L', 'R', 'B', 'I', 'N'};
+static constexpr uint32_t kVersion = 1, kEndian = 0x01020304u;
+static constexpr int kMaxHeight = 28;
+
+static long long file_header_bytes(int h) {
+  const long long b = 32 + 4LL * ((1LL << h) + (1LL << (h + 1)) - 2);
+  return (b + 7) & ~7LL;
+}
+
+static long long pool_elems(const hm_dtree *t, int which) {
+  long long e = 0;
+  for (const auto &b : t->blk) e += (long long)(which ? b.n : b.m) * b.s;
+  return e;
+}
+
+}  // namespace hm
+
+extern "C" int hm_hip_tree_save(const hm_dtree *t, const char *path, void *stream) {
+  if (!t || !path) {
+    set_error("hm_hip_tree_save: NULL argument");
+    return INPUT_ERROR;
+  }
+  HM_OK(ensure_init());
+  hipStream_t s = stream_of(stream);
+  const int h = t->height;
+  std::vector<int32_t> ints;
+  for (int c = 0; c < t->nleaf; c++) ints.push_back(t->leaf_off[c + 1] - t->leaf_off[c]);
+  for (const auto &b : t->blk) ints.push_back(b.s);
+  const long long nd = t->diag_off[t->nleaf], nu = pool_elems(t, 0), nv = pool_elems(t, 1);
+  std::vector<double> pool((size_t)std::max(std::max(nd, nu), nv) + 1);
+  FILE *f = fopen(path, "wb");
+  if (!f) {
+    set_error("hm_hip_tree_save: cannot open %s", path);
+    return INPUT_ERROR;
+  }
+  unsigned char head[32];
+  memset(head, 0, sizeof(head));
+  memcpy(head, kMagic, 8);
+  const int32_t hh = h;
+  const int64_t n = t->leaf_off[t->nleaf];
+  memcpy(head + 8, &kVersion, 4);
+  memcpy(head + 12, &kEndian, 4);
+  memcpy(head + 16, &hh, 4);
+  memcpy(head + 24, &n, 8);
+  const char zeros[8] = {0};
+  const size_t pad = (size_t)(file_header_bytes(h) - 32 - 4 * (long long)ints.size());
+  bool ok = fwrite(head, 1, 32, f) == 32 &&
+            fwrite(ints.data(), 4, ints.size(), f) == ints.size() &&
+            (pad == 0 || fwrite(zeros, 1, pad, f) == pad);
+  const double *src[3] = {t->d_diag, t->d_U, t->d_V};
+  const long long cnt[3] = {nd, nu, nv};
+  int rc = SUCCESS;
+  for (int k = 0; k < 3 && ok; k++) {
+    if (cnt[k] == 0) continue;
+    hipError_t e = hipMemcpyAsync(pool.data(), src[k], (size_t)cnt[k] * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error("hm_hip_tree_save: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
+      ok = false;
+      break;
+    }
+    ok = fwrite(pool.data(), 8, (size_t)cnt[k], f) == (size_t)cnt[k];
+  }
+  if (fclose(f) != 0) ok = false;
+  if (rc == SUCCESS && !ok) {
+    set_error("hm_hip_tree_save: write to %s failed", path);
+    rc = INPUT_ERROR;
+  }
+  return rc;
+}
+
+extern "C" int hm_hip_tree_load(const char *path, hm_dtree **out, void *stream) {
+  if (!path || !out) {
+    set_error("hm_hip_tree_load: NULL argument");
+    return INPUT_ERROR;
+  }
+  *out = nullptr;
+  HM_OK(ensure_init());
+  hipStream_t s = stream_of(stream);
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    set_error("hm_hip_tree_load: cannot open %s", path);
+    return INPUT_ERROR;
+  }
+  auto fail = [&](const char *why) {
+    set_error("hm_hip_tree_load: %s: %s", path, why);
+    fclose(f);
+    return INPUT_ERROR;
+  };
+  unsigned char head[32];
+  if (fread(head, 1, 32, f) != 32 || memcmp(head, kMagic, 8) != 0) return fail("not a HODLR file");
+  uint32_t ver, end;
+  int32_t h;
+  int64_t n;
+  memcpy(&ver, head + 8, 4);
+  memcpy(&end, head + 12, 4);
+  memcpy(&h, head + 16, 4);
+  memcpy(&n, head + 24, 8);
+  if (ver != kVersion || end != kEndian || h < 1 || h > kMaxHeight || n < 0)
+    return fail("unsupported header");
+  hm_dtree *t = new hm_dtree();
+  t->height = h;
+  t->nleaf = 1 << h;
+  t->nblock = (1 << (h + 1)) - 2;
+  std::vector<int32_t> ints((size_t)t->nleaf + t->nblock);
+  if (fread(ints.data(), 4, ints.size(), f) != ints.size()) {
+    delete t;
+    return fail("truncated header");
+  }
+  t->leaf_off.assign(t->nleaf + 1, 0);
+  bool ok = true;
+  for (int c = 0; c < t->nleaf; c++) {
+    ok &= ints[c] >= 0;
+    t->leaf_off[c + 1] = t->leaf_off[c] + ints[c];
+  }
+  ok &= t->leaf_off[t->nleaf] == n;
+  t->blk.resize(t->nblock);
+  for (int l = 1; l <= h && ok; l++) {
+    const int span = 1 << (h - l);
+    for (int j = 0; j < (1 << l); j++) {
+      BlockMeta &b = t->blk[blk_index(l, j)];
+      b.r0 = t->leaf_off[j * span];
+      b.m = t->leaf_off[(j + 1) * span] - b.r0;
+      b.c0 = t->leaf_off[(j ^ 1) * span];
+      b.n = t->leaf_off[((j ^ 1) + 1) * span] - b.c0;
+      b.level = l;
+      b.j = j;
+      b.pad = 0;
+      b.s = ints[t->nleaf + blk_index(l, j)];
+      ok &= b.s >= 0;
+    }
+  }
+  if (!ok) {
+    delete t;
+    return fail("inconsistent sizes");
+  }
+  long long nd = 0;
+  for (int c = 0; c < t->nleaf; c++) nd += (long long)ints[c] * ints[c];
+  const long long nu = pool_elems(t, 0), nv = pool_elems(t, 1);
+  long long fsize = -1;
+  if (fseek(f, 0, SEEK_END) == 0) fsize = ftell(f);
+  if (fsize != file_header_bytes(h) + 8 * (nd + nu + nv) ||
+      fseek(f, (long)file_header_bytes(h), SEEK_SET) != 0) {
+    delete t;
+    return fail("length does not match the header");
+  }
+  int rc = tree_finalize_meta(t, s);
+  if (rc == SUCCESS) rc = tree_alloc_pools(t);
+  std::vector<double> pool((size_t)std::max(std::max(nd, nu), nv) + 1);
+  double *dst[3] = {t->d_diag, t->d_U, t->d_V};
+  const long long cnt[3] = {nd, nu, nv};
+  for (int k = 0; k < 3 && rc == SUCCESS; k++) {
+    if (cnt[k] == 0) continue;
+    if (fread(pool.data(), 8, (size_t)cnt[k], f) != (size_t)cnt[k]) {
+      set_error("hm_hip_tree_load: %s: read failed", path);
+      rc = INPUT_ERROR;
+      break;
+    }
+    hipError_t e = hipMemcpyAsync(dst[k], pool.data(), (size_t)cnt[k] * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error("hm_hip_tree_load: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
+    }
+  }
+  fclose(f);
+  if (rc != SUCCESS) {
+    tree_free(t);
+    return rc;
+  }
+  *out = t;
+  return SUCCESS;
+}

@@ -143,6 +143,16 @@ class DeviceTree:
         capi.check(capi.hip().hm_hip_matmat(self.h, trans, R, dX.ptr, xs[0], xs[1],
                                             dY.ptr, ys[0], ys[1], None), "matmat")
 
+    def save(self, path: str):
+        """Write the tree in the binary HODLR layout (hm_hip_tree_save)."""
+        capi.check(capi.hip().hm_hip_tree_save(self.h, path.encode(), None), "tree_save")
+
+    @classmethod
+    def load(cls, path: str) -> "DeviceTree":
+        out = C.c_void_p()
+        capi.check(capi.hip().hm_hip_tree_load(path.encode(), C.byref(out), None), "tree_load")
+        return cls(out)
+
     def multiply(self, other: "DeviceTree", thr: float) -> "DeviceTree":
         """self @ other as a new recompressed device tree (H x H)."""
         out = C.c_void_p()

@@ -183,6 +183,14 @@ HM_HIP_API int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
                             double *const *d_v, double thr, int method,
                             int *k_out, void *stream);
 
+/* ---- binary files -------------------------------------------------------
+ * The layout of hmat_amd_save_tree (include/hmat_lib_amd.h), written from /
+ * read into the device pools directly, so a file saved from a host tree
+ * loads as a device tree and vice versa.  Load rejects a malformed or
+ * truncated file with INPUT_ERROR.  No counterpart in the reference. */
+HM_HIP_API int hm_hip_tree_save(const hm_dtree *tree, const char *path, void *stream);
+HM_HIP_API int hm_hip_tree_load(const char *path, hm_dtree **out, void *stream);
+
 /* ---- HODLR x HODLR ------------------------------------------------------
  * *out = a b for two device trees of the same height and leaf sizes (ranks
  * may differ), all on the device.  Every block of the product is formed

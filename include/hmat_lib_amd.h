@@ -38,6 +38,24 @@ void hmat_amd_attach(struct TreeHODLR *tree, hm_dtree *dtree, int *ierr);
 /* Number of cached device trees (diagnostics). */
 int hmat_amd_cache_size(void);
 
+/* Binary HODLR files (one layout for host and device trees, see
+ * hm_hip_tree_save): header (magic "HODLRBIN", version 1, byte-order mark,
+ * height, n, leaf sizes, block ranks in level-major order), then the
+ * diagonal leaves, all U panels and all V panels, column-major FP64.
+ * Save returns an ErrorCode (INPUT_ERROR on a bad tree or an I/O error);
+ * load returns a tree allocated as allocate_tree does with malloc'd
+ * arrays (free with free_tree_hodlr), or NULL with *ierr set (INPUT_ERROR
+ * for a missing, malformed or truncated file).  No counterpart in the
+ * reference. */
+int hmat_amd_save_tree(const struct TreeHODLR *tree, const char *path);
+struct TreeHODLR *hmat_amd_load_tree(const char *path, int *ierr);
+
+/* Dense n x n matrix in the reference's text test format (one row per
+ * line, whitespace-separated; n = the first line's entry count), returned
+ * malloc'd column-major; NULL with *ierr = INPUT_ERROR on a missing or
+ * malformed file.  Replaces tests/utils/io.c:15-81 (read_dense_matrix). */
+double *hmat_amd_read_dense_text(const char *path, int *n, int *ierr);
+
 #ifdef __cplusplus
 }
 #endif

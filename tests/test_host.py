@@ -289,3 +289,106 @@ def test_hh_dropin_argument_errors():
     assert ierr.value == capi.INPUT_ERROR
     lib.free_tree_hodlr(C.byref(t2))
     lib.free_tree_hodlr(C.byref(t3))
+
+
+def _flat_equal(fa, fb):
+    import numpy as np
+    assert fa.height == fb.height and list(fa.leaf_sizes) == list(fb.leaf_sizes)
+    for d1, d2 in zip(fa.diag, fb.diag):
+        np.testing.assert_array_equal(d1, d2)
+    assert fa.blocks.keys() == fb.blocks.keys()
+    for k in fa.blocks:
+        np.testing.assert_array_equal(fa.blocks[k][0], fb.blocks[k][0])
+        np.testing.assert_array_equal(fa.blocks[k][1], fb.blocks[k][1])
+
+
+def test_binary_tree_round_trip(tmp_path):
+    """save -> load gives back every array bit for bit (uneven and empty
+    leaves, ranks 0..4), the internal node sizes, and a file of exactly
+    header + 8 * (sum m_c^2 + sum (m_b + n_b) s_b) bytes."""
+    import ctypes as C
+    import numpy as np
+    from hmat_lib_amd import capi
+    from hmat_lib_amd import tree as T
+    from tests import helpers as H
+    lib = capi.hmat()
+    rng = np.random.default_rng(4)
+    leaves = [7, 0, 12, 5, 9, 1, 6, 10]
+    ft = H.random_flat_tree(rng, leaves, lambda l, j: int(rng.integers(0, 5)))
+    keep = T.Keep()
+    t = T.fill_tree(lib, ft, keep)
+    path = str(tmp_path / "t.hodlr")
+    assert lib.hmat_amd_save_tree(t, path.encode()) == capi.SUCCESS
+    nblk = sum(1 << l for l in range(1, 4))
+    head = (32 + 4 * (8 + nblk) + 7) // 8 * 8
+    payload = sum(m * m for m in leaves)
+    for (l, j), (u, v) in ft.blocks.items():
+        payload += (u.shape[0] + v.shape[0]) * u.shape[1]
+    assert (tmp_path / "t.hodlr").stat().st_size == head + 8 * payload
+    ierr = C.c_int(-1)
+    t2 = lib.hmat_amd_load_tree(path.encode(), C.byref(ierr))
+    assert ierr.value == capi.SUCCESS and t2
+    _flat_equal(ft, T.flatten(t2))
+    lev = T.levels(t2.contents.root, 3)
+    assert t2.contents.root.contents.m == sum(leaves)
+    assert [p.contents.m for p in lev[1]] == [sum(leaves[:4]), sum(leaves[4:])]
+    lib.free_tree_hodlr(C.byref(t2))
+    T.detach_and_free(lib, t)
+
+
+def test_binary_tree_rejects_bad_files(tmp_path):
+    """Wrong magic, a truncated payload, leaf sizes that do not add up and a
+    missing file are INPUT_ERROR with no tree."""
+    import ctypes as C
+    import struct
+    import numpy as np
+    from hmat_lib_amd import capi
+    from hmat_lib_amd import tree as T
+    from tests import helpers as H
+    lib = capi.hmat()
+    rng = np.random.default_rng(5)
+    ft = H.random_flat_tree(rng, [4, 3, 5, 2], lambda l, j: 1)
+    keep = T.Keep()
+    t = T.fill_tree(lib, ft, keep)
+    good = tmp_path / "good.hodlr"
+    assert lib.hmat_amd_save_tree(t, str(good).encode()) == capi.SUCCESS
+    T.detach_and_free(lib, t)
+    data = good.read_bytes()
+    cases = {"magic": b"XODLRBIN" + data[8:], "truncated": data[:-8],
+             "longer": data + b"\0" * 8,
+             "sum": data[:24] + struct.pack("<q", 15) + data[32:]}
+    for name, blob in cases.items():
+        p = tmp_path / f"{name}.hodlr"
+        p.write_bytes(blob)
+        ierr = C.c_int(0)
+        assert not lib.hmat_amd_load_tree(str(p).encode(), C.byref(ierr)), name
+        assert ierr.value == capi.INPUT_ERROR, name
+    ierr = C.c_int(0)
+    assert not lib.hmat_amd_load_tree(str(tmp_path / "none").encode(), C.byref(ierr))
+    assert ierr.value == capi.INPUT_ERROR
+
+
+def test_read_dense_text_reference_format(tmp_path):
+    """The reference's test format (tests/utils/io.c:15-81): one row per
+    line, spaces or tabs; read into column-major storage.  A file without a
+    newline or with missing entries is INPUT_ERROR."""
+    import ctypes as C
+    import numpy as np
+    from hmat_lib_amd import capi
+    lib = capi.hmat()
+    a = np.random.default_rng(6).standard_normal((5, 5))
+    lines = ["\t".join(f"{x:.17g}" for x in row) if i % 2 else
+             "  ".join(f"{x:.17g}" for x in row) for i, row in enumerate(a)]
+    p = tmp_path / "m.txt"
+    p.write_text("\n".join(lines) + "\n")
+    n, ierr = C.c_int(0), C.c_int(-1)
+    ptr = lib.hmat_amd_read_dense_text(str(p).encode(), C.byref(n), C.byref(ierr))
+    assert ierr.value == capi.SUCCESS and n.value == 5
+    got = np.ctypeslib.as_array(ptr, shape=(25,)).reshape(5, 5, order="F").copy()
+    np.testing.assert_array_equal(got, a)
+    C.CDLL("libc.so.6").free(C.cast(ptr, C.c_void_p))
+    for name, text in (("nonl", "1 2 3"), ("short", "1 2\n3\n")):
+        q = tmp_path / name
+        q.write_text(text)
+        assert not lib.hmat_amd_read_dense_text(str(q).encode(), C.byref(n), C.byref(ierr))
+        assert ierr.value == capi.INPUT_ERROR, name
