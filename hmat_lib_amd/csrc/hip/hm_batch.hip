@@ -46,7 +46,12 @@ struct BatchOperands {
 
 void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, const double *X,
                      long long ldx, int R, double *Y, long long ldy, std::vector<GemmJob> &wj,
-                     std::vector<GemmChain> &chains, std::vector<GemmLink> &links) {
+                     std::vector<GemmChain> &chains, std::vector<GemmLink> &links,
+                     bool rowmajor) {
+  // element (row r, column j) of X^T / Y^T when rowmajor, of X / Y otherwise
+  auto at = [&](auto *base, long long ld, long long r) {
+    return rowmajor ? base + r * ld : base + r;
+  };
   if (R <= 0) return;
   const int h = H->height, span = 1 << (h - l);
   const int r0 = H->leaf_off[q * span];
@@ -62,8 +67,8 @@ void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, cons
       const int K = trans ? b.m : b.n;
       const int src = (trans ? b.r0 : b.c0) - r0;
       // W_b (s x R, ld s) = F^T X[src : src + K, :]
-      wj.push_back({F, X + src, W[g], nullptr, std::max(K, 1), ldx, b.s, b.s, R, K, 1, 0, 0,
-                    1.0, 0.0});
+      wj.push_back({F, at(X, ldx, src), W[g], nullptr, std::max(K, 1), ldx, b.s, b.s, R, K, 1,
+                    0, 0, 1.0, 0.0});
     }
   }
   // Y_c = op(D_c) X_c + sum over levels of the output factor's rows of
@@ -72,12 +77,15 @@ void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, cons
     const int lo = H->leaf_off[c], mc = H->leaf_off[c + 1] - lo;
     if (mc <= 0) continue;
     GemmChain ch;
-    ch.C = Y + (lo - r0);
+    ch.C = at(Y, ldy, lo - r0);
     ch.ldc = ldy;
     ch.M = mc;
     ch.N = R;
+    ch.tc = rowmajor ? 1 : 0;
     ch.link0 = (int)links.size();
-    links.push_back({H->d_diag + H->diag_off[c], X + (lo - r0), mc, ldx, mc, trans ? 1 : 0});
+    GemmLink dl{H->d_diag + H->diag_off[c], at(X, ldx, lo - r0), mc, ldx, mc, trans ? 1 : 0};
+    dl.tb = rowmajor ? 1 : 0;
+    links.push_back(dl);
     for (int l2 = l + 1; l2 <= h; l2++) {
       const int q2 = c >> (h - l2);
       // H X: block (l2, q2) has the leaf's rows (U); H^T X: block
@@ -94,15 +102,18 @@ void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, cons
   }
 }
 
+// trans: 0 Y = H X, 1 Y = H^T X, 2 Y = X H (X, Y: R x n), taken as
+// Y^T = H^T X^T.
 static int build_batch(Program &p, Arena &ar, const std::vector<hm_dtree *> &trees, int trans,
                        const std::vector<BatchOperands> &ops) {
   std::vector<GemmJob> wj;
   std::vector<GemmChain> chains;
   std::vector<GemmLink> links;
+  const bool rowmajor = trans == 2;
   for (size_t i = 0; i < trees.size(); i++)
-    subtree_product(ar, trees[i], 0, 0, trans, ops[i].X, ops[i].ldx, ops[i].R, ops[i].Y,
-                    ops[i].ldy, wj, chains, links);
-  HM_OK(gemm_add(p, ar, wj, true, false, "batch W", -1));
+    subtree_product(ar, trees[i], 0, 0, trans ? 1 : 0, ops[i].X, ops[i].ldx, ops[i].R,
+                    ops[i].Y, ops[i].ldy, wj, chains, links, rowmajor);
+  HM_OK(gemm_add(p, ar, wj, true, rowmajor, "batch W", -1));
   HM_OK(gemm_chain_add(p, chains, links, "batch apply", -1));
   return SUCCESS;
 }
@@ -127,15 +138,18 @@ int hm_hip_batch_create(int ntrees, hm_dtree *const *trees, int trans, const int
   std::vector<MemRange> inputs;
   for (int i = 0; i < ntrees; i++) {
     const hm_dtree *t = tv[i];
-    if (!t || R[i] < 0 || (R[i] > 0 && (!d_X[i] || !d_Y[i] || ldx[i] < std::max(1, t->n) ||
-                                        ldy[i] < std::max(1, t->n)))) {
+    // H X, H^T X: X, Y are n x R; X H: R x n
+    const long rows = trans == 2 ? R[i] : (t ? t->n : 0);
+    const long cols = trans == 2 ? (t ? t->n : 0) : R[i];
+    if (!t || R[i] < 0 || (R[i] > 0 && (!d_X[i] || !d_Y[i] || ldx[i] < std::max(1L, rows) ||
+                                        ldy[i] < std::max(1L, rows)))) {
       set_error("hm_hip_batch_create: tree %d: bad operands", i);
       return INPUT_ERROR;
     }
     ops[i] = {R[i], d_X[i], ldx[i], d_Y[i], ldy[i]};
-    if (R[i] > 0) {
-      inputs.push_back({d_X[i], ((size_t)(R[i] - 1) * ldx[i] + t->n) * sizeof(double)});
-      inputs.push_back({d_Y[i], ((size_t)(R[i] - 1) * ldy[i] + t->n) * sizeof(double)});
+    if (R[i] > 0 && cols > 0) {
+      inputs.push_back({d_X[i], ((size_t)(cols - 1) * ldx[i] + rows) * sizeof(double)});
+      inputs.push_back({d_Y[i], ((size_t)(cols - 1) * ldy[i] + rows) * sizeof(double)});
     }
     long long u = 0, v = 0;
     for (const auto &b : t->blk) {
@@ -160,7 +174,7 @@ int hm_hip_batch_create(int ntrees, hm_dtree *const *trees, int trans, const int
   }
   hm_batch *b = new hm_batch();
   b->trees = tv;
-  b->trans = trans ? 1 : 0;
+  b->trans = trans;
   Arena ar;
   ar.counting = false;
   ar.cap = count.off + 256;

@@ -107,18 +107,21 @@ static constexpr int GEMM_MAX_GRID = 4096;
 int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
              const char *name, int variant = 0);
 
-// Chained GEMM: C (M x N) = sum over its links of op(A_i) B_i, the links
-// concatenated along K (B column-major; op(A) = A^T when ta).  One pass
-// over C; same tile variants as gemm_add.
+// Chained GEMM: C (M x N) = sum over its links of op(A_i) op(B_i), the links
+// concatenated along K (column-major; op(A) = A^T when ta, op(B) = B^T when
+// tb, B then stored N x K; with tc, C is stored transposed, C(i, j) at
+// C[j + i * ldc]).  One pass over C; same tile variants as gemm_add.
 struct GemmLink {
   const double *A, *B;
   long long lda, ldb;
   int K, ta;
+  int tb = 0, pad = 0;
 };
 struct GemmChain {
   double *C;
   long long ldc;
   int M, N, link0, nlink;
+  int tc = 0, pad = 0;
 };
 int gemm_chain_add(Program &p, const std::vector<GemmChain> &chains,
                    const std::vector<GemmLink> &links, const char *name, int variant = 0);
@@ -162,8 +165,13 @@ int colnorm_add(Program &p, const std::vector<ColNorm> &items);
 // GEMM per leaf c: Y_c = op(D_c) X_c + sum over the levels below l of
 // F_out(rows of c) W_b, the diagonal leaf and the blocks concatenated
 // along K.  F_in / F_out are V / U for H X and U / V for H^T X.
+// rowmajor: X and Y are given as R x n column-major (leading dims ldx,
+// ldy, starting at the node's first column) and the product is taken on
+// their transposes, Y^T = op(H_qq) X^T — the W jobs then need gemm_add
+// with tb = true.
 void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, const double *X,
                      long long ldx, int R, double *Y, long long ldy, std::vector<GemmJob> &wj,
-                     std::vector<GemmChain> &chains, std::vector<GemmLink> &links);
+                     std::vector<GemmChain> &chains, std::vector<GemmLink> &links,
+                     bool rowmajor = false);
 
 }  // namespace hm

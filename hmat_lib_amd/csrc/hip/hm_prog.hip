@@ -234,24 +234,38 @@ __global__ __launch_bounds__(256) void k_gemm_chain(const GemmChain *chains,
     tile.zero();
     for (int q = ch.link0; q < ch.link0 + ch.nlink; q++) {
       const GemmLink lk = links[q];
-      auto lb = [&](int k, int j) -> double {
-        return (k < lk.K && j0 + j < ch.N) ? lk.B[k + (long long)(j0 + j) * lk.ldb] : 0.0;
+      auto la_n = [&](int i, int k) -> double {
+        return (i0 + i < ch.M && k < lk.K) ? lk.A[(i0 + i) + (long long)k * lk.lda] : 0.0;
       };
-      if (lk.ta) {
-        auto la = [&](int i, int k) -> double {
-          return (i0 + i < ch.M && k < lk.K) ? lk.A[k + (long long)(i0 + i) * lk.lda] : 0.0;
+      auto la_t = [&](int i, int k) -> double {
+        return (i0 + i < ch.M && k < lk.K) ? lk.A[k + (long long)(i0 + i) * lk.lda] : 0.0;
+      };
+      if (lk.tb) {
+        auto lb = [&](int k, int j) -> double {
+          return (k < lk.K && j0 + j < ch.N) ? lk.B[(j0 + j) + (long long)k * lk.ldb] : 0.0;
         };
-        tile.template run<false, false>(lds, lk.K, la, lb);
+        if (lk.ta)
+          tile.template run<false, true>(lds, lk.K, la_t, lb);
+        else
+          tile.template run<true, true>(lds, lk.K, la_n, lb);
       } else {
-        auto la = [&](int i, int k) -> double {
-          return (i0 + i < ch.M && k < lk.K) ? lk.A[(i0 + i) + (long long)k * lk.lda] : 0.0;
+        auto lb = [&](int k, int j) -> double {
+          return (k < lk.K && j0 + j < ch.N) ? lk.B[k + (long long)(j0 + j) * lk.ldb] : 0.0;
         };
-        tile.template run<true, false>(lds, lk.K, la, lb);
+        if (lk.ta)
+          tile.template run<false, false>(lds, lk.K, la_t, lb);
+        else
+          tile.template run<true, false>(lds, lk.K, la_n, lb);
       }
     }
-    tile.store([&](int i, int j, double v) {
-      if (i0 + i < ch.M && j0 + j < ch.N) ch.C[(i0 + i) + (long long)(j0 + j) * ch.ldc] = v;
-    });
+    if (ch.tc)
+      tile.store([&](int i, int j, double v) {
+        if (i0 + i < ch.M && j0 + j < ch.N) ch.C[(j0 + j) + (long long)(i0 + i) * ch.ldc] = v;
+      });
+    else
+      tile.store([&](int i, int j, double v) {
+        if (i0 + i < ch.M && j0 + j < ch.N) ch.C[(i0 + i) + (long long)(j0 + j) * ch.ldc] = v;
+      });
     __syncthreads();
   }
 }
@@ -263,13 +277,15 @@ int gemm_chain_add(Program &p, const std::vector<GemmChain> &chains,
     const GemmChain &c = chains[q];
     if (c.M <= 0 || c.N <= 0) continue;
     live.push_back((int)q);
-    p.touch(c.C, ((size_t)(c.N - 1) * c.ldc + c.M) * sizeof(double), name);
+    const size_t cb = c.tc ? ((size_t)(c.M - 1) * c.ldc + c.N) : ((size_t)(c.N - 1) * c.ldc + c.M);
+    p.touch(c.C, cb * sizeof(double), name);
     for (int i = c.link0; i < c.link0 + c.nlink; i++) {
       const GemmLink &l = links[i];
       if (l.K <= 0) continue;
       const size_t ab = l.ta ? ((size_t)(c.M - 1) * l.lda + l.K) : ((size_t)(l.K - 1) * l.lda + c.M);
+      const size_t bb = l.tb ? ((size_t)(l.K - 1) * l.ldb + c.N) : ((size_t)(c.N - 1) * l.ldb + l.K);
       p.touch(l.A, ab * sizeof(double), name);
-      p.touch(l.B, ((size_t)(c.N - 1) * l.ldb + l.K) * sizeof(double), name);
+      p.touch(l.B, bb * sizeof(double), name);
     }
   }
   if (live.empty()) return SUCCESS;

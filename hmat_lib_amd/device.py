@@ -231,7 +231,9 @@ class Batch:
         n = len(trees)
         self.trees, self.xs, self.ys = trees, xs, ys
         pt = (C.c_void_p * n)(*[t.h.value for t in trees])
-        R = (C.c_int * n)(*[(x.shape[1] if len(x.shape) > 1 else 1) for x in xs])
+        # trans 0 / 1: X, Y are n x R; trans 2 (Y = X H): R x n
+        R = (C.c_int * n)(*[(x.shape[0] if trans == 2 else
+                             (x.shape[1] if len(x.shape) > 1 else 1)) for x in xs])
         px = (C.c_void_p * n)(*[x.ptr.value for x in xs])
         py = (C.c_void_p * n)(*[y.ptr.value for y in ys])
         lx = (C.c_long * n)(*[x.shape[0] for x in xs])

@@ -240,12 +240,14 @@ HM_HIP_API int hm_hip_compress_batch(int nmat, const double *const *d_A, const l
                                      const hm_compress_opts *opts, hm_dtree **out,
                                      hm_compress_stats *stats, void *stream);
 
-/* A batched product Y_i = op(H_i) X_i over ntrees trees with R[i] columns
- * each (X_i, Y_i device, column-major, leading dimensions ldx[i], ldy[i]),
- * fixed at creation: one launch program of 2 + max height batched-GEMM
- * launches.  The batch keeps pointers to the trees and operands (it owns
- * neither).  Replaces per-tree loops of multiply_hodlr_dense /
- * multiply_hodlr_transpose_dense (src/dense_algebra.c:361-648). */
+/* A batched product over ntrees trees, fixed at creation: trans 0:
+ * Y_i = H_i X_i, 1: Y_i = H_i^T X_i (X_i, Y_i n_i x R[i]); 2: Y_i = X_i H_i
+ * (X_i, Y_i R[i] x n_i); device arrays, column-major, leading dimensions
+ * ldx[i], ldy[i].  One launch program of two batched-GEMM launches
+ * whatever the number of trees or levels.  The batch keeps pointers to
+ * the trees and operands (it owns neither).  Replaces per-tree loops of
+ * multiply_hodlr_dense / multiply_hodlr_transpose_dense /
+ * multiply_dense_hodlr (src/dense_algebra.c:361-648, 957-1026). */
 typedef struct hm_batch hm_batch;
 HM_HIP_API int hm_hip_batch_create(int ntrees, hm_dtree *const *trees, int trans, const int *R,
                                    const double *const *d_X, const long *ldx,

@@ -93,6 +93,36 @@ def test_batch_products_vs_dense(gpu, trans):
     b.free()
 
 
+def test_batch_dense_times_hodlr(gpu):
+    """Batched X_i H_i (X_i, Y_i: R_i x n_i, the batched form of
+    multiply_dense_hodlr): taken as Y^T = H^T X^T with transposed operand
+    access and transposed stores; untuned, tuned and replayed from a graph."""
+    from hmat_lib_amd import device as D
+    _say("case: batch of 14 uneven trees, X H")
+    forest, rng = _forest(23, 14)
+    xs, ys, xh, dense = [], [], [], []
+    for ft, dt, R in forest:
+        x = np.asfortranarray(rng.standard_normal((R, ft.n)))
+        xh.append(x)
+        xs.append(D.DeviceArray.from_numpy(x))
+        ys.append(D.DeviceArray((R, ft.n)))
+        dense.append(ft.to_dense())
+    b = D.Batch([f[1] for f in forest], xs, ys, trans=2)
+
+    def check():
+        for i in range(len(forest)):
+            assert H.rel_err(ys[i].numpy(), xh[i] @ dense[i]) < TOL, i
+
+    b.run()
+    D.synchronize()
+    check()
+    b.tune()
+    b.run(graph=True)
+    D.synchronize()
+    check()
+    b.free()
+
+
 def test_batch_compress_exp1d(gpu):
     """Six sorted 1-D exponential-kernel matrices of uneven size in one batched
     compression: every block is exactly rank 1 (closed form), ranks equal the
