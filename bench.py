@@ -37,7 +37,7 @@ PEAK_FP64 = 78.6e12        # MI355X dense FP64 (matrix), flop/s
 # (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by the counts of a
 # 256 MiB device copy (fetch x2.000, write x1.000); profiles/ holds the
 # commands and the per-kernel table.
-MEASURED_TRAFFIC = {("exp1d_64k", "k_mm_apply"): 206633862}
+MEASURED_TRAFFIC = {("exp1d_64k", "k_mm_apply"): 206557491}
 
 
 def log(msg):
