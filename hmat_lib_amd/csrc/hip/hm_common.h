@@ -190,6 +190,9 @@ struct MfmaTile {
   static constexpr int LDS_DOUBLES = KC * LDA + KC * LDB;
   static constexpr int AREG = (TM * KC + NT - 1) / NT;
   static constexpr int BREG = (TN * KC + NT - 1) / NT;
+  // every thread owns whole elements (threadIdx.x < NT by the launch)
+  static constexpr bool AEXACT = (TM * KC) % NT == 0;
+  static constexpr bool BEXACT = (TN * KC) % NT == 0;
 
   d4 acc[FM][FN];
   double ra[AREG], rb[BREG];
@@ -210,7 +213,7 @@ struct MfmaTile {
       int i, k;
       if (A_MFAST) { i = idx % TM; k = idx / TM; }
       else         { k = idx % KC; i = idx / KC; }
-      ra[e] = (idx < TM * KC) ? la(i, k0 + k) : 0.0;
+      ra[e] = (AEXACT || idx < TM * KC) ? la(i, k0 + k) : 0.0;
     }
 #pragma unroll
     for (int e = 0; e < BREG; e++) {
@@ -218,7 +221,7 @@ struct MfmaTile {
       int j, k;
       if (B_NFAST) { j = idx % TN; k = idx / TN; }
       else         { k = idx % KC; j = idx / KC; }
-      rb[e] = (idx < TN * KC) ? lb(k0 + k, j) : 0.0;
+      rb[e] = (BEXACT || idx < TN * KC) ? lb(k0 + k, j) : 0.0;
     }
   }
 
@@ -232,7 +235,7 @@ struct MfmaTile {
       int i, k;
       if (A_MFAST) { i = idx % TM; k = idx / TM; }
       else         { k = idx % KC; i = idx / KC; }
-      if (idx < TM * KC) As[k * LDA + i] = ra[e];
+      if (AEXACT || idx < TM * KC) As[k * LDA + i] = ra[e];
     }
 #pragma unroll
     for (int e = 0; e < BREG; e++) {
@@ -240,7 +243,7 @@ struct MfmaTile {
       int j, k;
       if (B_NFAST) { j = idx % TN; k = idx / TN; }
       else         { k = idx % KC; j = idx / KC; }
-      if (idx < TN * KC) Bs[k * LDB + j] = rb[e];
+      if (BEXACT || idx < TN * KC) Bs[k * LDB + j] = rb[e];
     }
   }
 
