@@ -38,6 +38,9 @@ PEAK_FP64 = 78.6e12        # MI355X dense FP64 (matrix), flop/s
 # 256 MiB device copy (fetch x2.000, write x1.000); profiles/ holds the
 # commands and the per-kernel table.
 MEASURED_TRAFFIC = {("exp1d_64k", "k_mm_apply"): 206557491}
+# Apply kernel launched for each tile variant the autotuner can keep (0-4:
+# k_mm_apply tile shapes; 5, 6: the leaf-resident kernels).
+APPLY_KERNELS = {5: "k_leaf_apply", 6: "k_leaf_apply2"}
 
 
 def log(msg):
@@ -489,13 +492,15 @@ def main():
         roof = {"bound": "hbm", "achieved": bytes_alg / k_avg / 1e9,
                 "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = MEASURED_TRAFFIC.get((w.name, "k_mm_apply"))
-    roof["kernel"] = "k_mm_apply"
+    variant = int(capi.hip().hm_hip_matmat_variant(dt.h, 0, R))
+    kname = APPLY_KERNELS.get(variant, "k_mm_apply")
+    roof["traffic"] = MEASURED_TRAFFIC.get((w.name, kname))
+    roof["kernel"] = kname
     roof["kernel_avg_us"] = k_avg * 1e6
     roof["algorithmic_bytes"] = bytes_alg
     roof["algorithmic_flops"] = flops_alg
     roof["hbm_frac"] = bytes_alg / k_avg / PEAK_HBM
-    roof["tile_variant"] = int(capi.hip().hm_hip_matmat_variant(dt.h, 0, R))
+    roof["tile_variant"] = variant
 
     # CPU baseline: reference multiply_hodlr_dense on the same tree
     cpu = None

@@ -226,7 +226,7 @@ struct MfmaTile {
   }
 
   template <bool A_MFAST, bool B_NFAST>
-  __device__ void stash(double *lds) {
+  __device__ void stash_regs(double *lds, const double *xa, const double *xb) const {
     double *As = lds, *Bs = lds + KC * LDA;
     const int t = threadIdx.x;
 #pragma unroll
@@ -235,7 +235,7 @@ struct MfmaTile {
       int i, k;
       if (A_MFAST) { i = idx % TM; k = idx / TM; }
       else         { k = idx % KC; i = idx / KC; }
-      if (AEXACT || idx < TM * KC) As[k * LDA + i] = ra[e];
+      if (AEXACT || idx < TM * KC) As[k * LDA + i] = xa[e];
     }
 #pragma unroll
     for (int e = 0; e < BREG; e++) {
@@ -243,8 +243,13 @@ struct MfmaTile {
       int j, k;
       if (B_NFAST) { j = idx % TN; k = idx / TN; }
       else         { k = idx % KC; j = idx / KC; }
-      if (BEXACT || idx < TN * KC) Bs[k * LDB + j] = rb[e];
+      if (BEXACT || idx < TN * KC) Bs[k * LDB + j] = xb[e];
     }
+  }
+
+  template <bool A_MFAST, bool B_NFAST>
+  __device__ void stash(double *lds) {
+    stash_regs<A_MFAST, B_NFAST>(lds, ra, rb);
   }
 
   __device__ void compute(const double *lds) {
@@ -286,7 +291,9 @@ struct MfmaTile {
   // Interior form of run(): A is column-major with leading dimension lda
   // (rows contiguous), B(k, j) sits at B[k * brs + j * bcs]; the tile lies
   // fully inside both operands and K is a multiple of KC, so each thread
-  // walks fixed pointers with no index arithmetic or bounds checks.
+  // walks fixed pointers with no index arithmetic or bounds checks.  Two
+  // register sets alternate: the loads of chunk c + 2 are issued as soon as
+  // chunk c is in LDS, so each chunk has two compute phases to arrive.
   template <bool B_NFAST>
   __device__ void run_interior(double *lds, int K, const double *A, long long lda,
                                const double *B, long long brs, long long bcs) {
@@ -305,22 +312,54 @@ struct MfmaTile {
       bes = (long long)(NT / KC) * bcs;
     }
     const long long bstep = (long long)KC * brs;
+    double ra2[AREG], rb2[BREG];
+    auto load = [&](double *xa, double *xb) {
 #pragma unroll
-    for (int e = 0; e < AREG; e++) ra[e] = pa[e * aes];
+      for (int e = 0; e < AREG; e++) xa[e] = pa[e * aes];
 #pragma unroll
-    for (int e = 0; e < BREG; e++) rb[e] = pb[e * bes];
-    for (int k0 = 0; k0 < K; k0 += KC) {
+      for (int e = 0; e < BREG; e++) xb[e] = pb[e * bes];
+      pa += astep;
+      pb += bstep;
+    };
+    const int nch = K / KC;
+    load(ra, rb);
+    // second set unconditionally (chunk 0 again when there is one chunk):
+    // with no conditional loads before or inside the loop the compiler's
+    // wait counts stay exact and each stash waits for its own set only
+    if (nch == 1) {
+      pa -= astep;
+      pb -= bstep;
+    }
+    load(ra2, rb2);
+    int c = 0;
+    for (; c + 3 < nch; c += 2) {
       __syncthreads();
-      stash<true, B_NFAST>(lds);
+      stash_regs<true, B_NFAST>(lds, ra, rb);
       __syncthreads();
-      if (k0 + KC < K) {
-        pa += astep;
-        pb += bstep;
-#pragma unroll
-        for (int e = 0; e < AREG; e++) ra[e] = pa[e * aes];
-#pragma unroll
-        for (int e = 0; e < BREG; e++) rb[e] = pb[e * bes];
-      }
+      load(ra, rb);
+      compute(lds);
+      __syncthreads();
+      stash_regs<true, B_NFAST>(lds, ra2, rb2);
+      __syncthreads();
+      load(ra2, rb2);
+      compute(lds);
+    }
+    // last one to three chunks (c and c + 1 already in flight)
+    __syncthreads();
+    stash_regs<true, B_NFAST>(lds, ra, rb);
+    __syncthreads();
+    if (c + 2 < nch) load(ra, rb);
+    compute(lds);
+    if (c + 1 < nch) {
+      __syncthreads();
+      stash_regs<true, B_NFAST>(lds, ra2, rb2);
+      __syncthreads();
+      compute(lds);
+    }
+    if (c + 2 < nch) {
+      __syncthreads();
+      stash_regs<true, B_NFAST>(lds, ra, rb);
+      __syncthreads();
       compute(lds);
     }
     __syncthreads();

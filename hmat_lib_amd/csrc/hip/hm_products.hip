@@ -495,6 +495,420 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   return a;
 }
 
+// ------------------------------------------------ leaf-resident apply
+// One workgroup of NW waves per (leaf, row tile of 32 NW rows, 64 RHS);
+// wave w owns 32 rows of the tile and all 64 columns (2 x 4 MFMA blocks).
+// No element of D is used by two waves, so D streams from HBM straight into
+// registers, one 32-column chunk ahead; X_c streams through two LDS buffers
+// of 32 rows x 64 columns, one barrier per chunk.  In LDS the pair (rows
+// 2p, 2p + 1) of column j sits in 16-byte slot j * 16 + (p ^ (j & 15)), so
+// the 16 columns one MFMA operand reads fall in distinct banks while each
+// 16 threads still load 256 contiguous bytes of a column.  NCH (32-column
+// chunks per leaf) is a template argument: the diagonal pipeline is fully
+// unrolled and no load inside it is conditional.  The low-rank part (K over
+// the leaf's (level, rank) pairs, A = rows of U_b, B = W_b) follows as
+// further chunks of at most 32, trimmed to whole MFMA k-steps; the first
+// one is fetched while the last diagonal chunk computes.
+// Requires (host-checked): uniform leaves of 32 NCH rows, 32 NW | leaf,
+// no transpose, X column-major (x_rs = 1) with 16-byte aligned columns,
+// R a multiple of 64.
+static constexpr int LEAF_KC = 32, LEAF_TN = 64;
+typedef double d2 __attribute__((ext_vector_type(2)));
+// Load through a global-address-space pointer (pointers kept in LDS are
+// generic; generic loads also count against the LDS wait counter).
+__device__ __forceinline__ double ldg(const double *p) {
+  return *(const __attribute__((address_space(1))) double *)(p);
+}
+
+template <int NW, int NCH>
+__global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, int ntiles) {
+  constexpr int NT = NW * 64;
+  constexpr int KC = LEAF_KC, TN = LEAF_TN, M = KC * NCH;
+  constexpr int XPAIRS = KC / 2 * TN;
+  constexpr int XPT = XPAIRS / NT;
+  static_assert(XPAIRS % NT == 0 && M % (32 * NW) == 0, "leaf tile shape");
+  __shared__ d2 xs[2][XPAIRS];
+  __shared__ LevelInfo li[MAX_LEVELS];
+  __shared__ int lp[MAX_LEVELS + 1];
+  __shared__ const double *tabA[KC];
+  __shared__ const double *tabB[KC];
+  __shared__ int tabS[KC];
+  int c, rt, nt;
+  if (!apply_tile_of(blockIdx.x, a.nleaf, rtiles, ntiles, c, rt, nt)) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int i16 = lane & 15, lk = lane >> 4;
+  const int lo = a.leaf_off[c];
+  const int wr = rt * (32 * NW) + 32 * w;  // this wave's first row in the leaf
+  const int n0 = nt * TN;
+  const double *pA = a.diag + a.diag_off[c] + wr + i16 + (long long)lk * M;
+  // X pairs of this thread: p = t & 15 (row pair), columns j_e = (t >> 4) + e NT / 16
+  const int xp = t & 15;
+  const double *pX = a.X + lo + 2 * xp + (long long)(n0 + (t >> 4)) * a.x_cs;
+  const long long xes = (long long)(NT / 16) * a.x_cs;
+
+  d4 acc[2][4];
+#pragma unroll
+  for (int f = 0; f < 2; f++)
+#pragma unroll
+    for (int g = 0; g < 4; g++) acc[f][g] = d4{0.0, 0.0, 0.0, 0.0};
+  double ar[2][16];
+  d2 xr[XPT];
+  double lr[16];
+  d2 lb[XPT];
+  auto loadA = [&](int ch, double *dst) {
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+#pragma unroll
+      for (int f = 0; f < 2; f++) dst[ks * 2 + f] = pA[16 * f + (long long)(KC * ch + 4 * ks) * M];
+  };
+  auto loadX = [&](int ch) {
+#pragma unroll
+    for (int e = 0; e < XPT; e++)
+      xr[e] = *reinterpret_cast<const d2 *>(pX + KC * ch + e * xes);
+  };
+  auto slot = [&](int e) {
+    const int j = (t >> 4) + e * (NT / 16);
+    return j * 16 + (xp ^ (j & 15));
+  };
+  // k-steps 0 .. nks-1 of the chunk in LDS buffer buf (nks = 8: all)
+  auto compute = [&](int buf, const double *av, int nks) {
+    const double *xb = reinterpret_cast<const double *>(xs[buf]);
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) {
+      if (ks < nks) {
+        const int pp = 2 * ks + (lk >> 1);
+        double b[4];
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+          b[g] = xb[((16 * g + i16) * 16 + (pp ^ i16)) * 2 + (lk & 1)];
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+          for (int g = 0; g < 4; g++) acc[f][g] = mfma_f64(av[ks * 2 + f], b[g], acc[f][g]);
+      }
+    }
+  };
+  // low-rank chunk operands from the table (every entry first, then every
+  // load, unconditional from a valid stand-in address where empty)
+  auto loadLR = [&]() {
+    const double *qa[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) qa[ks] = tabA[4 * ks + lk];
+    const double *q0 = tabB[2 * xp], *q1 = tabB[2 * xp + 1];
+    const long long s0 = tabS[2 * xp], s1 = tabS[2 * xp + 1];
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+#pragma unroll
+      for (int f = 0; f < 2; f++)
+        lr[ks * 2 + f] = ldg(qa[ks] ? qa[ks] + wr + 16 * f + i16 : a.diag);
+#pragma unroll
+    for (int e = 0; e < XPT; e++) {
+      const int j = n0 + (t >> 4) + e * (NT / 16);
+      lb[e].x = ldg(q0 ? q0 + j * s0 : a.diag);
+      lb[e].y = ldg(q1 ? q1 + j * s1 : a.diag);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+#pragma unroll
+      for (int f = 0; f < 2; f++)
+        if (!qa[ks]) lr[ks * 2 + f] = 0.0;
+#pragma unroll
+    for (int e = 0; e < XPT; e++) {
+      if (!q0) lb[e].x = 0.0;
+      if (!q1) lb[e].y = 0.0;
+    }
+  };
+  // table of low-rank chunk k0: per K index its U column at the leaf's
+  // first row, its W_b row and W_b's leading dimension (threads 0..31)
+  auto table = [&](int k0, int K2) {
+    if (t < KC) {
+      const int kap = k0 + t;
+      const double *qa = nullptr, *qb = nullptr;
+      int st = 0;
+      if (kap < K2) {
+        int l = 0;
+        while (l + 1 < a.h && kap >= lp[l + 1]) l++;
+        const int kk = kap - lp[l];
+        qa = li[l].F + (long long)kk * li[l].ld;
+        qb = li[l].P + kk;
+        st = li[l].s;
+      }
+      tabA[t] = qa;
+      tabB[t] = qb;
+      tabS[t] = st;
+    }
+  };
+
+  // metadata of block (l, q) holding the leaf's rows at level l (thread
+  // l - 1): its loads go out before the first diagonal chunk's, so waiting
+  // for them does not wait for the chunk
+  const int lvl = t + 1;
+  BlockMeta bm;
+  int qfirst = 0, gsel = 0;
+  long long wo = 0;
+  if (lvl <= a.h) {
+    const int q = c >> (a.h - lvl);
+    gsel = blk_index(lvl, q);
+    bm = a.blk[gsel];
+    qfirst = a.leaf_off[q << (a.h - lvl)];
+    wo = a.wofs[gsel];
+  }
+  loadX(0);
+  loadA(0, ar[0]);
+  if (lvl <= a.h) {
+    LevelInfo x;
+    x.F = a.U + bm.uoff + (lo - qfirst);
+    x.P = a.W + wo * a.R;
+    x.ld = bm.m;
+    x.s = bm.s;
+    x.g = gsel;
+    x.pad = 0;
+    li[lvl - 1] = x;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int l = 0; l < a.h; l++) {
+      lp[l] = run;
+      run += li[l].s;
+    }
+    lp[a.h] = run;
+  }
+  __syncthreads();
+  const int K2 = lp[a.h];
+  table(0, K2);
+
+  // diagonal part: D_c (rows of this wave) X_c, chunk by chunk
+#pragma unroll
+  for (int e = 0; e < XPT; e++) xs[0][slot(e)] = xr[e];
+  if (NCH > 1) loadX(1);
+  __syncthreads();
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {
+    if (ch + 1 < NCH) loadA(ch + 1, ar[(ch + 1) & 1]);
+    if (ch + 1 == NCH) loadLR();  // unconditional: an empty table loads stand-ins
+    compute(ch & 1, ar[ch & 1], 8);
+    if (ch + 1 < NCH) {
+#pragma unroll
+      for (int e = 0; e < XPT; e++) xs[(ch + 1) & 1][slot(e)] = xr[e];
+      if (ch + 2 < NCH) loadX(ch + 2);
+      __syncthreads();
+    }
+  }
+
+  // low-rank part: chunk k0 goes to the buffer the last diagonal chunk did
+  // not use, then alternates (the barrier before each compute also orders
+  // the previous chunk's reads before the next write)
+  for (int k0 = 0, buf = NCH & 1; k0 < K2; k0 += KC, buf ^= 1) {
+    if (k0 > 0) {
+      __syncthreads();
+      table(k0, K2);
+      __syncthreads();
+      loadLR();
+    }
+#pragma unroll
+    for (int e = 0; e < XPT; e++) xs[buf][slot(e)] = lb[e];
+    __syncthreads();
+    compute(buf, lr, (min(KC, K2 - k0) + 3) / 4);
+  }
+#pragma unroll
+  for (int f = 0; f < 2; f++)
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = wr + 16 * f + lk + 4 * v;
+        const int j = n0 + 16 * g + i16;
+        a.Y[(long long)(lo + i) * a.y_rs + (long long)j * a.y_cs] = acc[f][g][v];
+      }
+}
+
+// Two-pass form for leaves of 128 or 256 rows whose low-rank K is at most
+// 32: all of X_c stays in LDS (32 NCH rows x 64 columns, then W in one
+// more 32-row buffer), and the 8 waves sweep the leaf in passes of 128
+// rows (16 rows and all 64 columns per wave), so the stores of one pass
+// drain while the next pass computes.  The D stream runs one chunk ahead
+// across pass boundaries; only the first pass waits on barriers (X_c
+// arriving chunk by chunk).  Same LDS slot layout as k_leaf_apply.
+template <int NCH>
+__global__ __launch_bounds__(512) void k_leaf_apply2(ProdArgs a, int ntiles) {
+  constexpr int NT = 512, KC = LEAF_KC, TN = LEAF_TN, M = KC * NCH;
+  constexpr int NPASS = M / 128, XPAIRS = KC / 2 * TN, XPT = XPAIRS / NT;
+  static_assert(M % 128 == 0 && XPAIRS % NT == 0, "leaf tile shape");
+  __shared__ d2 xs[NCH + 1][XPAIRS];
+  __shared__ LevelInfo li[MAX_LEVELS];
+  __shared__ int lp[MAX_LEVELS + 1];
+  __shared__ const double *tabA[KC];
+  __shared__ const double *tabB[KC];
+  __shared__ int tabS[KC];
+  int c, rt, nt;
+  if (!apply_tile_of(blockIdx.x, a.nleaf, 1, ntiles, c, rt, nt)) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int i16 = lane & 15, lk = lane >> 4;
+  const int lo = a.leaf_off[c];
+  const int n0 = nt * TN;
+  // pass p: this wave's rows are 128 p + 16 w .. + 15
+  const double *pD = a.diag + a.diag_off[c] + 16 * w + i16 + (long long)lk * M;
+  const int xp = t & 15;
+  const double *pX = a.X + lo + 2 * xp + (long long)(n0 + (t >> 4)) * a.x_cs;
+  const long long xes = (long long)(NT / 16) * a.x_cs;
+
+  d4 acc[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++) acc[g] = d4{0.0, 0.0, 0.0, 0.0};
+  double ar[2][8];
+  d2 xr[XPT];
+  double lr[8];
+  d2 lb[XPT];
+  auto loadA = [&](int it, double *dst) {  // iteration it = pass * NCH + chunk
+    const int p = it / NCH, ch = it % NCH;
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) dst[ks] = pD[128 * p + (long long)(KC * ch + 4 * ks) * M];
+  };
+  auto loadX = [&](int ch) {
+#pragma unroll
+    for (int e = 0; e < XPT; e++)
+      xr[e] = *reinterpret_cast<const d2 *>(pX + KC * ch + e * xes);
+  };
+  auto slot = [&](int e) {
+    const int j = (t >> 4) + e * (NT / 16);
+    return j * 16 + (xp ^ (j & 15));
+  };
+  auto compute = [&](int buf, const double *av, int nks) {
+    const double *xb = reinterpret_cast<const double *>(xs[buf]);
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) {
+      if (ks < nks) {
+        const int pp = 2 * ks + (lk >> 1);
+        double b[4];
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+          b[g] = xb[((16 * g + i16) * 16 + (pp ^ i16)) * 2 + (lk & 1)];
+#pragma unroll
+        for (int g = 0; g < 4; g++) acc[g] = mfma_f64(av[ks], b[g], acc[g]);
+      }
+    }
+  };
+  // U rows of pass p for the low-rank chunk (unconditional loads from a
+  // valid stand-in address where the table entry is empty)
+  auto loadLRA = [&](int p) {
+    const double *qa[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) qa[ks] = tabA[4 * ks + lk];
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) lr[ks] = ldg(qa[ks] ? qa[ks] + 128 * p + 16 * w + i16 : a.diag);
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+      if (!qa[ks]) lr[ks] = 0.0;
+  };
+  auto loadLRB = [&]() {
+    const double *q0 = tabB[2 * xp], *q1 = tabB[2 * xp + 1];
+    const long long s0 = tabS[2 * xp], s1 = tabS[2 * xp + 1];
+#pragma unroll
+    for (int e = 0; e < XPT; e++) {
+      const int j = n0 + (t >> 4) + e * (NT / 16);
+      lb[e].x = ldg(q0 ? q0 + j * s0 : a.diag);
+      lb[e].y = ldg(q1 ? q1 + j * s1 : a.diag);
+    }
+#pragma unroll
+    for (int e = 0; e < XPT; e++) {
+      if (!q0) lb[e].x = 0.0;
+      if (!q1) lb[e].y = 0.0;
+    }
+  };
+
+  const int lvl = t + 1;
+  BlockMeta bm;
+  int qfirst = 0, gsel = 0;
+  long long wo = 0;
+  if (lvl <= a.h) {
+    const int q = c >> (a.h - lvl);
+    gsel = blk_index(lvl, q);
+    bm = a.blk[gsel];
+    qfirst = a.leaf_off[q << (a.h - lvl)];
+    wo = a.wofs[gsel];
+  }
+  loadX(0);
+  loadA(0, ar[0]);
+  if (lvl <= a.h) {
+    LevelInfo x;
+    x.F = a.U + bm.uoff + (lo - qfirst);
+    x.P = a.W + wo * a.R;
+    x.ld = bm.m;
+    x.s = bm.s;
+    x.g = gsel;
+    x.pad = 0;
+    li[lvl - 1] = x;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int l = 0; l < a.h; l++) {
+      lp[l] = run;
+      run += li[l].s;
+    }
+    lp[a.h] = run;
+  }
+  __syncthreads();
+  const int K2 = lp[a.h];  // at most 32 (host-checked)
+  if (t < KC) {
+    const double *qa = nullptr, *qb = nullptr;
+    int st = 0;
+    if (t < K2) {
+      int l = 0;
+      while (l + 1 < a.h && t >= lp[l + 1]) l++;
+      const int kk = t - lp[l];
+      qa = li[l].F + (long long)kk * li[l].ld;
+      qb = li[l].P + kk;
+      st = li[l].s;
+    }
+    tabA[t] = qa;
+    tabB[t] = qb;
+    tabS[t] = st;
+  }
+#pragma unroll
+  for (int e = 0; e < XPT; e++) xs[0][slot(e)] = xr[e];
+  if (NCH > 1) loadX(1);
+  __syncthreads();
+
+  const int nks_lr = (K2 + 3) / 4;
+#pragma unroll
+  for (int p = 0; p < NPASS; p++) {
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++) {
+      const int it = p * NCH + ch;
+      if (it + 1 < NPASS * NCH) loadA(it + 1, ar[(it + 1) & 1]);
+      if (ch + 1 == NCH) {
+        loadLRA(p);
+        if (p == 0) loadLRB();
+      }
+      compute(ch, ar[it & 1], 8);
+      if (p == 0 && ch + 1 < NCH) {
+#pragma unroll
+        for (int e = 0; e < XPT; e++) xs[ch + 1][slot(e)] = xr[e];
+        if (ch + 2 < NCH) loadX(ch + 2);
+        __syncthreads();
+      }
+    }
+    if (p == 0) {
+#pragma unroll
+      for (int e = 0; e < XPT; e++) xs[NCH][slot(e)] = lb[e];
+      __syncthreads();
+    }
+    compute(NCH, lr, nks_lr);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = 128 * p + 16 * w + lk + 4 * v;
+        const int j = n0 + 16 * g + i16;
+        a.Y[(long long)(lo + i) * a.y_rs + (long long)j * a.y_cs] = acc[g][v];
+      }
+      acc[g] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  }
+}
+
 // Apply-kernel variants: (WM, FM, FN, KC) -> tile TM = WM*FM*16 rows x
 // TN = (4/WM)*FN*16 columns, K-chunk KC.
 struct ApplyVariant {
@@ -534,9 +948,65 @@ static const ApplyVariant kVariants[] = {
     {128, 32, 32, launch_apply<4, 2, 2, 32>},  // 128 x 32, KC 32
     {128, 64, 16, launch_apply<4, 2, 4, 16>},  // 128 x 64, KC 16, 32 x 64 per wave
 };
-static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+static constexpr int kTileVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+// Variants kTileVariants and kTileVariants + 1 are the leaf-resident
+// kernels (one chunk ring / X_c resident in two passes); on trees they do
+// not accept they run variant 0.
+static constexpr int kNumVariants = kTileVariants + 2;
+
+// Leaf size the leaf-resident kernel takes this product at (0: not eligible).
+static int leaf_apply_size(const hm_dtree *t, const ProdArgs &a, bool nfast) {
+  if (a.trans || nfast || a.R % LEAF_TN != 0 || t->nleaf <= 0) return 0;
+  if ((reinterpret_cast<uintptr_t>(a.X) & 15) != 0 || (a.x_cs & 1) != 0) return 0;
+  const int m = t->leaf_off[1] - t->leaf_off[0];
+  if (m != 128 && m != 256 && m != 512) return 0;
+  for (int c = 1; c < t->nleaf; c++)
+    if (t->leaf_off[c + 1] - t->leaf_off[c] != m) return 0;
+  return m;
+}
+
+template <int NW, int NCH>
+static void launch_leaf(const hm_dtree *t, const ProdArgs &a, hipStream_t s) {
+  const int rtiles = (LEAF_KC * NCH) / (32 * NW);
+  const int ntiles = a.R / LEAF_TN;
+  const int grid = ((t->nleaf + 7) / 8) * 8 * rtiles * ntiles;
+  hipLaunchKernelGGL((k_leaf_apply<NW, NCH>), dim3(grid), dim3(NW * 64), 0, s, a, rtiles,
+                     ntiles);
+}
+
+// Largest low-rank K of any leaf: the sum of its ancestors' block ranks.
+static int max_leaf_rank_sum(const hm_dtree *t) {
+  int best = 0;
+  for (int c = 0; c < t->nleaf; c++) {
+    int k = 0;
+    for (int l = 1; l <= t->height; l++) k += t->blk[blk_index(l, c >> (t->height - l))].s;
+    best = std::max(best, k);
+  }
+  return best;
+}
 
 static void run_apply(hm_dtree *t, const ProdArgs &a, int v, bool nfast, hipStream_t s) {
+  if (v == kTileVariants + 1) {
+    const int m = leaf_apply_size(t, a, nfast);
+    if ((m == 128 || m == 256) && max_leaf_rank_sum(t) <= LEAF_KC) {
+      const int ntiles = a.R / LEAF_TN;
+      const int grid = ((t->nleaf + 7) / 8) * 8 * ntiles;
+      if (m == 128)
+        hipLaunchKernelGGL(k_leaf_apply2<4>, dim3(grid), dim3(512), 0, s, a, ntiles);
+      else
+        hipLaunchKernelGGL(k_leaf_apply2<8>, dim3(grid), dim3(512), 0, s, a, ntiles);
+      return;
+    }
+    v = 0;
+  }
+  if (v == kTileVariants) {
+    switch (leaf_apply_size(t, a, nfast)) {
+      case 128: launch_leaf<4, 4>(t, a, s); return;
+      case 256: launch_leaf<8, 8>(t, a, s); return;
+      case 512: launch_leaf<8, 16>(t, a, s); return;
+      default: v = 0;
+    }
+  }
   const ApplyVariant &av = kVariants[v];
   const int rtiles = (t->max_leaf + av.tm - 1) / av.tm;
   const int ntiles = (a.R + av.tn - 1) / av.tn;

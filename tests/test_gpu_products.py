@@ -108,7 +108,7 @@ def test_products_vs_dense(gpu, monkeypatch, name, leaves, ranks, R, wpath):
         T.detach_and_free(lib, t)
 
 
-@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("variant", range(7))
 def test_every_apply_tile_variant(gpu, monkeypatch, variant):
     """Each tile shape of the apply kernel (forced instead of tuned), on a
     uniform tree (bounds-check-free path) and a ragged one, H X and H^T X,
@@ -136,6 +136,58 @@ def test_every_apply_tile_variant(gpu, monkeypatch, variant):
                 dt.free()
             finally:
                 T.detach_and_free(lib, t)
+
+
+LEAF_CASES = [
+    ("leaf 256 h3 rank 1", [256] * 8, lambda l, j: 1),
+    ("leaf 256 h3 ranks to 24 (two low-rank chunks)", [256] * 8, lambda l, j: 24 - 3 * l + j % 2),
+    ("leaf 128 h2 some rank 0", [128] * 4, lambda l, j: 0 if (l + j) % 3 == 0 else 5 + j),
+    ("leaf 512 h2", [512] * 4, lambda l, j: 7 + l),
+]
+
+
+@pytest.mark.parametrize("variant", [5, 6])
+@pytest.mark.parametrize("case", range(len(LEAF_CASES)))
+def test_leaf_resident_apply(gpu, monkeypatch, case, variant):
+    """The leaf-resident apply kernels (variant 5: one workgroup per leaf,
+    D streamed into registers, X through a two-chunk swizzled LDS ring;
+    variant 6: X_c resident in LDS, two passes of 128 rows, low-rank K at
+    most 32) on uniform trees of leaf 128 / 256 / 512, low-rank K of one and
+    several chunks and blocks of rank 0, R = 64 and 192; then operands they
+    do not take (R = 100, X with an odd leading dimension, H^T X; for
+    variant 6 also leaf 512 and low-rank K above 32), which fall back to
+    variant 0."""
+    from hmat_lib_amd import device as D
+    monkeypatch.setenv("HM_APPLY_VARIANT", str(variant))
+    name, leaves, ranks = LEAF_CASES[case]
+    rng = np.random.default_rng(40 + case)
+    ft = H.random_flat_tree(rng, leaves, ranks)
+    lib = capi.hmat()
+    keep = T.Keep()
+    t = T.fill_tree(lib, ft, keep)
+    try:
+        dense = ft.to_dense()
+        n = ft.n
+        dt = D.DeviceTree.upload(t.contents.root, ft.height)
+        for R in (64, 192, 100):
+            _say(f"case: leaf apply v{variant} {name} R={R}")
+            x = np.asfortranarray(rng.standard_normal((n, R)))
+            dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((n, R))
+            dt.matmat(dx, dy)
+            assert capi.hip().hm_hip_matmat_variant(dt.h, 0, R) == variant
+            assert H.rel_err(dy.numpy(), dense @ x) < TOL
+        _say(f"case: leaf apply {name} odd leading dimension and transpose")
+        R = 64
+        xp = np.asfortranarray(rng.standard_normal((n + 1, R)))
+        dxp = D.DeviceArray.from_numpy(xp)
+        dy = D.DeviceArray((n, R))
+        dt.matmat(dxp, dy, R=R, x_strides=(1, n + 1))
+        assert H.rel_err(dy.numpy(), dense @ xp[:n]) < TOL
+        dt.matmat(dxp, dy, trans=1, R=R, x_strides=(1, n + 1))
+        assert H.rel_err(dy.numpy(), dense.T @ xp[:n]) < TOL
+        dt.free()
+    finally:
+        T.detach_and_free(lib, t)
 
 
 def test_dropin_api_vs_oracle(gpu):
