@@ -37,7 +37,9 @@ PEAK_FP64 = 78.6e12        # MI355X dense FP64 (matrix), flop/s
 # (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by the counts of a
 # 256 MiB device copy (fetch x2.000, write x1.000); profiles/ holds the
 # commands and the per-kernel table.
-MEASURED_TRAFFIC = {("exp1d_64k", "k_mm_apply"): 206557491}
+MEASURED_TRAFFIC = {("exp1d_64k", "k_mm_apply"): 206557491,
+                    ("exp1d_64k", "k_leaf_apply"): 206640230,
+                    ("exp1d_64k", "k_leaf_apply2"): 206708633}
 # Apply kernel launched for each tile variant the autotuner can keep (0-4:
 # k_mm_apply tile shapes; 5, 6: the leaf-resident kernels).
 APPLY_KERNELS = {5: "k_leaf_apply", 6: "k_leaf_apply2"}
