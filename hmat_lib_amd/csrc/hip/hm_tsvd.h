@@ -22,7 +22,8 @@ struct TsvdBlock {
   int need;      // out: singular values of the core >= accept/20 * thr (* sigma_0)
   double thr;
   double err;    // out: residual estimate (Frobenius-type bound)
-  double perr;   // out: residual estimate after one power step (spectral)
+  double perr;   // residual estimate after one power step (spectral; +inf until computed)
+  double terr;   // max_t |(I - Q Q^T) A w_t| over the test columns (input of the decision)
   double sigma0; // out: sigma_0 lower bound, then exact top singular value
   long long yoff, ooff, noff, qoff, boff, moff, soff;
   unsigned long long uid;

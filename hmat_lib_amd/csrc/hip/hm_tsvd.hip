@@ -251,15 +251,16 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
     B.k = cut;
     B.sigma0 = s0;
     kk = cut;
-    // posterior test: B.err = max_t |(I - Q Q^T) A w_t| bounds |(I - Q Q^T) A|_2
-    // by 10 sqrt(2/pi) B.err (Halko et al. Lemma 4.1; loose by up to the
+    // posterior test: B.terr = max_t |(I - Q Q^T) A w_t| bounds |(I - Q Q^T) A|_2
+    // by 10 sqrt(2/pi) B.terr (Halko et al. Lemma 4.1; loose by up to the
     // square root of the tail length when the tail is flat); B.perr = the
     // same residual after one power step, |E E^T z| / |E^T z| ~ sigma_1(E),
-    // taken with a safety factor 2.  Either passing accepts.
+    // taken with a safety factor 2 (+inf until the power step has run: it
+    // runs only for blocks the first test rejects).  Either passing accepts.
     const double scale = accept * B.thr * (B.rule == RULE_RELATIVE ? s0 : 1.0);
     bool ok = true;
     if (B.ntest > 0) {
-      const double est = 10.0 * 0.7978845608028654 * B.err;
+      const double est = 10.0 * 0.7978845608028654 * B.terr;
       const double pest = 2.0 * B.perr;
       const bool tiny = (B.rule == RULE_RELATIVE) && !(s0 > B.thr);
       ok = tiny || est <= scale || pest <= scale;
@@ -642,7 +643,8 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       b.soff = stot; stot += b.ell;
       b.status = 0;
       b.err = 0.0;
-      b.perr = 0.0;
+      b.terr = 0.0;
+      b.perr = INFINITY;
       st.max_sketch = std::max(st.max_sketch, b.ell);
     }
     TsvdBufs bf;
@@ -719,8 +721,8 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         timing_end(t1, st);
       });
       std::vector<QrJob> qy, qb;
-      std::vector<GemmJob> t1, t2, p1, p2, p3, p4;
-      std::vector<ColNorm> cn, pn;
+      std::vector<GemmJob> t1, t2;
+      std::vector<ColNorm> cn;
       std::vector<Copy2D> tr;
       for (int i : order) {
         const TsvdBlock &b = blocks[i];
@@ -736,25 +738,9 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
                         1, 0, 0, 1.0, 0.0});
           t2.push_back({bf.Q + b.qoff, T, Yt, nullptr, b.m, b.ell, b.m, b.m, b.ntest, b.ell,
                         1, 0, 0, -1.0, 1.0});
-          double *err = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
-                                                   offsetof(TsvdBlock, err));
-          cn.push_back({Yt, b.m, b.m, b.ntest, err, nullptr, 0, 0, 0});
-          // one power step on the residual E = (I - Q Q^T) A: W = A^T Z,
-          // U = (I - Q Q^T) A W, perr = max_t |U_t| / |W_t|
-          double *Wt = ar.take<double>((size_t)b.n * b.ntest);
-          double *Ut = ar.take<double>((size_t)b.m * b.ntest);
-          double *T2 = ar.take<double>((size_t)b.ell * b.ntest);
-          p1.push_back({b.A, Yt, Wt, nullptr, b.lda, b.m, b.n, b.n, b.ntest, b.m, 1, 0, 0,
-                        1.0, 0.0});
-          p2.push_back({b.A, Wt, Ut, nullptr, b.lda, b.n, b.m, b.m, b.ntest, b.n, 1, 0, 0,
-                        1.0, 0.0});
-          p3.push_back({bf.Q + b.qoff, Ut, T2, nullptr, b.m, b.m, b.ell, b.ell, b.ntest, b.m,
-                        1, 0, 0, 1.0, 0.0});
-          p4.push_back({bf.Q + b.qoff, T2, Ut, nullptr, b.m, b.ell, b.m, b.m, b.ntest, b.ell,
-                        1, 0, 0, -1.0, 1.0});
-          double *perr = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
-                                                    offsetof(TsvdBlock, perr));
-          pn.push_back({Ut, b.m, b.m, b.ntest, perr, Wt, b.n, b.n, 0});
+          double *terr = reinterpret_cast<double *>(reinterpret_cast<char *>(d_blocks + i) +
+                                                    offsetof(TsvdBlock, terr));
+          cn.push_back({Yt, b.m, b.m, b.ntest, terr, nullptr, 0, 0, 0});
         }
       }
       HM_OK(qr_add(pg, ar, qy));
@@ -763,13 +749,6 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         HM_OK(gemm_add(pg, ar, t2, false, false, "test upd"));
       }
       HM_OK(colnorm_add(pg, cn));
-      HM_OK(gemm_add(pg, ar, p1, true, false, "power At"));
-      HM_OK(gemm_add(pg, ar, p2, false, false, "power A"));
-      for (int rep = 0; rep < 2; rep++) {
-        HM_OK(gemm_add(pg, ar, p3, true, false, "power proj"));
-        HM_OK(gemm_add(pg, ar, p4, false, false, "power upd"));
-      }
-      HM_OK(colnorm_add(pg, pn));
       pg.add("project", [=](hipStream_t st, const unsigned char *) {
         Timed t3 = timing_begin("tsvd_project", st);
         hipLaunchKernelGGL(k_project, dim3(nwp), dim3(256), 0, st, d_blocks, d_wp, bf);
@@ -789,6 +768,62 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
                           hipMemcpyDeviceToHost, s));
     HM_HIP(hipStreamSynchronize(s));
+    // one power step on the residual E = (I - Q Q^T) A, only for the blocks
+    // the plain test rejected below a saturated sketch (two more passes over
+    // their A): W = A^T Z, U = (I - Q Q^T) A W, perr = max_t |U_t| / |W_t|;
+    // then their decision is taken again with both estimates
+    std::vector<int> power;
+    for (int i : order)
+      if (!hb[i].status && blocks[i].ntest > 0 && hb[i].k < blocks[i].ell) power.push_back(i);
+    if (!power.empty()) {
+      std::vector<MemRange> pin = {{bf.Y, (size_t)ytot * 8 + 16},
+                                   {bf.Q, (size_t)qtot * 8 + 16},
+                                   {d_blocks, (size_t)nb * sizeof(TsvdBlock)}};
+      for (int i : power) {
+        const TsvdBlock &b = blocks[i];
+        pin.push_back({b.A, ((size_t)(b.n - 1) * b.lda + b.m) * sizeof(double)});
+      }
+      HM_OK(program_exec(
+          [&](Program &pg, Arena &ar) -> int {
+            std::vector<GemmJob> p1, p2, p3, p4;
+            std::vector<ColNorm> pn;
+            for (int i : power) {
+              const TsvdBlock &b = blocks[i];
+              double *Yt = bf.Y + b.yoff + (long long)b.ell * b.m;
+              double *Wt = ar.take<double>((size_t)b.n * b.ntest);
+              double *Ut = ar.take<double>((size_t)b.m * b.ntest);
+              double *T2 = ar.take<double>((size_t)b.ell * b.ntest);
+              p1.push_back({b.A, Yt, Wt, nullptr, b.lda, b.m, b.n, b.n, b.ntest, b.m, 1, 0, 0,
+                            1.0, 0.0});
+              p2.push_back({b.A, Wt, Ut, nullptr, b.lda, b.n, b.m, b.m, b.ntest, b.n, 1, 0, 0,
+                            1.0, 0.0});
+              p3.push_back({bf.Q + b.qoff, Ut, T2, nullptr, b.m, b.m, b.ell, b.ell, b.ntest,
+                            b.m, 1, 0, 0, 1.0, 0.0});
+              p4.push_back({bf.Q + b.qoff, T2, Ut, nullptr, b.m, b.ell, b.m, b.m, b.ntest,
+                            b.ell, 1, 0, 0, -1.0, 1.0});
+              double *perr = reinterpret_cast<double *>(
+                  reinterpret_cast<char *>(d_blocks + i) + offsetof(TsvdBlock, perr));
+              pn.push_back({Ut, b.m, b.m, b.ntest, perr, Wt, b.n, b.n, 0});
+            }
+            HM_OK(gemm_add(pg, ar, p1, true, false, "power At"));
+            HM_OK(gemm_add(pg, ar, p2, false, false, "power A"));
+            for (int rep = 0; rep < 2; rep++) {
+              HM_OK(gemm_add(pg, ar, p3, true, false, "power proj"));
+              HM_OK(gemm_add(pg, ar, p4, false, false, "power upd"));
+            }
+            return colnorm_add(pg, pn);
+          },
+          pin, s));
+      int *d_power = nullptr;
+      HM_OK(to_device(&d_power, power, s));
+      mem.keep(d_power);
+      hipLaunchKernelGGL(k_jacobi, dim3((unsigned)power.size()), dim3(256), 0, s, d_blocks,
+                         d_power, bf, accept, 1);
+      HM_LAUNCH_CHECK();
+      HM_HIP(hipMemcpyAsync(hb.data(), d_blocks, nb * sizeof(TsvdBlock),
+                            hipMemcpyDeviceToHost, s));
+      HM_HIP(hipStreamSynchronize(s));
+    }
     const bool first_round = (st.rounds == 1);
     std::vector<double> sig16;
     if (first_round && (opt.widen || opt.first_sigma)) {
