@@ -82,6 +82,16 @@ static constexpr int REDUCE_SEG = 16;
 static constexpr int WSEG_ROWS = 4096;
 static constexpr int WTILE = 64;  // rank rows per work item
 
+// One reduction workgroup: out = sum of n consecutive partial blocks of
+// s x R doubles starting at src (units of R doubles, in the chunk-partial
+// buffer P, or in the segment-sum buffer when from_seg), written at dst
+// (units of R doubles, segment-sum buffer or W).
+struct RedItem {
+  long long src, dst;
+  int n, s;
+  int from_seg, pad;
+};
+
 struct ProductPlan {
   int nchunk = 0;
   std::vector<int> ch_leaf, ch_r0, ch_rows;  // per chunk
@@ -92,16 +102,17 @@ struct ProductPlan {
   int s_out_max[2] = {0, 0};
   // two-stage reduction: block g's chunks are cut into segments of
   // REDUCE_SEG chunks; seg_list holds (block, segment) pairs of blocks
-  // with more than one segment, multi_list those blocks, part_ofs[g] the
-  // offset (x s_g x R) of block g's segment sums in the partial buffer.
+  // with more than one segment, part_ofs[g] the offset (x s_g x R) of block
+  // g's segment sums in the partial buffer.  The reduction kernels read
+  // flat per-workgroup items (red_seg: chunk partials -> segment sums,
+  // red_w: segment sums or chunk partials -> W_b), one table per
+  // transpose mode, so each workgroup costs one metadata load.
   std::vector<int2> seg_list;
-  std::vector<int> multi_list;
   std::vector<long long> part_ofs;
-  int2 *d_seg_list = nullptr;
-  int *d_multi_list = nullptr;
-  long long *d_part_ofs = nullptr;
+  std::vector<RedItem> red_seg[2], red_w[2];
+  RedItem *d_red_seg[2] = {nullptr, nullptr};
+  RedItem *d_red_w[2] = {nullptr, nullptr};
   int4 *d_ch = nullptr;                      // (leaf, r0, rows, 0)
-  int *d_leaf_chunk0 = nullptr;
   long long *d_pofs[2] = {nullptr, nullptr};
   long long *d_wofs = nullptr;
   int apply_variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};  // autotuned, by R bucket

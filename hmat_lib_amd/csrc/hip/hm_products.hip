@@ -30,12 +30,8 @@ struct ProdArgs {
   const BlockMeta *blk;
   const double *diag, *U, *V;
   const int4 *ch;
-  const int *leaf_chunk0;
   const long long *pofs;
   const long long *wofs;
-  const int2 *seg_list;
-  const int *multi_list;
-  const long long *part_ofs;
   double *P, *W, *Wp;
   const double *X;
   long long x_rs, x_cs;
@@ -175,49 +171,18 @@ __device__ __forceinline__ void group_sum(const double *base, long long stride, 
   }
 }
 
-// chunk range of the source node of block g (the node its columns span in
-// H X; its rows in H^T X)
-__device__ __forceinline__ void src_chunks(const ProdArgs &a, const BlockMeta &b, int &cb,
-                                           int &ce) {
-  const int span = 1 << (a.h - b.level);
-  const int src = a.trans ? b.j : (b.j ^ 1);
-  cb = a.leaf_chunk0[src * span];
-  ce = a.leaf_chunk0[(src + 1) * span];
-}
-
-// stage 1 (blocks with more than one segment): segment sums
-__global__ __launch_bounds__(256) void k_reduce_seg(ProdArgs a) {
+// One reduction item per workgroup (items precomputed on the host, so the
+// only dependent load is the item itself): stage 1 sums chunk partials into
+// segment sums, stage 2 segment sums (or a block's few chunk partials)
+// into W_b; both in fixed order.
+__global__ __launch_bounds__(256) void k_reduce_items(ProdArgs a, const RedItem *items,
+                                                      int to_w) {
   __shared__ double red[256];
-  const int2 it = a.seg_list[blockIdx.x];
-  const BlockMeta b = a.blk[it.x];
-  int cb, ce;
-  src_chunks(a, b, cb, ce);
-  const int c0 = cb + it.y * REDUCE_SEG;
-  const int nparts = max(0, min(ce, c0 + REDUCE_SEG) - c0);
-  const long long E = (long long)b.s * a.R;
-  const double *base = a.P + a.pofs[(long long)(b.level - 1) * a.nchunk + c0] * a.R;
-  double *out = a.Wp + (a.part_ofs[it.x] + (long long)it.y * b.s) * a.R;
-  group_sum(base, E, nparts, E, out, red);
-}
-
-// stage 2: W_b = sum of the segment sums (or of the chunk partials when the
-// block has a single segment)
-__global__ __launch_bounds__(256) void k_reduce_w(ProdArgs a) {
-  __shared__ double red[256];
-  const int g = blockIdx.x;
-  const BlockMeta b = a.blk[g];
-  if (b.s == 0) return;
-  int cb, ce;
-  src_chunks(a, b, cb, ce);
-  const long long E = (long long)b.s * a.R;
-  const int nch = ce - cb;
-  const int nseg = (nch + REDUCE_SEG - 1) / REDUCE_SEG;
-  if (nseg > 1) {
-    group_sum(a.Wp + a.part_ofs[g] * a.R, E, nseg, E, a.W + a.wofs[g] * a.R, red);
-  } else {
-    const double *base = a.P + a.pofs[(long long)(b.level - 1) * a.nchunk + cb] * a.R;
-    group_sum(base, E, nch, E, a.W + a.wofs[g] * a.R, red);
-  }
+  const RedItem it = items[blockIdx.x];
+  const long long E = (long long)it.s * a.R;
+  const double *base = (it.from_seg ? a.Wp : a.P) + it.src * a.R;
+  double *out = (to_w ? a.W : a.Wp) + it.dst * a.R;
+  group_sum(base, E, it.n, E, out, red);
 }
 
 // ------------------------------------------------------------ MFMA path
@@ -475,12 +440,8 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   a.U = t->d_U;
   a.V = t->d_V;
   a.ch = t->plan.d_ch;
-  a.leaf_chunk0 = t->plan.d_leaf_chunk0;
   a.pofs = t->plan.d_pofs[trans];
   a.wofs = t->plan.d_wofs;
-  a.seg_list = t->plan.d_seg_list;
-  a.multi_list = t->plan.d_multi_list;
-  a.part_ofs = t->plan.d_part_ofs;
   long long pmax = std::max(std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back()),
                             std::max(t->plan.wpofs[0].back(), t->plan.wpofs[1].back()));
   a.P = t->d_scratch;
@@ -1100,11 +1061,15 @@ static int run_block_w(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_t s
 }
 
 static int run_reduce(hm_dtree *t, const ProdArgs &a, hipStream_t s) {
+  const ProductPlan &p = t->plan;
+  const int tr_ = a.trans ? 1 : 0;
   Timed tr = timing_begin("reduce_w", s);
-  if (!t->plan.seg_list.empty())
-    hipLaunchKernelGGL(k_reduce_seg, dim3((unsigned)t->plan.seg_list.size()), dim3(256), 0,
-                       s, a);
-  hipLaunchKernelGGL(k_reduce_w, dim3(t->nblock), dim3(256), 0, s, a);
+  if (!p.red_seg[tr_].empty())
+    hipLaunchKernelGGL(k_reduce_items, dim3((unsigned)p.red_seg[tr_].size()), dim3(256), 0, s,
+                       a, p.d_red_seg[tr_], 0);
+  if (!p.red_w[tr_].empty())
+    hipLaunchKernelGGL(k_reduce_items, dim3((unsigned)p.red_w[tr_].size()), dim3(256), 0, s,
+                       a, p.d_red_w[tr_], 1);
   timing_end(tr, s);
   HM_LAUNCH_CHECK();
   return SUCCESS;

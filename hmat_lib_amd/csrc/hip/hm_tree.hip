@@ -21,10 +21,10 @@ void tree_free(hm_dtree *t) {
   dfree(t->d_blk);
   for (int k = 0; k < 2; k++) dfree(t->plan.d_pofs[k]);
   dfree(t->plan.d_ch);
-  dfree(t->plan.d_seg_list);
-  dfree(t->plan.d_multi_list);
-  dfree(t->plan.d_part_ofs);
-  dfree(t->plan.d_leaf_chunk0);
+  for (int k = 0; k < 2; k++) {
+    dfree(t->plan.d_red_seg[k]);
+    dfree(t->plan.d_red_w[k]);
+  }
   dfree(t->plan.d_wofs);
   for (int k = 0; k < 2; k++) {
     dfree(t->plan.d_witems[k]);
@@ -108,8 +108,8 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
   p.wofs.assign(t->nblock + 1, 0);
   for (int g = 0; g < t->nblock; g++) p.wofs[g + 1] = p.wofs[g] + t->blk[g].s;
   p.seg_list.clear();
-  p.multi_list.clear();
   p.part_ofs.assign(t->nblock + 1, 0);
+  std::vector<int> nseg_of(t->nblock, 0);
   long long pr = 0;
   for (int g = 0; g < t->nblock; g++) {
     const BlockMeta &b = t->blk[g];
@@ -120,12 +120,57 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     const int nseg = (nch + REDUCE_SEG - 1) / REDUCE_SEG;
     p.part_ofs[g] = pr;
     if (nseg > 1 && b.s > 0) {
-      p.multi_list.push_back(g);
+      nseg_of[g] = nseg;
       for (int sgi = 0; sgi < nseg; sgi++) p.seg_list.push_back(make_int2(g, sgi));
       pr += (long long)nseg * b.s;
     }
   }
   p.part_ofs[t->nblock] = pr;
+  for (int tr = 0; tr < 2; tr++) {
+    // source node of block g: its columns' node (H X) or its rows' (H^T X)
+    auto src_chunks = [&](const BlockMeta &b, int &cb, int &ce) {
+      const int span = 1 << (h - b.level);
+      const int src = tr ? b.j : (b.j ^ 1);
+      cb = p.leaf_chunk0[src * span];
+      ce = p.leaf_chunk0[(src + 1) * span];
+    };
+    p.red_seg[tr].clear();
+    p.red_w[tr].clear();
+    for (const int2 &it : p.seg_list) {
+      const BlockMeta &b = t->blk[it.x];
+      int cb, ce;
+      src_chunks(b, cb, ce);
+      const int c0 = cb + it.y * REDUCE_SEG;
+      RedItem r;
+      r.src = c0 < ce ? p.pofs[tr][(size_t)(b.level - 1) * p.nchunk + c0] : 0;
+      r.dst = p.part_ofs[it.x] + (long long)it.y * b.s;
+      r.n = std::max(0, std::min(ce, c0 + REDUCE_SEG) - c0);
+      r.s = b.s;
+      r.from_seg = 0;
+      r.pad = 0;
+      p.red_seg[tr].push_back(r);
+    }
+    for (int g = 0; g < t->nblock; g++) {
+      const BlockMeta &b = t->blk[g];
+      if (b.s <= 0) continue;
+      int cb, ce;
+      src_chunks(b, cb, ce);
+      RedItem r;
+      if (nseg_of[g] > 1) {
+        r.src = p.part_ofs[g];
+        r.n = nseg_of[g];
+        r.from_seg = 1;
+      } else {
+        r.src = ce > cb ? p.pofs[tr][(size_t)(b.level - 1) * p.nchunk + cb] : 0;
+        r.n = ce - cb;  // 0 for an empty source node: W_b = 0
+        r.from_seg = 0;
+      }
+      r.dst = p.wofs[g];
+      r.s = b.s;
+      r.pad = 0;
+      p.red_w[tr].push_back(r);
+    }
+  }
   // block path plan
   for (int tr = 0; tr < 2; tr++) {
     p.witems[tr].clear();
@@ -166,10 +211,10 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
   HM_OK(upload_vec(&t->d_diag_off, t->diag_off, s));
   HM_OK(upload_vec(&t->d_blk, t->blk, s));
   HM_OK(upload_vec(&p.d_ch, ch, s));
-  HM_OK(upload_vec(&p.d_seg_list, p.seg_list, s));
-  HM_OK(upload_vec(&p.d_multi_list, p.multi_list, s));
-  HM_OK(upload_vec(&p.d_part_ofs, p.part_ofs, s));
-  HM_OK(upload_vec(&p.d_leaf_chunk0, p.leaf_chunk0, s));
+  for (int tr = 0; tr < 2; tr++) {
+    HM_OK(upload_vec(&p.d_red_seg[tr], p.red_seg[tr], s));
+    HM_OK(upload_vec(&p.d_red_w[tr], p.red_w[tr], s));
+  }
   for (int tr = 0; tr < 2; tr++) HM_OK(upload_vec(&p.d_pofs[tr], p.pofs[tr], s));
   HM_OK(upload_vec(&p.d_wofs, p.wofs, s));
   for (int tr = 0; tr < 2; tr++) {
