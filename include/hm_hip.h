@@ -108,8 +108,13 @@ HM_HIP_API int hm_hip_matmat(hm_dtree *tree, int trans, int R, const double *d_X
  * keep FP64.  Builds the sliced copies of U and V on the device. */
 HM_HIP_API int hm_hip_tree_set_precision(hm_dtree *tree, int slices, void *stream);
 
-/* Apply-kernel tile variant the autotuner picked for this tree, transpose
- * mode and RHS width (-1 before the first product of that kind). */
+/* Apply-kernel variant the autotuner picked for this tree, transpose mode
+ * and RHS width (-1 before the first product of that kind): 0-4 tile shapes
+ * of the general kernel, 5 and 6 the leaf-resident kernels for uniform
+ * leaves of 128/256/512 rows (H X, X column-major, R a multiple of 64).
+ * HM_APPLY_VARIANT=v in the environment forces variant v (a leaf-resident
+ * variant on a tree it does not accept runs variant 0); HM_TUNE=0 disables
+ * tuning. */
 HM_HIP_API int hm_hip_matmat_variant(const hm_dtree *tree, int trans, int R);
 
 /* ---- compression -------------------------------------------------------
