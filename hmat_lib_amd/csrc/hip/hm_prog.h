@@ -53,12 +53,15 @@ struct Program {
   std::vector<std::string> names;
   std::vector<Tunable> tunables;
   unsigned char *d_blob = nullptr;
+  bool own_blob = true;  // false: d_blob belongs to the caller (not freed here)
   // audit data: address ranges touched by items / buffers they may touch
   bool record = false;
   std::vector<std::pair<MemRange, std::string>> touched;
   std::vector<MemRange> allowed;
 
-  ~Program() { dfree(d_blob); }
+  ~Program() {
+    if (own_blob) dfree(d_blob);
+  }
   template <class T>
   size_t push(const std::vector<T> &v) {
     size_t off = (blob.size() + 63) & ~(size_t)63;

@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <mutex>
+
 #include "hm_tsqr.h"
 
 namespace hm {
@@ -596,6 +598,40 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
   return SUCCESS;
 }
 
+// Buffers reused by consecutive program_exec calls (it synchronizes its
+// stream before returning, so they are idle in between): device scratch,
+// device work-item blob and a pinned host staging copy of the blob.  A call
+// that finds them taken (another host thread) allocates its own.
+namespace {
+struct ExecBuffers {
+  std::mutex mu;
+  bool busy = false;
+  char *arena = nullptr;
+  size_t arena_cap = 0;
+  unsigned char *blob = nullptr;
+  size_t blob_cap = 0;
+  unsigned char *host = nullptr;
+  size_t host_cap = 0;
+};
+ExecBuffers g_exec;
+
+// Programs whose scratch is larger than this allocate and free their own
+// (the kept buffers live until the process ends).
+constexpr size_t kKeepBytes = (size_t)1 << 30;
+
+template <class T>
+int grow_device(T **p, size_t *cap, size_t need) {
+  if (*cap >= need) return SUCCESS;
+  const size_t want = std::max(need, *cap + *cap / 2);
+  dfree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HM_OK(dalloc(p, want));
+  *cap = want;
+  return SUCCESS;
+}
+}  // namespace
+
 int program_exec(const std::function<int(Program &, Arena &)> &build,
                  const std::vector<MemRange> &inputs, hipStream_t s) {
   const auto h0 = std::chrono::steady_clock::now();
@@ -610,17 +646,71 @@ int program_exec(const std::function<int(Program &, Arena &)> &build,
     set_error("launch program audit: %s", msg.c_str());
     return INPUT_ERROR;
   }
+  bool cached = false;
+  if (count.off + 256 <= kKeepBytes && dry.blob.size() + 64 <= kKeepBytes / 4) {
+    std::lock_guard<std::mutex> lock(g_exec.mu);
+    if (!g_exec.busy) g_exec.busy = cached = true;
+  }
+  auto release = [&]() {
+    if (!cached) return;
+    std::lock_guard<std::mutex> lock(g_exec.mu);
+    g_exec.busy = false;
+  };
   Arena ar;
   ar.counting = false;
   ar.cap = count.off + 256;
-  HM_OK(dalloc(&ar.base, ar.cap));
+  int rc = SUCCESS;
+  if (cached) {
+    rc = grow_device(&g_exec.arena, &g_exec.arena_cap, ar.cap);
+    ar.base = g_exec.arena;
+  } else {
+    rc = dalloc(&ar.base, ar.cap);
+  }
+  if (rc != SUCCESS) {
+    release();
+    return rc;
+  }
   Program prog;
-  int rc = build(prog, ar);
+  rc = build(prog, ar);
   if (rc == SUCCESS && ar.off > ar.cap) {
     set_error("launch program: scratch grew between passes");
     rc = INPUT_ERROR;
   }
-  if (rc == SUCCESS) rc = prog.upload(s);
+  if (rc == SUCCESS) {
+    if (cached) {
+      const size_t nb = prog.blob.size() + 64;
+      rc = grow_device(&g_exec.blob, &g_exec.blob_cap, nb);
+      if (rc == SUCCESS && g_exec.host_cap < nb) {
+        if (g_exec.host) (void)hipHostFree(g_exec.host);
+        g_exec.host = nullptr;
+        g_exec.host_cap = 0;
+        const size_t want = std::max(nb, (size_t)1 << 20);
+        if (hipHostMalloc(reinterpret_cast<void **>(&g_exec.host), want) == hipSuccess)
+          g_exec.host_cap = want;
+        else
+          g_exec.host = nullptr;
+      }
+      if (rc == SUCCESS) {
+        prog.d_blob = g_exec.blob;
+        prog.own_blob = false;
+        if (!prog.blob.empty()) {
+          const unsigned char *src = prog.blob.data();
+          if (g_exec.host) {
+            memcpy(g_exec.host, prog.blob.data(), prog.blob.size());
+            src = g_exec.host;
+          }
+          hipError_t e = hipMemcpyAsync(prog.d_blob, src, prog.blob.size(),
+                                        hipMemcpyHostToDevice, s);
+          if (e != hipSuccess) {
+            set_error("launch program upload: %s", hipGetErrorString(e));
+            rc = ALLOCATION_FAILURE;
+          }
+        }
+      }
+    } else {
+      rc = prog.upload(s);
+    }
+  }
   timing_host("program build (host)",
               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0)
                   .count());
@@ -633,7 +723,8 @@ int program_exec(const std::function<int(Program &, Arena &)> &build,
     }
   }
   (void)hipStreamSynchronize(s);
-  dfree(ar.base);
+  if (!cached) dfree(ar.base);
+  release();
   return rc;
 }
 
