@@ -37,7 +37,8 @@ PEAK_FP64 = 78.6e12        # MI355X dense FP64 (matrix), flop/s
 # (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by the counts of a
 # 256 MiB device copy (fetch x2.000, write x1.000); profiles/ holds the
 # commands and the per-kernel table.
-MEASURED_TRAFFIC = {("exp1d_64k", "k_mm_apply"): 206557491,
+MEASURED_TRAFFIC = {("exp1d_64k", "k_mv_apply"): 141390848,
+                    ("exp1d_64k", "k_mm_apply"): 206557491,
                     ("exp1d_64k", "k_leaf_apply"): 206640230,
                     ("exp1d_64k", "k_leaf_apply2"): 206708633}
 # Apply kernel launched for each tile variant the autotuner can keep (0-4:
@@ -478,10 +479,18 @@ def main():
         dt.matvec(dxv, dyv)
     D.synchronize()
     t1 = time.perf_counter()
-    for _ in range(50):
+    for _ in range(200):
         dt.matvec(dxv, dyv)
     D.synchronize()
-    matvec_us = (time.perf_counter() - t1) / 50 * 1e6
+    matvec_us = (time.perf_counter() - t1) / 200 * 1e6
+    D.kernel_timing_reset()
+    D.kernel_timing(True, only="mv_apply")
+    for _ in range(200):
+        dt.matvec(dxv, dyv)
+    D.synchronize()
+    D.kernel_timing(False)
+    mv_tot, mv_cnt = D.kernel_timing_read("mv_apply")
+    mv_avg = mv_tot / max(1, mv_cnt) * 1e-3
 
     # algorithmic work of the dominant kernel (diag GEMM + low-rank apply)
     sum_d = info.diag_elems
@@ -511,6 +520,14 @@ def main():
     roof["algorithmic_flops"] = flops_alg
     roof["hbm_frac"] = bytes_alg / k_avg / PEAK_HBM
     roof["tile_variant"] = variant
+    # matvec: its apply kernel k_mv_apply reads D, the output-side factors
+    # and x once, writes y once: HBM-bound
+    mv_bytes = 8 * (sum_d + sum_ms + 2 * n + int(lib_ranks.sum()))
+    mv_roof = {"bound": "hbm", "kernel": "k_mv_apply", "kernel_avg_us": mv_avg * 1e6,
+               "algorithmic_bytes": mv_bytes, "achieved": mv_bytes / mv_avg / 1e9,
+               "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+               "frac": mv_bytes / mv_avg / PEAK_HBM,
+               "traffic": MEASURED_TRAFFIC.get((w.name, "k_mv_apply"))}
 
     # CPU baseline: reference multiply_hodlr_dense on the same tree
     cpu = None
@@ -594,6 +611,7 @@ def main():
         "rel_err_vs_dense": rel_err,
         "rel_err_vs_reference_same_tree": ref_same_tree_err,
         "matvec_us": matvec_us,
+        "matvec_roofline": mv_roof,
         "additions": extras,
     }
     print(json.dumps(line), flush=True)
