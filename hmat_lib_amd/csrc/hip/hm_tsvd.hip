@@ -297,9 +297,11 @@ static int to_device(T **d, const std::vector<T> &h, hipStream_t s) {
 // are cut into blocks of BJ_W; each round of a sweep pairs the blocks
 // round robin and one workgroup per pair (a, b) (1) forms the Gram matrix
 // G = M_S^T M_S of the <= 64 columns S = a u b on MFMA, (2) diagonalises G
-// in LDS by cyclic two-sided Jacobi (J accumulates the rotations), (3)
-// applies J to the columns S of M and of V.  A pair whose first inner
-// sweep rotates anything marks its matrix as not converged.
+// in LDS by cyclic two-sided Jacobi (J accumulates the rotations); then
+// k_bjac_apply, one workgroup per 64-row tile, (3) applies J to the columns
+// S of M and of V (a tile loop inside the pair's workgroup was its longest
+// phase).  A pair whose first inner sweep rotates anything marks its matrix
+// as not converged.
 static constexpr int BJ_W = 32;
 static constexpr int BJ_SCALAR_MAX = 64;
 // inner cyclic sweeps per pair visit: the outer sweeps revisit every pair
@@ -320,7 +322,8 @@ __global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list,
 }
 
 __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4 *items,
-                                                   TsvdBufs bf, int *flags) {
+                                                   TsvdBufs bf, int *flags, double *jout,
+                                                   int *rotated) {
   // the MFMA staging area lives inside G: the Gram tile is stored to G only
   // after its last K-chunk, and G is dead while J is applied; 67 KB of LDS
   // per workgroup lets two share a CU
@@ -338,8 +341,7 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
   const int cb = it.z * BJ_W, wb = min(BJ_W, nc - cb);
   const int W = wa + wb;
   auto col = [&](int i) -> int { return i < wa ? ca + i : cb + (i - wa); };
-  double *M = bf.M + B.moff;
-  double *Vm = bf.Vj + B.moff;
+  const double *M = bf.M + B.moff;
   const int tid = threadIdx.x;
   // (1) Gram matrix of the pair's columns
   {
@@ -358,6 +360,7 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
   if (tid == 0) rot = 0;
   __syncthreads();
   const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
+  int rot_any = 0;
   // (2) cyclic Jacobi on G (round robin over 64 indices, 32 disjoint pairs)
   for (int sweep = 0; sweep < BJ_INNER; sweep++) {
     __syncthreads();
@@ -419,26 +422,47 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
     const int go = rot;
     __syncthreads();
     if (sweep == 0 && go && tid == 0) flags[it.x] = 1;
+    rot_any |= go;
     if (!go) break;
     if (tid == 0) rot = 0;
   }
   __syncthreads();
-  // (3) M_S <- M_S J and V_S <- V_S J, 64-row tiles, in place
-  for (int side = 0; side < 2; side++) {
-    double *X = side ? Vm : M;
-    for (int r0 = 0; r0 < nc; r0 += 64) {
-      BjTile tile;
-      tile.zero();
-      auto la = [&](int i, int k) -> double {
-        return (r0 + i < nc && k < W) ? X[(r0 + i) + (long long)col(k) * nc] : 0.0;
-      };
-      auto lb = [&](int k, int j) -> double { return J[k][j]; };
-      tile.template run<true, false>(lds, 64, la, lb);
-      tile.store([&](int i, int j, double v) {
-        if (r0 + i < nc && j < W) X[(r0 + i) + (long long)col(j) * nc] = v;
-      });
-    }
-  }
+  // (3) J goes out for k_bjac_apply (one workgroup per 64-row tile of M_S
+  // and of V_S), with whether it differs from the identity
+  double *Jg = jout + (size_t)blockIdx.x * 4096;
+  for (int e = tid; e < 64 * 64; e += 256) Jg[e] = J[e >> 6][e & 63];
+  if (tid == 0) rotated[blockIdx.x] = rot_any;
+}
+
+// X_S <- X_S J for one 64-row tile of M (side 0) or V (side 1) of one pair:
+// in place (the tile's rows are read whole before they are written; pairs
+// of one round touch disjoint columns).
+__global__ __launch_bounds__(256) void k_bjac_apply(TsvdBlock *blocks, const int4 *items,
+                                                    TsvdBufs bf, const double *jin,
+                                                    const int *rotated) {
+  __shared__ double lds[BjTile::LDS_DOUBLES];
+  if (!rotated[blockIdx.x]) return;
+  const int4 it = items[blockIdx.x];
+  const TsvdBlock B = blocks[it.x];
+  const int nc = B.ell;
+  const int r0 = blockIdx.y * 64;
+  if (r0 >= nc) return;
+  const int ca = it.y * BJ_W, wa = min(BJ_W, nc - ca);
+  const int cb = it.z * BJ_W, wb = min(BJ_W, nc - cb);
+  const int W = wa + wb;
+  auto col = [&](int i) -> int { return i < wa ? ca + i : cb + (i - wa); };
+  double *X = (blockIdx.z ? bf.Vj : bf.M) + B.moff;
+  const double *Jg = jin + (size_t)blockIdx.x * 4096;
+  BjTile tile;
+  tile.zero();
+  auto la = [&](int i, int k) -> double {
+    return (r0 + i < nc && k < W) ? X[(r0 + i) + (long long)col(k) * nc] : 0.0;
+  };
+  auto lb = [&](int k, int j) -> double { return Jg[k * 64 + j]; };
+  tile.template run<true, false>(lds, 64, la, lb);
+  tile.store([&](int i, int j, double v) {
+    if (r0 + i < nc && j < W) X[(r0 + i) + (long long)col(j) * nc] = v;
+  });
 }
 
 // SVD cores of the listed blocks (M = the core, nc = ell): scalar Jacobi for
@@ -483,15 +507,23 @@ static int svd_cores(TsvdBlock *d_blocks, const std::vector<TsvdBlock> &blocks,
     }
     roff[maxr] = all.size();
     int4 *d_items = nullptr;
-    int *d_big = nullptr, *d_flags = nullptr;
+    int *d_big = nullptr, *d_flags = nullptr, *d_rot = nullptr;
+    double *d_j = nullptr;
+    size_t maxn = 1;
+    int maxnc = 1;
+    for (int r = 0; r < maxr; r++) maxn = std::max(maxn, roff[r + 1] - roff[r]);
+    for (int i : big) maxnc = std::max(maxnc, blocks[i].ell);
     rc = to_device(&d_items, all, s);
     if (rc == SUCCESS) { mem.push_back(d_items); rc = to_device(&d_big, big, s); }
     if (rc == SUCCESS) { mem.push_back(d_big); rc = dalloc(&d_flags, blocks.size()); }
-    if (rc == SUCCESS) mem.push_back(d_flags);
+    if (rc == SUCCESS) { mem.push_back(d_flags); rc = dalloc(&d_rot, maxn); }
+    if (rc == SUCCESS) { mem.push_back(d_rot); rc = dalloc(&d_j, maxn * 4096); }
+    if (rc == SUCCESS) mem.push_back(d_j);
     if (rc != SUCCESS) {
       cleanup();
       return rc;
     }
+    const unsigned row_tiles = (unsigned)((maxnc + 63) / 64);
     Timed tb = timing_begin("block jacobi", s);
     hipLaunchKernelGGL(k_eye, dim3((unsigned)big.size(), 64), dim3(256), 0, s, d_blocks, d_big, bf);
     std::vector<int> hflags(blocks.size());
@@ -499,9 +531,12 @@ static int svd_cores(TsvdBlock *d_blocks, const std::vector<TsvdBlock> &blocks,
       if (hipMemsetAsync(d_flags, 0, blocks.size() * sizeof(int), s) != hipSuccess) break;
       for (int r = 0; r < maxr; r++) {
         const unsigned n = (unsigned)(roff[r + 1] - roff[r]);
-        if (n)
+        if (n) {
           hipLaunchKernelGGL(k_bjac_pair, dim3(n), dim3(256), 0, s, d_blocks, d_items + roff[r],
-                             bf, d_flags);
+                             bf, d_flags, d_j, d_rot);
+          hipLaunchKernelGGL(k_bjac_apply, dim3(n, row_tiles, 2), dim3(256), 0, s, d_blocks,
+                             d_items + roff[r], bf, d_j, d_rot);
+        }
       }
       hipError_t e = hipMemcpyAsync(hflags.data(), d_flags, blocks.size() * sizeof(int),
                                     hipMemcpyDeviceToHost, s);
