@@ -306,10 +306,12 @@ static constexpr int BJ_W = 32;
 static constexpr int BJ_SCALAR_MAX = 64;
 // inner cyclic sweeps per pair visit: the outer sweeps revisit every pair
 // until none rotates, so a partly diagonalised pair only costs another
-// visit, while each inner sweep costs 189 barriers (one sweep measured
+// visit, while each inner sweep costs 126 barriers (one sweep measured
 // faster than two or four)
 static constexpr int BJ_INNER = 1;
-typedef MfmaTile<2, 2, 2, 2, 16> BjTile;  // 64 x 64
+typedef MfmaTile<2, 2, 2, 2, 16> BjTile;      // 64 x 64
+typedef MfmaTile<2, 2, 2, 2, 32> BjGramTile;  // 64 x 64, K-chunks of 32 (half the chunks)
+typedef MfmaTile<2, 2, 2, 2, 64> BjApplyTile; // 64 x 64, K = 64 in one chunk
 
 __global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list, TsvdBufs bf) {
   const TsvdBlock B = blocks[list[blockIdx.x]];
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
   // per workgroup lets two share a CU
   __shared__ double G[64][65];
   __shared__ double J[64][65];
-  static_assert(BjTile::LDS_DOUBLES <= 64 * 65, "staging fits in G");
+  static_assert(BjGramTile::LDS_DOUBLES <= 64 * 65, "staging fits in G");
   double *lds = &G[0][0];
   __shared__ double cs[32][2];
   __shared__ int pq[32][2];
@@ -345,7 +347,7 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
   const int tid = threadIdx.x;
   // (1) Gram matrix of the pair's columns
   {
-    BjTile tile;
+    BjGramTile tile;
     tile.zero();
     auto la = [&](int i, int k) -> double {
       return (i < W && k < nc) ? M[k + (long long)col(i) * nc] : 0.0;
@@ -388,27 +390,30 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
         pq[tid][1] = q;
       }
       __syncthreads();
-      for (int e = tid; e < 32 * 64; e += 256) {  // rows p, q of G
-        const int t = e >> 6, k = e & 63;
-        const double sn = cs[t][1];
-        if (sn != 0.0) {
-          const double c = cs[t][0];
-          const int p = pq[t][0], q = pq[t][1];
-          const double gp = G[p][k], gq = G[q][k];
-          G[p][k] = c * gp - sn * gq;
-          G[q][k] = sn * gp + c * gq;
+      // G <- R^T G R in one pass: thread owns whole 2 x 2 blocks (rows
+      // p_i, q_i of pair i x columns p_j, q_j of pair j), so the update is
+      // in place with no barrier between the row and the column rotation
+      for (int e = tid; e < 32 * 32; e += 256) {
+        const int ti = e >> 5, tj = e & 31;
+        const double si = cs[ti][1], sj = cs[tj][1];
+        if (si != 0.0 || sj != 0.0) {
+          const double ci = cs[ti][0], cj = cs[tj][0];
+          const int pi = pq[ti][0], qi = pq[ti][1], pj = pq[tj][0], qj = pq[tj][1];
+          const double b00 = G[pi][pj], b01 = G[pi][qj], b10 = G[qi][pj], b11 = G[qi][qj];
+          const double r00 = ci * b00 - si * b10, r01 = ci * b01 - si * b11;
+          const double r10 = si * b00 + ci * b10, r11 = si * b01 + ci * b11;
+          G[pi][pj] = cj * r00 - sj * r01;
+          G[pi][qj] = sj * r00 + cj * r01;
+          G[qi][pj] = cj * r10 - sj * r11;
+          G[qi][qj] = sj * r10 + cj * r11;
         }
       }
-      __syncthreads();
-      for (int e = tid; e < 32 * 64; e += 256) {  // columns p, q of G and J
+      for (int e = tid; e < 32 * 64; e += 256) {  // columns p, q of J
         const int t = e >> 6, k = e & 63;
         const double sn = cs[t][1];
         if (sn != 0.0) {
           const double c = cs[t][0];
           const int p = pq[t][0], q = pq[t][1];
-          const double gp = G[k][p], gq = G[k][q];
-          G[k][p] = c * gp - sn * gq;
-          G[k][q] = sn * gp + c * gq;
           const double jp = J[k][p], jq = J[k][q];
           J[k][p] = c * jp - sn * jq;
           J[k][q] = sn * jp + c * jq;
@@ -440,7 +445,7 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
 __global__ __launch_bounds__(256) void k_bjac_apply(TsvdBlock *blocks, const int4 *items,
                                                     TsvdBufs bf, const double *jin,
                                                     const int *rotated) {
-  __shared__ double lds[BjTile::LDS_DOUBLES];
+  __shared__ double lds[BjApplyTile::LDS_DOUBLES];
   if (!rotated[blockIdx.x]) return;
   const int4 it = items[blockIdx.x];
   const TsvdBlock B = blocks[it.x];
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(256) void k_bjac_apply(TsvdBlock *blocks, const int
   auto col = [&](int i) -> int { return i < wa ? ca + i : cb + (i - wa); };
   double *X = (blockIdx.z ? bf.Vj : bf.M) + B.moff;
   const double *Jg = jin + (size_t)blockIdx.x * 4096;
-  BjTile tile;
+  BjApplyTile tile;
   tile.zero();
   auto la = [&](int i, int k) -> double {
     return (r0 + i < nc && k < W) ? X[(r0 + i) + (long long)col(k) * nc] : 0.0;
