@@ -600,12 +600,14 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
 
 // Buffers reused by consecutive program_exec calls (it synchronizes its
 // stream before returning, so they are idle in between): device scratch,
-// device work-item blob and a pinned host staging copy of the blob.  A call
-// that finds them taken (another host thread) allocates its own.
+// device work-item blob and a pinned host staging copy of the blob, all on
+// the first device that used them.  A call that finds them taken (another
+// host thread) or runs on another device allocates its own.
 namespace {
 struct ExecBuffers {
   std::mutex mu;
   bool busy = false;
+  int device = -1;  // device the buffers live on
   char *arena = nullptr;
   size_t arena_cap = 0;
   unsigned char *blob = nullptr;
@@ -647,9 +649,14 @@ int program_exec(const std::function<int(Program &, Arena &)> &build,
     return INPUT_ERROR;
   }
   bool cached = false;
-  if (count.off + 256 <= kKeepBytes && dry.blob.size() + 64 <= kKeepBytes / 4) {
+  int dev = -1;
+  if (count.off + 256 <= kKeepBytes && dry.blob.size() + 64 <= kKeepBytes / 4 &&
+      hipGetDevice(&dev) == hipSuccess) {
     std::lock_guard<std::mutex> lock(g_exec.mu);
-    if (!g_exec.busy) g_exec.busy = cached = true;
+    if (!g_exec.busy && (g_exec.device < 0 || g_exec.device == dev)) {
+      g_exec.busy = cached = true;
+      g_exec.device = dev;
+    }
   }
   auto release = [&]() {
     if (!cached) return;
