@@ -199,6 +199,7 @@ struct MfmaTile {
   static constexpr int LDA = TM + 1;
   static constexpr int LDB = TN + 1;
   static constexpr int LDS_DOUBLES = KC * LDA + KC * LDB;
+  static constexpr int KC_ = KC;
   static constexpr int AREG = (TM * KC + NT - 1) / NT;
   static constexpr int BREG = (TN * KC + NT - 1) / NT;
   // every thread owns whole elements (threadIdx.x < NT by the launch)
@@ -370,6 +371,91 @@ struct MfmaTile {
     if (c + 2 < nch) {
       __syncthreads();
       stash_regs<true, B_NFAST>(lds, ra, rb);
+      __syncthreads();
+      compute(lds);
+    }
+    __syncthreads();
+  }
+
+  // Strided interior form: A(i, k) at A[i * ars + k * acs], B(k, j) at
+  // B[k * brs + j * bcs] (B_NFAST false: consecutive threads take
+  // consecutive k).  The tile's rows lie inside A and K is a multiple of
+  // KC; B's tile columns j >= bcols are read as zero (their loads are
+  // clamped to column bcols - 1, bcols >= 1).  Two register sets alternate
+  // as in run_interior.
+  template <bool A_MFAST>
+  __device__ void run_strided(double *lds, int K, const double *A, long long ars,
+                              long long acs, const double *B, long long brs, long long bcs,
+                              int bcols) {
+    static_assert(NT % TM == 0 && NT % KC == 0, "tile shape");
+    if (K <= 0) return;
+    const int t = threadIdx.x;
+    const double *pa;
+    long long aes;
+    if (A_MFAST) {
+      pa = A + (long long)(t % TM) * ars + (long long)(t / TM) * acs;
+      aes = (long long)(NT / TM) * acs;
+    } else {
+      pa = A + (long long)(t % KC) * acs + (long long)(t / KC) * ars;
+      aes = (long long)(NT / KC) * ars;
+    }
+    const long long astep = (long long)KC * acs, bstep = (long long)KC * brs;
+    const double *pb[BREG];
+    bool bok[BREG];
+#pragma unroll
+    for (int e = 0; e < BREG; e++) {
+      const int idx = t + e * NT;
+      const int k = idx % KC, j = idx / KC;
+      bok[e] = (BEXACT || idx < TN * KC) && j < bcols;
+      pb[e] = B + (long long)k * brs + (long long)min(j, bcols - 1) * bcs;
+    }
+    double ra2[AREG], rb2[BREG];
+    int c0 = 0;
+    auto load = [&](double *xa, double *xb) {
+#pragma unroll
+      for (int e = 0; e < AREG; e++) xa[e] = pa[e * aes];
+#pragma unroll
+      for (int e = 0; e < BREG; e++) {
+        const double v = pb[e][(long long)c0 * bstep];
+        xb[e] = bok[e] ? v : 0.0;
+      }
+      pa += astep;
+      c0++;
+    };
+    const int nch = K / KC;
+    load(ra, rb);
+    if (nch == 1) {
+      pa -= astep;
+      c0--;
+    }
+    load(ra2, rb2);
+    int c = 0;
+    for (; c + 3 < nch; c += 2) {
+      __syncthreads();
+      stash_regs<A_MFAST, false>(lds, ra, rb);
+      __syncthreads();
+      load(ra, rb);
+      compute(lds);
+      __syncthreads();
+      stash_regs<A_MFAST, false>(lds, ra2, rb2);
+      __syncthreads();
+      load(ra2, rb2);
+      compute(lds);
+    }
+    __syncthreads();
+    stash_regs<A_MFAST, false>(lds, ra, rb);
+    __syncthreads();
+    if (c + 2 < nch) load(ra, rb);
+    compute(lds);
+    if (c + 1 < nch) {
+      __syncthreads();
+      stash_regs<A_MFAST, false>(lds, ra2, rb2);
+      __syncthreads();
+      compute(lds);
+    }
+    if (c + 2 < nch) {
+      __syncthreads();
+      stash_regs<A_MFAST, false>(lds, ra, rb);
       __syncthreads();
       compute(lds);
     }

@@ -117,7 +117,11 @@ __global__ __launch_bounds__(256) void k_sketch(TsvdBlock *blocks,
   };
   SketchTile tile;
   tile.zero();
-  tile.run<true, false>(lds, n, la, lb);
+  if (i0 + SketchTile::TM <= m && n % SketchTile::KC_ == 0)
+    tile.run_strided<true>(lds, n, A + i0, 1, B.lda, Om + (long long)j0 * n, 1, n,
+                           min(SketchTile::TN, L - j0));
+  else
+    tile.run<true, false>(lds, n, la, lb);
   double *Y = bf.Y + B.yoff;
   tile.store([&](int i, int j, double v) {
     if (i0 + i < m && j0 + j < L) Y[(i0 + i) + (long long)(j0 + j) * m] = v;
@@ -142,7 +146,11 @@ __global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
   };
   SketchTile tile;
   tile.zero();
-  tile.run<false, false>(lds, m, la, lb);
+  if (i0 + SketchTile::TM <= n && m % SketchTile::KC_ == 0)
+    tile.run_strided<false>(lds, m, A + (long long)i0 * B.lda, B.lda, 1,
+                            Q + (long long)j0 * m, 1, m, min(SketchTile::TN, ell - j0));
+  else
+    tile.run<false, false>(lds, m, la, lb);
   double *Bt = bf.Bt + B.boff;
   tile.store([&](int i, int j, double v) {
     if (i0 + i < n && j0 + j < ell) Bt[(i0 + i) + (long long)(j0 + j) * n] = v;
