@@ -30,6 +30,7 @@ struct ProdArgs {
   const BlockMeta *blk;
   const double *diag, *U, *V;
   const int4 *ch;
+  const PartRow *prow;
   const long long *pofs;
   const long long *wofs;
   double *P, *W, *Wp;
@@ -195,38 +196,21 @@ __global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
   __shared__ const double *rowptr[16];
   __shared__ double *outptr[16];
   __shared__ int outld[16];
-  __shared__ LevelInfo li[MAX_LEVELS];
   int bid = blockIdx.x;
   const int nt = bid % ntiles;
   bid /= ntiles;
   const int mt = bid % mtiles;
   const int k = bid / mtiles;
   const int4 chk = a.ch[k];
-  const int c = chk.x, r0 = chk.y, rows = chk.z;
-  const int lo = a.leaf_off[c] + r0;
-  const int rho0 = mt * 16;
-  load_levels(a, c, k, 0, li);
-  __syncthreads();
+  const int rows = chk.z;
+  const int lo = chk.w;
+  // the chunk's stacked input-factor rows of this M tile (precomputed)
   if (threadIdx.x < 16) {
-    const int rho = rho0 + threadIdx.x;
-    const double *ptr = nullptr;
-    double *optr = nullptr;
-    int old = 1;
-    int base = 0;
-    for (int l = 0; l < a.h; l++) {
-      const LevelInfo x = li[l];
-      if (rho >= base && rho < base + x.s) {
-        const int kk = rho - base;
-        ptr = x.F + r0 + (long long)kk * x.ld;
-        optr = x.P + kk;
-        old = x.s;
-        break;
-      }
-      base += x.s;
-    }
-    rowptr[threadIdx.x] = ptr;
-    outptr[threadIdx.x] = optr;
-    outld[threadIdx.x] = old;
+    const PartRow r = a.prow[((long long)k * mtiles + mt) * 16 + threadIdx.x];
+    const double *F = a.trans ? a.U : a.V;
+    rowptr[threadIdx.x] = r.s ? F + r.f_off : nullptr;
+    outptr[threadIdx.x] = r.s ? a.P + r.p_base * a.R + r.kk : nullptr;
+    outld[threadIdx.x] = r.s ? r.s : 1;
   }
   __syncthreads();
   bool any = false;
@@ -440,6 +424,7 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   a.U = t->d_U;
   a.V = t->d_V;
   a.ch = t->plan.d_ch;
+  a.prow = t->plan.d_prow[trans ? 1 : 0];
   a.pofs = t->plan.d_pofs[trans];
   a.wofs = t->plan.d_wofs;
   long long pmax = std::max(std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back()),

@@ -22,6 +22,7 @@ void tree_free(hm_dtree *t) {
   for (int k = 0; k < 2; k++) dfree(t->plan.d_pofs[k]);
   dfree(t->plan.d_ch);
   for (int k = 0; k < 2; k++) {
+    dfree(t->plan.d_prow[k]);
     dfree(t->plan.d_red_seg[k]);
     dfree(t->plan.d_red_w[k]);
   }
@@ -205,13 +206,41 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
                      [](const int4 &x, const int4 &y) { return x.w > y.w; });
   }
   std::vector<int4> ch(p.nchunk);
-  for (int k = 0; k < p.nchunk; k++) ch[k] = make_int4(p.ch_leaf[k], p.ch_r0[k], p.ch_rows[k], 0);
+  for (int k = 0; k < p.nchunk; k++)
+    ch[k] = make_int4(p.ch_leaf[k], p.ch_r0[k], p.ch_rows[k],
+                      t->leaf_off[p.ch_leaf[k]] + p.ch_r0[k]);
+  for (int tr = 0; tr < 2; tr++) {
+    const int mtiles = (p.s_in_max[tr] + 15) / 16;
+    p.part_mtiles[tr] = mtiles;
+    p.prow[tr].assign((size_t)p.nchunk * mtiles * 16, PartRow{0, 0, 0, 1, 0, 0});
+    for (int k = 0; k < p.nchunk; k++) {
+      const int c = p.ch_leaf[k];
+      int rho = 0;
+      for (int l = 1; l <= h; l++) {
+        const int q = c >> (h - l);
+        // input block of the chunk's rows at level l: (l, q^1) for H X (its
+        // columns are the chunk's node), (l, q) for H^T X (its rows are)
+        const int g = blk_index(l, tr ? q : (q ^ 1));
+        const BlockMeta &b = t->blk[g];
+        const int off = t->leaf_off[c] - t->leaf_off[q << (h - l)] + p.ch_r0[k];
+        for (int kk = 0; kk < b.s; kk++, rho++) {
+          PartRow &r = p.prow[tr][(size_t)k * mtiles * 16 + rho];
+          r.ld = tr ? b.m : b.n;
+          r.f_off = (tr ? b.uoff : b.voff) + off + (long long)kk * r.ld;
+          r.p_base = p.pofs[tr][(size_t)(l - 1) * p.nchunk + k];
+          r.kk = kk;
+          r.s = b.s;
+        }
+      }
+    }
+  }
 
   HM_OK(upload_vec(&t->d_leaf_off, t->leaf_off, s));
   HM_OK(upload_vec(&t->d_diag_off, t->diag_off, s));
   HM_OK(upload_vec(&t->d_blk, t->blk, s));
   HM_OK(upload_vec(&p.d_ch, ch, s));
   for (int tr = 0; tr < 2; tr++) {
+    HM_OK(upload_vec(&p.d_prow[tr], p.prow[tr], s));
     HM_OK(upload_vec(&p.d_red_seg[tr], p.red_seg[tr], s));
     HM_OK(upload_vec(&p.d_red_w[tr], p.red_w[tr], s));
   }
