@@ -82,6 +82,16 @@ static constexpr int REDUCE_SEG = 16;
 static constexpr int WSEG_ROWS = 4096;
 static constexpr int WTILE = 64;  // rank rows per work item
 
+// One stacked input-factor row of a chunk's partial product (k_mm_partials):
+// factor column kk of level l's input block at the chunk's first row
+// (f_off, into V for H X / U for H^T X, leading dimension ld), its output
+// row in the chunk partial (p_base x R + kk, column stride s).  s == 0
+// marks a padding row.
+struct PartRow {
+  long long f_off, p_base;
+  int kk, ld, s, pad;
+};
+
 // One reduction workgroup: out = sum of n consecutive partial blocks of
 // s x R doubles starting at src (units of R doubles, in the chunk-partial
 // buffer P, or in the segment-sum buffer when from_seg), written at dst
@@ -112,7 +122,12 @@ struct ProductPlan {
   std::vector<RedItem> red_seg[2], red_w[2];
   RedItem *d_red_seg[2] = {nullptr, nullptr};
   RedItem *d_red_w[2] = {nullptr, nullptr};
-  int4 *d_ch = nullptr;                      // (leaf, r0, rows, 0)
+  int4 *d_ch = nullptr;                      // (leaf, r0, rows, first global row)
+  // stacked input-factor rows of each chunk, 16 per M tile, per transpose
+  // mode: [(k * part_mtiles[tr] + mt) * 16 + r]
+  int part_mtiles[2] = {0, 0};
+  std::vector<PartRow> prow[2];
+  PartRow *d_prow[2] = {nullptr, nullptr};
   long long *d_pofs[2] = {nullptr, nullptr};
   long long *d_wofs = nullptr;
   int apply_variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};  // autotuned, by R bucket
