@@ -476,9 +476,9 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
   __shared__ d2 xs[2][XPAIRS];
   __shared__ LevelInfo li[MAX_LEVELS];
   __shared__ int lp[MAX_LEVELS + 1];
-  __shared__ const double *tabA[KC];
-  __shared__ const double *tabB[KC];
-  __shared__ int tabS[KC];
+  __shared__ const double *tabA[2][KC];  // two tables: chunk c uses slot c & 1
+  __shared__ const double *tabB[2][KC];
+  __shared__ int tabS[2][KC];
   int c, rt, nt;
   if (!apply_tile_of(blockIdx.x, a.nleaf, rtiles, ntiles, c, rt, nt)) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -499,8 +499,8 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
     for (int g = 0; g < 4; g++) acc[f][g] = d4{0.0, 0.0, 0.0, 0.0};
   double ar[2][16];
   d2 xr[XPT];
-  double lr[16];
-  d2 lb[XPT];
+  double lr[16], lr2[16];  // low-rank operands of chunks 0, 2, 4 ... / 1, 3, ...
+  d2 lb[XPT], lb2[XPT];
   auto loadA = [&](int ch, double *dst) {
 #pragma unroll
     for (int ks = 0; ks < 8; ks++)
@@ -536,12 +536,12 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
   };
   // low-rank chunk operands from the table (every entry first, then every
   // load, unconditional from a valid stand-in address where empty)
-  auto loadLR = [&]() {
+  auto loadLR = [&](int ts, double *lr, d2 *lb) {
     const double *qa[8];
 #pragma unroll
-    for (int ks = 0; ks < 8; ks++) qa[ks] = tabA[4 * ks + lk];
-    const double *q0 = tabB[2 * xp], *q1 = tabB[2 * xp + 1];
-    const long long s0 = tabS[2 * xp], s1 = tabS[2 * xp + 1];
+    for (int ks = 0; ks < 8; ks++) qa[ks] = tabA[ts][4 * ks + lk];
+    const double *q0 = tabB[ts][2 * xp], *q1 = tabB[ts][2 * xp + 1];
+    const long long s0 = tabS[ts][2 * xp], s1 = tabS[ts][2 * xp + 1];
 #pragma unroll
     for (int ks = 0; ks < 8; ks++)
 #pragma unroll
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
   };
   // table of low-rank chunk k0: per K index its U column at the leaf's
   // first row, its W_b row and W_b's leading dimension (threads 0..31)
-  auto table = [&](int k0, int K2) {
+  auto table = [&](int ts, int k0, int K2) {
     if (t < KC) {
       const int kap = k0 + t;
       const double *qa = nullptr, *qb = nullptr;
@@ -579,9 +579,9 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
         qb = li[l].P + kk;
         st = li[l].s;
       }
-      tabA[t] = qa;
-      tabB[t] = qb;
-      tabS[t] = st;
+      tabA[ts][t] = qa;
+      tabB[ts][t] = qb;
+      tabS[ts][t] = st;
     }
   };
 
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
   }
   __syncthreads();
   const int K2 = lp[a.h];
-  table(0, K2);
+  table(0, 0, K2);
 
   // diagonal part: D_c (rows of this wave) X_c, chunk by chunk
 #pragma unroll
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
 #pragma unroll
   for (int ch = 0; ch < NCH; ch++) {
     if (ch + 1 < NCH) loadA(ch + 1, ar[(ch + 1) & 1]);
-    if (ch + 1 == NCH) loadLR();  // unconditional: an empty table loads stand-ins
+    if (ch + 1 == NCH) loadLR(0, lr, lb);  // unconditional: an empty table loads stand-ins
     compute(ch & 1, ar[ch & 1], 8);
     if (ch + 1 < NCH) {
 #pragma unroll
@@ -642,20 +642,26 @@ __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, 
     }
   }
 
-  // low-rank part: chunk k0 goes to the buffer the last diagonal chunk did
-  // not use, then alternates (the barrier before each compute also orders
-  // the previous chunk's reads before the next write)
-  for (int k0 = 0, buf = NCH & 1; k0 < K2; k0 += KC, buf ^= 1) {
-    if (k0 > 0) {
-      __syncthreads();
-      table(k0, K2);
-      __syncthreads();
-      loadLR();
-    }
+  // low-rank part, chunks of 32 alternating between the LDS buffers (chunk
+  // 0 in the one the last diagonal chunk did not use): each step builds the
+  // next chunk's table, stages its own operands, and after the one barrier
+  // issues the next chunk's loads (always: an empty table loads stand-ins)
+  // before computing, so each chunk's loads have a compute phase to land.
+  // A buffer or table slot is rewritten two steps after its last use, with
+  // a barrier in between.
+  auto lr_step = [&](int k0, int buf, double *lrc, d2 *lbc, double *lrn, d2 *lbn) {
+    const int tn = ((k0 / KC) + 1) & 1;
+    table(tn, k0 + KC, K2);
 #pragma unroll
-    for (int e = 0; e < XPT; e++) xs[buf][slot(e)] = lb[e];
+    for (int e = 0; e < XPT; e++) xs[buf][slot(e)] = lbc[e];
     __syncthreads();
-    compute(buf, lr, (min(KC, K2 - k0) + 3) / 4);
+    loadLR(tn, lrn, lbn);
+    compute(buf, lrc, (min(KC, K2 - k0) + 3) / 4);
+  };
+  for (int k0 = 0; k0 < K2; k0 += 2 * KC) {
+    lr_step(k0, NCH & 1, lr, lb, lr2, lb2);
+    if (k0 + KC >= K2) break;
+    lr_step(k0 + KC, (NCH & 1) ^ 1, lr2, lb2, lr, lb);
   }
 #pragma unroll
   for (int f = 0; f < 2; f++)
