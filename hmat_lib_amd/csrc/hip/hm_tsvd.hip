@@ -128,14 +128,20 @@ __global__ __launch_bounds__(256) void k_sketch(TsvdBlock *blocks,
   });
 }
 
-// B^T = A^T Q, tiles of 128 (columns of A) x 32 (ell).
+// B^T = A^T Q, tiles of 128 (columns of A) x 32 (ell); ProjTile16 (128 x
+// 16) for sketches of at most 16 columns, which would leave half of the
+// 32-wide tile's MFMA work on zero columns (the pass is close to MFMA-bound:
+// a 128 x 32 x 16 FP64 step per 16 KB of A).
+typedef MfmaTile<4, 1, 2, 1, 16> ProjTile16;
+
+template <class Tile>
 __global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
                                                  const int4 *work, TsvdBufs bf) {
-  __shared__ double lds[SketchTile::LDS_DOUBLES];
+  __shared__ double lds[Tile::LDS_DOUBLES];
   const int4 w = work[blockIdx.x];
   const TsvdBlock B = blocks[w.x];
   const int m = B.m, n = B.n, ell = B.ell;
-  const int i0 = w.y * SketchTile::TM, j0 = w.z * SketchTile::TN;
+  const int i0 = w.y * Tile::TM, j0 = w.z * Tile::TN;
   const double *A = B.A;
   const double *Q = bf.Q + B.qoff;
   auto la = [&](int i, int k) -> double {
@@ -144,13 +150,13 @@ __global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
   auto lb = [&](int k, int j) -> double {
     return (k < m && j0 + j < ell) ? Q[k + (long long)(j0 + j) * m] : 0.0;
   };
-  SketchTile tile;
+  Tile tile;
   tile.zero();
-  if (i0 + SketchTile::TM <= n && m % SketchTile::KC_ == 0)
-    tile.run_strided<false>(lds, m, A + (long long)i0 * B.lda, B.lda, 1,
-                            Q + (long long)j0 * m, 1, m, min(SketchTile::TN, ell - j0));
+  if (i0 + Tile::TM <= n && m % Tile::KC_ == 0)
+    tile.template run_strided<false>(lds, m, A + (long long)i0 * B.lda, B.lda, 1,
+                                     Q + (long long)j0 * m, 1, m, min(Tile::TN, ell - j0));
   else
-    tile.run<false, false>(lds, m, la, lb);
+    tile.template run<false, false>(lds, m, la, lb);
   double *Bt = bf.Bt + B.boff;
   tile.store([&](int i, int j, double v) {
     if (i0 + i < n && j0 + j < ell) Bt[(i0 + i) + (long long)(j0 + j) * n] = v;
@@ -714,7 +720,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       return (long long)blocks[x].n > (long long)blocks[y].n;
     });
     std::vector<int2> wo;
-    std::vector<int4> ws, wp;
+    std::vector<int4> ws, wp, wp16;
     for (int i : order) {
       const TsvdBlock &b = blocks[i];
       const int L = b.ell + b.ntest;
@@ -730,10 +736,11 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     });
     for (int i : order_m) {
       const TsvdBlock &b = blocks[i];
+      const bool narrow = b.ell <= ProjTile16::TN;
       const int mt = (b.n + SketchTile::TM - 1) / SketchTile::TM;
-      const int ntl = (b.ell + SketchTile::TN - 1) / SketchTile::TN;
+      const int ntl = narrow ? 1 : (b.ell + SketchTile::TN - 1) / SketchTile::TN;
       for (int a = 0; a < mt; a++)
-        for (int c = 0; c < ntl; c++) wp.push_back(make_int4(i, a, c, 0));
+        for (int c = 0; c < ntl; c++) (narrow ? wp16 : wp).push_back(make_int4(i, a, c, 0));
     }
     int2 *d_wo;
     int4 *d_ws, *d_wp;
@@ -741,6 +748,8 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     HM_OK(to_device(&d_wo, wo, s)); mem.keep(d_wo);
     HM_OK(to_device(&d_ws, ws, s)); mem.keep(d_ws);
     HM_OK(to_device(&d_wp, wp, s)); mem.keep(d_wp);
+    int4 *d_wp16;
+    HM_OK(to_device(&d_wp16, wp16, s)); mem.keep(d_wp16);
     HM_OK(to_device(&d_list, order, s)); mem.keep(d_list);
     timing_host("round setup (host)", std::chrono::duration<double, std::milli>(
                                           std::chrono::steady_clock::now() - r0).count());
@@ -757,7 +766,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     }
     const unsigned long long seed = opt.seed;
     const unsigned nwo = (unsigned)wo.size(), nws = (unsigned)ws.size(),
-                   nwp = (unsigned)wp.size();
+                   nwp = (unsigned)wp.size(), nwp16 = (unsigned)wp16.size();
     const double accept = opt.accept;
     auto build = [&](Program &pg, Arena &ar) -> int {
       pg.add("omega+sketch", [=](hipStream_t st, const unsigned char *) {
@@ -799,7 +808,12 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       HM_OK(colnorm_add(pg, cn));
       pg.add("project", [=](hipStream_t st, const unsigned char *) {
         Timed t3 = timing_begin("tsvd_project", st);
-        hipLaunchKernelGGL(k_project, dim3(nwp), dim3(256), 0, st, d_blocks, d_wp, bf);
+        if (nwp)
+          hipLaunchKernelGGL(k_project<SketchTile>, dim3(nwp), dim3(256), 0, st, d_blocks, d_wp,
+                             bf);
+        if (nwp16)
+          hipLaunchKernelGGL(k_project<ProjTile16>, dim3(nwp16), dim3(256), 0, st, d_blocks,
+                             d_wp16, bf);
         timing_end(t3, st);
       });
       HM_OK(qr_add(pg, ar, qb));
