@@ -296,8 +296,43 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
   }
 }
 
+// lds_reflect with the reflector's two values per lane (rows k + lane,
+// k + lane + 64) already in registers.
+__device__ __forceinline__ void lds_reflect_v(double *M, double v0, double v1, int k, int rows,
+                                              double tau, int jf, int jend, int lane) {
+  const int i0 = k + lane, i1 = i0 + 64;
+  const bool h0 = i0 < rows, h1 = i1 < rows;
+  for (int j = jf; j < jend; j += 16) {
+    double d[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int jj = j + 4 * t;
+      double x = 0.0;
+      if (jj < jend) {
+        const double *mj = M + jj * TQ_CR;
+        x = (h0 ? v0 * mj[i0] : 0.0) + (h1 ? v1 * mj[i1] : 0.0);
+      }
+      d[t] = x;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) d[t] = tau * wave_sum(d[t]);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int jj = j + 4 * t;
+      if (jj < jend) {
+        double *mj = M + jj * TQ_CR;
+        if (h0) mj[i0] -= d[t] * v0;
+        if (h1) mj[i1] -= d[t] * v1;
+      }
+    }
+  }
+}
+
+// Q of one chunk: the reflectors are read from the factored panel in
+// global memory (each lane's two values of reflector k - 1 are loaded while
+// reflector k is applied), so LDS holds only the Q slice (64 KB: two
+// workgroups per CU).
 __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
-  __shared__ double V[TQ_CR * TQ_FAST_COLS];
   __shared__ double Qs[TQ_CR * TQ_FAST_COLS];
   __shared__ double tauS[TQ_FAST_COLS];
   const QrItem it = items[blockIdx.x];
@@ -306,10 +341,6 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
   const double *G = it.panel + it.r0;
   const long long ld = it.ld;
   const int nref = min(rows, nc);
-  for (int e = tid; e < rows * nref; e += TQ_NT) {
-    const int i = e % rows, j = e / rows;
-    V[i + j * TQ_CR] = G[i + j * ld];
-  }
   for (int k = tid; k < nref; k += TQ_NT) tauS[k] = it.tau[k];
   for (int e = tid; e < rows * nq; e += TQ_NT) {
     const int i = e % rows, j = e / rows;
@@ -317,11 +348,22 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
     if (i < nc) v = it.cin ? it.cin[it.crow0 + i + (long long)j * it.ldc] : (i == j ? 1.0 : 0.0);
     Qs[i + j * TQ_CR] = v;
   }
+  auto loadv = [&](int k, double &v0, double &v1) {
+    const int i0 = k + lane, i1 = i0 + 64;
+    const double *gk = G + (long long)k * ld;
+    v0 = i0 < rows ? (i0 == k ? 1.0 : gk[i0]) : 0.0;
+    v1 = i1 < rows ? gk[i1] : 0.0;
+  };
+  double c0 = 0.0, c1 = 0.0;
+  if (nref > 0) loadv(nref - 1, c0, c1);
   __syncthreads();
   for (int k = nref - 1; k >= 0; k--) {
+    double n0 = 0.0, n1 = 0.0;
+    if (k > 0) loadv(k - 1, n0, n1);
     const double tau = tauS[k];
-    if (tau == 0.0) continue;
-    lds_reflect(Qs, V + k * TQ_CR, k, rows, tau, wave, nq, lane);
+    if (tau != 0.0) lds_reflect_v(Qs, c0, c1, k, rows, tau, wave, nq, lane);
+    c0 = n0;
+    c1 = n1;
   }
   __syncthreads();
   double *Q = it.qout + it.r0;
