@@ -769,14 +769,6 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
                    nwp = (unsigned)wp.size(), nwp16 = (unsigned)wp16.size();
     const double accept = opt.accept;
     auto build = [&](Program &pg, Arena &ar) -> int {
-      pg.add("omega+sketch", [=](hipStream_t st, const unsigned char *) {
-        Timed t0 = timing_begin("tsvd_omega", st);
-        hipLaunchKernelGGL(k_gen_omega, dim3(nwo), dim3(256), 0, st, d_blocks, d_wo, bf, seed);
-        timing_end(t0, st);
-        Timed t1 = timing_begin("tsvd_sketch", st);
-        hipLaunchKernelGGL(k_sketch, dim3(nws), dim3(256), 0, st, d_blocks, d_ws, bf);
-        timing_end(t1, st);
-      });
       std::vector<QrJob> qy, qb;
       std::vector<GemmJob> t1, t2;
       std::vector<ColNorm> cn;
@@ -820,6 +812,19 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       HM_OK(copy2d_add(pg, tr));
       return SUCCESS;
     };
+    // the sketch needs none of the program's scratch: it is launched first
+    // and runs while the host builds the rest of the round's program (all
+    // of the round's host-to-device table copies are queued above, so
+    // nothing below waits for it before the program's own upload)
+    {
+      Timed t0 = timing_begin("tsvd_omega", s);
+      hipLaunchKernelGGL(k_gen_omega, dim3(nwo), dim3(256), 0, s, d_blocks, d_wo, bf, seed);
+      timing_end(t0, s);
+      Timed t1 = timing_begin("tsvd_sketch", s);
+      hipLaunchKernelGGL(k_sketch, dim3(nws), dim3(256), 0, s, d_blocks, d_ws, bf);
+      timing_end(t1, s);
+      HM_HIP(hipGetLastError());
+    }
     HM_OK(program_exec(build, inputs, s));
     {
       Timed t5 = timing_begin("tsvd_jacobi", s);
