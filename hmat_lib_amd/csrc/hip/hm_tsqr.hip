@@ -191,45 +191,108 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down(const QrItem *items) {
 // Panels of at most TQ_FAST_COLS columns use chunks of at most TQ_CR rows
 // (rows spread evenly, so every chunk of a multi-chunk level has >= ncol
 // rows) held entirely in LDS.  Column j belongs to wave j % 4: the owner of
-// column k forms reflector k, one barrier publishes it, and every wave
-// applies it to its own columns with wavefront reductions.  Building Q needs
-// no barrier per reflector at all (each wave owns its columns of Q).
+// column k forms reflector k as soon as reflector k - 1 has reached that
+// column, one barrier per reflector publishes it, and every wave applies it
+// to its own columns with wavefront reductions.  Building Q needs no
+// barrier per reflector at all (each wave owns its columns of Q).
 static constexpr int TQ_CR = 128;
 static constexpr int TQ_FAST_COLS = 64;
 
-// Applies H = I - tau v v^T (v = [1; vk[k+1..rows)]) to the LDS columns
-// jf, jf + 4, ... < jend of M (column stride TQ_CR, rows <= TQ_CR <= 128):
-// each lane holds rows k + lane and k + lane + 64; four columns per pass
-// so that their wavefront reductions overlap.
-__device__ __forceinline__ void lds_reflect(double *M, const double *vk, int k, int rows,
-                                            double tau, int jf, int jend, int lane) {
+// Global memory accessed through explicitly global pointers: a generic
+// (flat) access shares its completion counter with LDS, so every LDS wait
+// in the loops below would also wait for it.
+typedef __attribute__((address_space(1))) double gdbl;
+__device__ __forceinline__ gdbl *gmem(double *p) { return (gdbl *)p; }
+__device__ __forceinline__ const gdbl *gmem(const double *p) { return (const gdbl *)p; }
+
+// LDS panel (TQ_CR column stride) <- rows x ncol of a global panel: thread
+// t holds row t % 128 of columns t / 128, + 2, ...; eight loads in flight.
+__device__ __forceinline__ void lds_load_panel(double *S, const double *A, long long ld, int rows,
+                                               int ncol) {
+  const int i = threadIdx.x & 127, j0 = threadIdx.x >> 7;
+  if (i >= rows) return;
+  const gdbl *g = gmem(A) + i;
+  for (int j = j0; j < ncol; j += 16) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = g[(long long)min(j + 2 * u, ncol - 1) * ld];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (j + 2 * u < ncol) S[i + (j + 2 * u) * TQ_CR] = v[u];
+  }
+}
+
+// global panel <- LDS panel (the inverse of lds_load_panel)
+__device__ __forceinline__ void lds_store_panel(const double *S, double *A, long long ld, int rows,
+                                                int ncol) {
+  const int i = threadIdx.x & 127, j0 = threadIdx.x >> 7;
+  if (i >= rows) return;
+  gdbl *g = gmem(A) + i;
+  for (int j = j0; j < ncol; j += 2) g[(long long)j * ld] = S[i + j * TQ_CR];
+}
+
+// Applies H = I - tau v v^T to the LDS columns jf, jf + 4, ... < jend of M
+// (column stride TQ_CR, rows <= TQ_CR <= 128).  Each lane holds the
+// reflector's rows k + lane and k + lane + 64 (v0, v1: zero past `rows`,
+// v0 = 1 on lane 0).  NT columns per pass: their LDS reads are issued
+// together (addresses clamped into the array, so no read is conditional:
+// a conditional read gets a wait of its own) and their wavefront
+// reductions overlap; the update reuses the values read.
+template <int NT>
+__device__ __forceinline__ void lds_reflect_v(double *M, double v0, double v1, int k, int rows,
+                                              double tau, int jf, int jend, int lane) {
   const int i0 = k + lane, i1 = i0 + 64;
   const bool h0 = i0 < rows, h1 = i1 < rows;
-  const double v0 = h0 ? (i0 == k ? 1.0 : vk[i0]) : 0.0;
-  const double v1 = h1 ? vk[i1] : 0.0;
-  for (int j = jf; j < jend; j += 16) {
-    double d[4];
+  const int r0 = min(i0, TQ_CR - 1), r1 = min(i1, TQ_CR - 1);
+  for (int j = jf; j < jend; j += 4 * NT) {
+    double a0[NT], a1[NT], d[NT];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const int jj = j + 4 * t;
-      double x = 0.0;
-      if (jj < jend) {
-        const double *mj = M + jj * TQ_CR;
-        x = (h0 ? v0 * mj[i0] : 0.0) + (h1 ? v1 * mj[i1] : 0.0);
-      }
-      d[t] = x;
+    for (int t = 0; t < NT; t++) {
+      const double *mj = M + min(j + 4 * t, TQ_FAST_COLS - 1) * TQ_CR;
+      a0[t] = mj[r0];
+      a1[t] = mj[r1];
     }
 #pragma unroll
-    for (int t = 0; t < 4; t++) d[t] = tau * wave_sum(d[t]);
+    for (int t = 0; t < NT; t++) d[t] = (h0 ? v0 * a0[t] : 0.0) + (h1 ? v1 * a1[t] : 0.0);
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
+    for (int t = 0; t < NT; t++)
+      if (j + 4 * t < jend) d[t] = tau * wave_sum(d[t]);
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
       const int jj = j + 4 * t;
       if (jj < jend) {
         double *mj = M + jj * TQ_CR;
-        if (h0) mj[i0] -= d[t] * v0;
-        if (h1) mj[i1] -= d[t] * v1;
+        if (h0) mj[i0] = a0[t] - d[t] * v0;
+        if (h1) mj[i1] = a1[t] - d[t] * v1;
       }
     }
+  }
+}
+
+// Householder reflector of LDS column k (LAPACK dlarfg convention) formed
+// by one wave: the tail is scaled in place, beta goes to row k, tau to
+// *tau_out.
+__device__ __forceinline__ void lds_make_reflector(double *yk, int k, int rows, int lane,
+                                                   double *tau_out) {
+  const int i0 = k + lane, i1 = i0 + 64;
+  const double a0 = yk[min(i0, TQ_CR - 1)], a1 = yk[min(i1, TQ_CR - 1)];
+  const double x0 = (i0 > k && i0 < rows) ? a0 : 0.0, x1 = i1 < rows ? a1 : 0.0;
+  const double x2 = wave_sum(x0 * x0 + x1 * x1);
+  const double alpha = readlane_f64(a0, 0);
+  double tau = 0.0, beta = alpha, scal = 1.0;
+  if (x2 > 0.0) {
+    const double hh = hypot(alpha, sqrt(x2));
+    beta = (alpha >= 0.0) ? -hh : hh;
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  if (tau != 0.0) {
+    if (i0 > k && i0 < rows) yk[i0] = x0 * scal;
+    if (i1 < rows) yk[i1] = x1 * scal;
+  }
+  if (lane == 0) {
+    yk[k] = beta;
+    *tau_out = tau;
   }
 }
 
@@ -241,49 +304,45 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
   const int lane = tid & 63, wave = tid >> 6;
   double *G = it.panel + it.r0;
   const long long ld = it.ld;
-  for (int e = tid; e < rows * nc; e += TQ_NT) {
-    const int i = e % rows, j = e / rows;
-    Y[i + j * TQ_CR] = G[i + j * ld];
-  }
+  lds_load_panel(Y, G, ld, rows, nc);
   __syncthreads();
+  // Look-ahead: while reflector k is applied, the owner of column k + 1
+  // (wave (k + 1) % 4) updates that column first and forms reflector k + 1
+  // at once, so one barrier per reflector publishes both the updated
+  // columns and the next reflector.  Every column still receives the
+  // reflectors in order with the same arithmetic.
   const int nref = min(rows, nc);
-  for (int k = 0; k < nref; k++) {
-    double *yk = Y + k * TQ_CR;
-    if ((k & 3) == wave) {
-      double x2 = 0.0;
-      for (int i = k + 1 + lane; i < rows; i += 64) x2 += yk[i] * yk[i];
-      x2 = wave_sum(x2);
-      const double alpha = yk[k];
-      double tau = 0.0, beta = alpha, scal = 1.0;
-      if (x2 > 0.0) {
-        const double hh = hypot(alpha, sqrt(x2));
-        beta = (alpha >= 0.0) ? -hh : hh;
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-      if (tau != 0.0)
-        for (int i = k + 1 + lane; i < rows; i += 64) yk[i] *= scal;
-      if (lane == 0) {
-        yk[k] = beta;
-        tauS[k] = tau;
-      }
-    }
-    __syncthreads();
-    const double tau = tauS[k];
-    if (tau == 0.0) continue;
-    lds_reflect(Y, yk, k, rows, tau, k + 1 + ((wave - (k + 1)) & 3), nc, lane);
-  }
+  if (nref > 0 && wave == 0) lds_make_reflector(Y, 0, rows, lane, &tauS[0]);
   __syncthreads();
-  for (int k = tid; k < nc; k += TQ_NT) it.tau[k] = k < nref ? tauS[k] : 0.0;
-  for (int e = tid; e < rows * nc; e += TQ_NT) {
-    const int i = e % rows, j = e / rows;
-    G[i + j * ld] = Y[i + j * TQ_CR];
+  for (int k = 0; k < nref; k++) {
+    const double *yk = Y + k * TQ_CR;
+    const double tau = tauS[k];
+    const int i0 = k + lane, i1 = i0 + 64;
+    const double a0 = yk[min(i0, TQ_CR - 1)], a1 = yk[min(i1, TQ_CR - 1)];
+    const double v0 = i0 < rows ? (i0 == k ? 1.0 : a0) : 0.0;
+    const double v1 = i1 < rows ? a1 : 0.0;
+    int jf = k + 1 + ((wave - (k + 1)) & 3);
+    if (k + 1 < nref && jf == k + 1) {
+      if (tau != 0.0) lds_reflect_v<1>(Y, v0, v1, k, rows, tau, k + 1, k + 2, lane);
+      lds_make_reflector(Y + (k + 1) * TQ_CR, k + 1, rows, lane, &tauS[k + 1]);
+      jf += 4;
+    }
+    if (tau != 0.0) lds_reflect_v<8>(Y, v0, v1, k, rows, tau, jf, nc, lane);
+    __syncthreads();
   }
-  for (int e = tid; e < nc * nc; e += TQ_NT) {
-    const int i = e % nc, j = e / nc;
-    const double r = (i <= j && i < rows) ? Y[i + j * TQ_CR] : 0.0;
-    it.rout[it.rrow0 + i + j * it.ldr] = r;
-    if (it.is_top && it.mt) it.mt[j + i * nc] = r;
+  for (int k = tid; k < nc; k += TQ_NT) gmem(it.tau)[k] = k < nref ? tauS[k] : 0.0;
+  lds_store_panel(Y, G, ld, rows, nc);
+  {
+    // R (upper nc x nc, zero below the diagonal and past `rows`)
+    const int i = tid & 63;
+    gdbl *ro = gmem(it.rout) + it.rrow0 + i;
+    gdbl *mt = (it.is_top && it.mt) ? gmem(it.mt) + (long long)i * nc : nullptr;
+    if (i < nc)
+      for (int j = tid >> 6; j < nc; j += 4) {
+        const double r = (i <= j && i < rows) ? Y[i + j * TQ_CR] : 0.0;
+        ro[(long long)j * it.ldr] = r;
+        if (mt) mt[j] = r;
+      }
   }
   if (it.is_top && it.est && tid == 0) {
     double mx = 0.0;
@@ -292,39 +351,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
       for (int i = it.ell; i <= t && i < rows; i++) ss += Y[i + t * TQ_CR] * Y[i + t * TQ_CR];
       mx = fmax(mx, sqrt(ss));
     }
-    *it.est = mx;
-  }
-}
-
-// lds_reflect with the reflector's two values per lane (rows k + lane,
-// k + lane + 64) already in registers.
-__device__ __forceinline__ void lds_reflect_v(double *M, double v0, double v1, int k, int rows,
-                                              double tau, int jf, int jend, int lane) {
-  const int i0 = k + lane, i1 = i0 + 64;
-  const bool h0 = i0 < rows, h1 = i1 < rows;
-  for (int j = jf; j < jend; j += 16) {
-    double d[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const int jj = j + 4 * t;
-      double x = 0.0;
-      if (jj < jend) {
-        const double *mj = M + jj * TQ_CR;
-        x = (h0 ? v0 * mj[i0] : 0.0) + (h1 ? v1 * mj[i1] : 0.0);
-      }
-      d[t] = x;
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++) d[t] = tau * wave_sum(d[t]);
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const int jj = j + 4 * t;
-      if (jj < jend) {
-        double *mj = M + jj * TQ_CR;
-        if (h0) mj[i0] -= d[t] * v0;
-        if (h1) mj[i1] -= d[t] * v1;
-      }
-    }
+    *gmem(it.est) = mx;
   }
 }
 
@@ -341,36 +368,54 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
   const double *G = it.panel + it.r0;
   const long long ld = it.ld;
   const int nref = min(rows, nc);
-  for (int k = tid; k < nref; k += TQ_NT) tauS[k] = it.tau[k];
-  for (int e = tid; e < rows * nq; e += TQ_NT) {
-    const int i = e % rows, j = e / rows;
-    double v = 0.0;
-    if (i < nc) v = it.cin ? it.cin[it.crow0 + i + (long long)j * it.ldc] : (i == j ? 1.0 : 0.0);
-    Qs[i + j * TQ_CR] = v;
+  for (int k = tid; k < nref; k += TQ_NT) tauS[k] = gmem(it.tau)[k];
+  {
+    // [C; 0]: C = the parent's nc x nq slice, or the identity at the top
+    const int i = tid & 127, j0 = tid >> 7;
+    if (i < rows) {
+      if (i < nc && it.cin) {
+        const gdbl *c = gmem(it.cin) + it.crow0 + i;
+        for (int j = j0; j < nq; j += 16) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = c[(long long)min(j + 2 * u, nq - 1) * it.ldc];
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            if (j + 2 * u < nq) Qs[i + (j + 2 * u) * TQ_CR] = v[u];
+        }
+      } else {
+        for (int j = j0; j < nq; j += 2) Qs[i + j * TQ_CR] = (i < nc && i == j) ? 1.0 : 0.0;
+      }
+    }
   }
+  // reflector k's two values per lane (rows k + lane, k + lane + 64); the
+  // address is clamped into the panel so that the load is unconditional
+  const gdbl *Gg = gmem(G);
   auto loadv = [&](int k, double &v0, double &v1) {
     const int i0 = k + lane, i1 = i0 + 64;
-    const double *gk = G + (long long)k * ld;
-    v0 = i0 < rows ? (i0 == k ? 1.0 : gk[i0]) : 0.0;
-    v1 = i1 < rows ? gk[i1] : 0.0;
+    const gdbl *gk = Gg + (long long)k * ld;
+    const double a0 = gk[min(i0, rows - 1)], a1 = gk[min(i1, rows - 1)];
+    v0 = i0 < rows ? (i0 == k ? 1.0 : a0) : 0.0;
+    v1 = i1 < rows ? a1 : 0.0;
   };
   double c0 = 0.0, c1 = 0.0;
   if (nref > 0) loadv(nref - 1, c0, c1);
   __syncthreads();
   for (int k = nref - 1; k >= 0; k--) {
-    double n0 = 0.0, n1 = 0.0;
-    if (k > 0) loadv(k - 1, n0, n1);
+    double n0, n1;
+    loadv(max(k - 1, 0), n0, n1);
     const double tau = tauS[k];
-    if (tau != 0.0) lds_reflect_v(Qs, c0, c1, k, rows, tau, wave, nq, lane);
+    if (tau != 0.0) {
+      if (nq > 32)
+        lds_reflect_v<16>(Qs, c0, c1, k, rows, tau, wave, nq, lane);
+      else
+        lds_reflect_v<4>(Qs, c0, c1, k, rows, tau, wave, nq, lane);
+    }
     c0 = n0;
     c1 = n1;
   }
   __syncthreads();
-  double *Q = it.qout + it.r0;
-  for (int e = tid; e < rows * nq; e += TQ_NT) {
-    const int i = e % rows, j = e / rows;
-    Q[i + (long long)j * it.ldq] = Qs[i + j * TQ_CR];
-  }
+  lds_store_panel(Qs, it.qout + it.r0, it.ldq, rows, nq);
 }
 
 // chunking of one level: the LDS path spreads rows evenly over
