@@ -188,6 +188,19 @@ int mixed_matmat(hm_dtree *t, int R, const double *X, long ldx, double *Y, long 
 int products_diag(hm_dtree *t, int R, const double *X, long x_cs, double *Y, long y_cs,
                   hipStream_t s);
 
+// ------------------------------------------------- global memory access
+// Kernels reach global memory through pointers read from work-item tables
+// or LDS, which the compiler types as generic: a generic (flat) access
+// counts against the same completion counter as LDS, so every LDS wait
+// after it (each MFMA step reading its staged operands) would also wait
+// for the global load in flight.  Accesses through these helpers are
+// global ones and leave LDS waits alone.
+typedef __attribute__((address_space(1))) double gdbl;
+__device__ __forceinline__ gdbl *gmem(double *p) { return (gdbl *)p; }
+__device__ __forceinline__ const gdbl *gmem(const double *p) { return (const gdbl *)p; }
+__device__ __forceinline__ double ldg(const double *p) { return *gmem(p); }
+__device__ __forceinline__ void stg(double *p, double v) { *gmem(p) = v; }
+
 // ----------------------------------------------------------- MFMA tiles
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -327,15 +340,15 @@ struct MfmaTile {
     static_assert(NT % TM == 0 && NT % KC == 0 && NT % TN == 0, "tile shape");
     if (K <= 0) return;
     const int t = threadIdx.x;
-    const double *pa = A + (t % TM) + (long long)(t / TM) * lda;
+    const gdbl *pa = gmem(A) + (t % TM) + (long long)(t / TM) * lda;
     const long long aes = (long long)(NT / TM) * lda, astep = (long long)KC * lda;
-    const double *pb;
+    const gdbl *pb;
     long long bes;
     if (B_NFAST) {
-      pb = B + (long long)(t % TN) * bcs + (long long)(t / TN) * brs;
+      pb = gmem(B) + (long long)(t % TN) * bcs + (long long)(t / TN) * brs;
       bes = (long long)(NT / TN) * brs;
     } else {
-      pb = B + (long long)(t % KC) * brs + (long long)(t / KC) * bcs;
+      pb = gmem(B) + (long long)(t % KC) * brs + (long long)(t / KC) * bcs;
       bes = (long long)(NT / KC) * bcs;
     }
     const long long bstep = (long long)KC * brs;
@@ -405,24 +418,24 @@ struct MfmaTile {
     static_assert(NT % TM == 0 && NT % KC == 0, "tile shape");
     if (K <= 0) return;
     const int t = threadIdx.x;
-    const double *pa;
+    const gdbl *pa;
     long long aes;
     if (A_MFAST) {
-      pa = A + (long long)(t % TM) * ars + (long long)(t / TM) * acs;
+      pa = gmem(A) + (long long)(t % TM) * ars + (long long)(t / TM) * acs;
       aes = (long long)(NT / TM) * acs;
     } else {
-      pa = A + (long long)(t % KC) * acs + (long long)(t / KC) * ars;
+      pa = gmem(A) + (long long)(t % KC) * acs + (long long)(t / KC) * ars;
       aes = (long long)(NT / KC) * ars;
     }
     const long long astep = (long long)KC * acs, bstep = (long long)KC * brs;
-    const double *pb[BREG];
+    const gdbl *pb[BREG];
     bool bok[BREG];
 #pragma unroll
     for (int e = 0; e < BREG; e++) {
       const int idx = t + e * NT;
       const int k = idx % KC, j = idx / KC;
       bok[e] = (BEXACT || idx < TN * KC) && j < bcols;
-      pb[e] = B + (long long)k * brs + (long long)min(j, bcols - 1) * bcs;
+      pb[e] = gmem(B) + (long long)k * brs + (long long)min(j, bcols - 1) * bcs;
     }
     double ra2[AREG], rb2[BREG];
     int c0 = 0;

@@ -42,6 +42,13 @@ static constexpr int MX_KCH = 128;      // exactness chunk along K
 static constexpr int MX_SEG_CHUNKS = 32;  // W split-K segment (chunks)
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+// global (not generic) loads, so that LDS waits do not wait for them
+__device__ __forceinline__ bf8 ldg8(const __bf16 *p) {
+  return *(const __attribute__((address_space(1))) bf8 *)(p);
+}
+__device__ __forceinline__ int ldgi(const int *p) {
+  return *(const __attribute__((address_space(1))) int *)(p);
+}
 
 __device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -152,12 +159,12 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
     auto fetch = [&](int k0) {
 #pragma unroll
       for (int a = 0; a < S; a++) {
-        ra0[a] = ldA0 ? *reinterpret_cast<const bf8 *>(ga + a * sd.a_slice + k0) : bf8{};
-        rb[a] = ldB ? *reinterpret_cast<const bf8 *>(gb + a * sd.b_slice + k0) : bf8{};
+        ra0[a] = ldA0 ? ldg8(ga + a * sd.a_slice + k0) : bf8{};
+        rb[a] = ldB ? ldg8(gb + a * sd.b_slice + k0) : bf8{};
       }
     };
-    const int kbeg = sd.cs[0], kend = sd.cs[sd.nch];
-    int c = 0, cend = sd.cs[1];
+    const int kbeg = ldgi(sd.cs), kend = ldgi(sd.cs + sd.nch);
+    int c = 0, cend = ldgi(sd.cs + 1);
     f4 acc[S][FM][FN];
 #pragma unroll
     for (int d = 0; d < S; d++)
@@ -209,8 +216,8 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
 #pragma unroll
               for (int d = S - 1; d >= 0; d--) t = t * 0x1p-8 + (double)acc[d][f][g][v];
               if (row < it.M && col < it.N)
-                acc64[f][g][v] += sd.sA[(long long)(sd.a_c0 + c) * sd.ldsa + row] *
-                                  sd.sB[(long long)(sd.b_c0 + c) * sd.ldsb + col] * 0x1p-14 * t;
+                acc64[f][g][v] += ldg(sd.sA + (long long)(sd.a_c0 + c) * sd.ldsa + row) *
+                                  ldg(sd.sB + (long long)(sd.b_c0 + c) * sd.ldsb + col) * 0x1p-14 * t;
             }
 #pragma unroll
         for (int d = 0; d < S; d++)
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
 #pragma unroll
             for (int g = 0; g < FN; g++) acc[d][f][g] = f4{0.f, 0.f, 0.f, 0.f};
         c++;
-        cend = c < sd.nch ? sd.cs[c + 1] : 0;
+        cend = c < sd.nch ? ldgi(sd.cs + c + 1) : 0;
       }
     }
   }
@@ -233,7 +240,7 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
         const int col = it.j0 + wn * 32 + g * 16 + li;
         if (row < it.M && col < it.N) {
           double *cp = it.C + row + (long long)col * it.ldc;
-          *cp = it.mode ? *cp + acc64[f][g][v] : acc64[f][g][v];
+          stg(cp, it.mode ? ldg(cp) + acc64[f][g][v] : acc64[f][g][v]);
         }
       }
 }

@@ -89,9 +89,9 @@ __global__ __launch_bounds__(256) void k_mv_partials(ProdArgs a) {
     const LevelInfo x = li[l];
     for (int kk = 0; kk < x.s; kk++) {
       if (((o + kk) & 3) != wave) continue;
-      double acc = lane < rows ? x.F[r0 + lane + (long long)kk * x.ld] * xi : 0.0;
+      double acc = lane < rows ? ldg(x.F + r0 + lane + (long long)kk * x.ld) * xi : 0.0;
       acc = wave_sum(acc);
-      if (lane == 0) x.P[kk] = acc;
+      if (lane == 0) stg(x.P + kk, acc);
     }
     o += x.s;
   }
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void k_mv_apply(ProdArgs a, int rtiles) {
   __syncthreads();
   for (int l = wave; l < a.h; l += 4) {
     const LevelInfo x = li[l];
-    for (int k = 0; k < x.s; k++) acc += x.F[ii + (long long)k * x.ld] * x.P[k];
+    for (int k = 0; k < x.s; k++) acc += ldg(x.F + ii + (long long)k * x.ld) * ldg(x.P + k);
   }
   red[wave][lane] = acc;
   __syncthreads();
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
   const int n0 = nt * 64;
   auto la = [&](int i, int kx) -> double {
     const double *p = rowptr[i];
-    return (p && kx < rows) ? p[kx] : 0.0;
+    return (p && kx < rows) ? ldg(p + kx) : 0.0;
   };
   auto lb = [&](int kx, int j) -> double {
     return (kx < rows && n0 + j < a.R)
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
   tile.template run<false, B_NFAST>(lds, rows, la, lb);
   tile.store([&](int i, int j, double v) {
     double *o = outptr[i];
-    if (o && n0 + j < a.R) o[(long long)(n0 + j) * outld[i]] = v;
+    if (o && n0 + j < a.R) stg(o + (long long)(n0 + j) * outld[i], v);
   });
 }
 
@@ -389,11 +389,11 @@ __global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles, int nt
     __syncthreads();
     auto la2 = [&](int i, int k) -> double {
       const double *p = tabA[k];
-      return (p && (FAST || r0 + i < mc)) ? p[i] : 0.0;
+      return (p && (FAST || r0 + i < mc)) ? ldg(p + i) : 0.0;
     };
     auto lb2 = [&](int k, int j) -> double {
       const double *p = tabB[k];
-      return (p && (FAST || n0 + j < a.R)) ? p[(long long)(n0 + j) * tabS[k]] : 0.0;
+      return (p && (FAST || n0 + j < a.R)) ? ldg(p + (long long)(n0 + j) * tabS[k]) : 0.0;
     };
     tile.template fetch<true, false>(0, la2, lb2);
     tile.template stash<true, false>(lds);
@@ -460,11 +460,6 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
 // R a multiple of 64.
 static constexpr int LEAF_KC = 32, LEAF_TN = 64;
 typedef double d2 __attribute__((ext_vector_type(2)));
-// Load through a global-address-space pointer (pointers kept in LDS are
-// generic; generic loads also count against the LDS wait counter).
-__device__ __forceinline__ double ldg(const double *p) {
-  return *(const __attribute__((address_space(1))) double *)(p);
-}
 
 template <int NW, int NCH>
 __global__ __launch_bounds__(NW * 64) void k_leaf_apply(ProdArgs a, int rtiles, int ntiles) {

@@ -85,11 +85,11 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, int4 t, double *lds)
   const int M = J.M, N = J.N;
   auto la = [&](int i, int k) -> double {
     if (i0 + i >= M || k >= kl) return 0.0;
-    return TA ? A[k + (long long)(i0 + i) * lda] : A[(i0 + i) + (long long)k * lda];
+    return ldg(A + (TA ? k + (long long)(i0 + i) * lda : (i0 + i) + (long long)k * lda));
   };
   auto lb = [&](int k, int j) -> double {
     if (k >= kl || j0 + j >= N) return 0.0;
-    return TB ? B[(j0 + j) + (long long)k * ldb] : B[k + (long long)(j0 + j) * ldb];
+    return ldg(B + (TB ? (j0 + j) + (long long)k * ldb : k + (long long)(j0 + j) * ldb));
   };
   Tile tile;
   tile.zero();
@@ -101,13 +101,13 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, int4 t, double *lds)
     tile.store([&](int i, int j, double v) {
       if (i0 + i < M && j0 + j < N) {
         double *c = C + (i0 + i) + (long long)(j0 + j) * ldc;
-        *c = beta != 0.0 ? alpha * v + beta * *c : alpha * v;
+        stg(c, beta != 0.0 ? alpha * v + beta * ldg(c) : alpha * v);
       }
     });
   } else {
     double *P = J.part + (long long)t.w * M * N;
     tile.store([&](int i, int j, double v) {
-      if (i0 + i < M && j0 + j < N) P[(i0 + i) + (long long)(j0 + j) * M] = v;
+      if (i0 + i < M && j0 + j < N) stg(P + (i0 + i) + (long long)(j0 + j) * M, v);
     });
   }
   __syncthreads();
@@ -235,14 +235,14 @@ __global__ __launch_bounds__(256) void k_gemm_chain(const GemmChain *chains,
     for (int q = ch.link0; q < ch.link0 + ch.nlink; q++) {
       const GemmLink lk = links[q];
       auto la_n = [&](int i, int k) -> double {
-        return (i0 + i < ch.M && k < lk.K) ? lk.A[(i0 + i) + (long long)k * lk.lda] : 0.0;
+        return (i0 + i < ch.M && k < lk.K) ? ldg(lk.A + (i0 + i) + (long long)k * lk.lda) : 0.0;
       };
       auto la_t = [&](int i, int k) -> double {
-        return (i0 + i < ch.M && k < lk.K) ? lk.A[k + (long long)(i0 + i) * lk.lda] : 0.0;
+        return (i0 + i < ch.M && k < lk.K) ? ldg(lk.A + k + (long long)(i0 + i) * lk.lda) : 0.0;
       };
       if (lk.tb) {
         auto lb = [&](int k, int j) -> double {
-          return (k < lk.K && j0 + j < ch.N) ? lk.B[(j0 + j) + (long long)k * lk.ldb] : 0.0;
+          return (k < lk.K && j0 + j < ch.N) ? ldg(lk.B + (j0 + j) + (long long)k * lk.ldb) : 0.0;
         };
         if (lk.ta)
           tile.template run<false, true>(lds, lk.K, la_t, lb);
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_gemm_chain(const GemmChain *chains,
           tile.template run<true, true>(lds, lk.K, la_n, lb);
       } else {
         auto lb = [&](int k, int j) -> double {
-          return (k < lk.K && j0 + j < ch.N) ? lk.B[k + (long long)(j0 + j) * lk.ldb] : 0.0;
+          return (k < lk.K && j0 + j < ch.N) ? ldg(lk.B + k + (long long)(j0 + j) * lk.ldb) : 0.0;
         };
         if (lk.ta)
           tile.template run<false, false>(lds, lk.K, la_t, lb);
@@ -260,11 +260,11 @@ __global__ __launch_bounds__(256) void k_gemm_chain(const GemmChain *chains,
     }
     if (ch.tc)
       tile.store([&](int i, int j, double v) {
-        if (i0 + i < ch.M && j0 + j < ch.N) ch.C[(j0 + j) + (long long)(i0 + i) * ch.ldc] = v;
+        if (i0 + i < ch.M && j0 + j < ch.N) stg(ch.C + (j0 + j) + (long long)(i0 + i) * ch.ldc, v);
       });
     else
       tile.store([&](int i, int j, double v) {
-        if (i0 + i < ch.M && j0 + j < ch.N) ch.C[(i0 + i) + (long long)(j0 + j) * ch.ldc] = v;
+        if (i0 + i < ch.M && j0 + j < ch.N) stg(ch.C + (i0 + i) + (long long)(j0 + j) * ch.ldc, v);
       });
     __syncthreads();
   }

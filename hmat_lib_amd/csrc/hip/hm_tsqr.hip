@@ -198,13 +198,6 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down(const QrItem *items) {
 static constexpr int TQ_CR = 128;
 static constexpr int TQ_FAST_COLS = 64;
 
-// Global memory accessed through explicitly global pointers: a generic
-// (flat) access shares its completion counter with LDS, so every LDS wait
-// in the loops below would also wait for it.
-typedef __attribute__((address_space(1))) double gdbl;
-__device__ __forceinline__ gdbl *gmem(double *p) { return (gdbl *)p; }
-__device__ __forceinline__ const gdbl *gmem(const double *p) { return (const gdbl *)p; }
-
 // LDS panel (TQ_CR column stride) <- rows x ncol of a global panel: thread
 // t holds row t % 128 of columns t / 128, + 2, ...; eight loads in flight.
 __device__ __forceinline__ void lds_load_panel(double *S, const double *A, long long ld, int rows,

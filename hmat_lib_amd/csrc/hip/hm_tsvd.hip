@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_sketch(TsvdBlock *blocks,
   const double *A = B.A;
   const double *Om = bf.Om + B.ooff;
   auto la = [&](int i, int k) -> double {
-    return (i0 + i < m && k < n) ? A[(long long)(i0 + i) + (long long)k * B.lda] : 0.0;
+    return (i0 + i < m && k < n) ? ldg(A + (long long)(i0 + i) + (long long)k * B.lda) : 0.0;
   };
   auto lb = [&](int k, int j) -> double {
     return (k < n && j0 + j < L) ? Om[k + (long long)(j0 + j) * n] : 0.0;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
   const double *A = B.A;
   const double *Q = bf.Q + B.qoff;
   auto la = [&](int i, int k) -> double {
-    return (i0 + i < n && k < m) ? A[(long long)k + (long long)(i0 + i) * B.lda] : 0.0;
+    return (i0 + i < n && k < m) ? ldg(A + (long long)k + (long long)(i0 + i) * B.lda) : 0.0;
   };
   auto lb = [&](int k, int j) -> double {
     return (k < m && j0 + j < ell) ? Q[k + (long long)(j0 + j) * m] : 0.0;
@@ -620,16 +620,16 @@ __global__ __launch_bounds__(256) void k_outer(const OuterJob *jobs, const int4 
   const OuterJob J = jobs[w.x];
   const int i0 = w.y * DenseTile::TM, j0 = w.z * DenseTile::TN;
   auto la = [&](int i, int k) -> double {
-    return (i0 + i < J.m && k < J.s) ? J.U[(i0 + i) + (long long)k * J.m] : 0.0;
+    return (i0 + i < J.m && k < J.s) ? ldg(J.U + (i0 + i) + (long long)k * J.m) : 0.0;
   };
   auto lb = [&](int k, int j) -> double {
-    return (k < J.s && j0 + j < J.n) ? J.V[(j0 + j) + (long long)k * J.n] : 0.0;
+    return (k < J.s && j0 + j < J.n) ? ldg(J.V + (j0 + j) + (long long)k * J.n) : 0.0;
   };
   DenseTile tile;
   tile.zero();
   tile.run<true, true>(lds, J.s, la, lb);
   tile.store([&](int i, int j, double v) {
-    if (i0 + i < J.m && j0 + j < J.n) J.D[(i0 + i) + (long long)(j0 + j) * J.m] = v;
+    if (i0 + i < J.m && j0 + j < J.n) stg(J.D + (i0 + i) + (long long)(j0 + j) * J.m, v);
   });
 }
 
