@@ -204,6 +204,8 @@ def _declare_hip_api(lib):
         "hm_hip_batch_destroy": ([V], c_int),
         "hm_hip_tree_set_precision": ([V, c_int, V], c_int),
         "hm_hip_kernel_timing_dump": ([C.c_char_p, c_int], c_int),
+        "hm_hip_qr_batch": ([c_int, PInt, PInt, POINTER(V), POINTER(c_long), POINTER(V),
+                             POINTER(c_long), POINTER(V), POINTER(c_long), V], c_int),
         "hm_hip_audit_qr": ([c_int, POINTER(c_int), POINTER(c_int), c_int, C.c_char_p,
                              c_int], c_int),
         "hm_hip_audit_hh": ([c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int), C.c_char_p,

@@ -190,15 +190,20 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down(const QrItem *items) {
 // ------------------------------------------------ LDS-resident chunks
 // Panels of at most TQ_FAST_COLS columns use chunks of at most TQ_CR rows
 // (rows spread evenly, so every chunk of a multi-chunk level has >= ncol
-// rows) held entirely in LDS.  Column j belongs to wave j % 4: the owner of
-// column k forms reflector k as soon as reflector k - 1 has reached that
-// column, one barrier per reflector publishes it, and every wave applies it
-// to its own columns with wavefront reductions.  Building Q needs no
-// barrier per reflector at all (each wave owns its columns of Q).
+// rows).  Column per lane: lane j of wave w keeps rows 32w .. 32w + 31 of
+// column j in registers.  Applying a reflector to every column is then a
+// per-lane dot product over the wave's 32 rows (the reflector read from LDS
+// as a broadcast) and a sum of the four waves' partials through LDS: no
+// cross-lane reductions per column, one or two barriers per reflector.
+// LDS stages the panel so that global reads and writes stay coalesced.
 static constexpr int TQ_CR = 128;
 static constexpr int TQ_FAST_COLS = 64;
+static constexpr int TQ_NW = TQ_NT / 64;     // waves per chunk
+static constexpr int TQ_RW = TQ_CR / TQ_NW;  // rows per wave
+static constexpr int TQ_LS = TQ_CR + 2;      // LDS column stride (16-byte aligned columns)
+static_assert(TQ_NT == 256 && TQ_RW == 32, "chunk layout");
 
-// LDS panel (TQ_CR column stride) <- rows x ncol of a global panel: thread
+// LDS panel (column stride TQ_LS) <- rows x ncol of a global panel: thread
 // t holds row t % 128 of columns t / 128, + 2, ...; eight loads in flight.
 __device__ __forceinline__ void lds_load_panel(double *S, const double *A, long long ld, int rows,
                                                int ncol) {
@@ -211,7 +216,7 @@ __device__ __forceinline__ void lds_load_panel(double *S, const double *A, long 
     for (int u = 0; u < 8; u++) v[u] = g[(long long)min(j + 2 * u, ncol - 1) * ld];
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (j + 2 * u < ncol) S[i + (j + 2 * u) * TQ_CR] = v[u];
+      if (j + 2 * u < ncol) S[i + (j + 2 * u) * TQ_LS] = v[u];
   }
 }
 
@@ -221,110 +226,103 @@ __device__ __forceinline__ void lds_store_panel(const double *S, double *A, long
   const int i = threadIdx.x & 127, j0 = threadIdx.x >> 7;
   if (i >= rows) return;
   gdbl *g = gmem(A) + i;
-  for (int j = j0; j < ncol; j += 2) g[(long long)j * ld] = S[i + j * TQ_CR];
-}
-
-// Applies H = I - tau v v^T to the LDS columns jf, jf + 4, ... < jend of M
-// (column stride TQ_CR, rows <= TQ_CR <= 128).  Each lane holds the
-// reflector's rows k + lane and k + lane + 64 (v0, v1: zero past `rows`,
-// v0 = 1 on lane 0).  NT columns per pass: their LDS reads are issued
-// together (addresses clamped into the array, so no read is conditional:
-// a conditional read gets a wait of its own) and their wavefront
-// reductions overlap; the update reuses the values read.
-template <int NT>
-__device__ __forceinline__ void lds_reflect_v(double *M, double v0, double v1, int k, int rows,
-                                              double tau, int jf, int jend, int lane) {
-  const int i0 = k + lane, i1 = i0 + 64;
-  const bool h0 = i0 < rows, h1 = i1 < rows;
-  const int r0 = min(i0, TQ_CR - 1), r1 = min(i1, TQ_CR - 1);
-  for (int j = jf; j < jend; j += 4 * NT) {
-    double a0[NT], a1[NT], d[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const double *mj = M + min(j + 4 * t, TQ_FAST_COLS - 1) * TQ_CR;
-      a0[t] = mj[r0];
-      a1[t] = mj[r1];
-    }
-#pragma unroll
-    for (int t = 0; t < NT; t++) d[t] = (h0 ? v0 * a0[t] : 0.0) + (h1 ? v1 * a1[t] : 0.0);
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-      if (j + 4 * t < jend) d[t] = tau * wave_sum(d[t]);
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const int jj = j + 4 * t;
-      if (jj < jend) {
-        double *mj = M + jj * TQ_CR;
-        if (h0) mj[i0] = a0[t] - d[t] * v0;
-        if (h1) mj[i1] = a1[t] - d[t] * v1;
-      }
-    }
-  }
-}
-
-// Householder reflector of LDS column k (LAPACK dlarfg convention) formed
-// by one wave: the tail is scaled in place, beta goes to row k, tau to
-// *tau_out.
-__device__ __forceinline__ void lds_make_reflector(double *yk, int k, int rows, int lane,
-                                                   double *tau_out) {
-  const int i0 = k + lane, i1 = i0 + 64;
-  const double a0 = yk[min(i0, TQ_CR - 1)], a1 = yk[min(i1, TQ_CR - 1)];
-  const double x0 = (i0 > k && i0 < rows) ? a0 : 0.0, x1 = i1 < rows ? a1 : 0.0;
-  const double x2 = wave_sum(x0 * x0 + x1 * x1);
-  const double alpha = readlane_f64(a0, 0);
-  double tau = 0.0, beta = alpha, scal = 1.0;
-  if (x2 > 0.0) {
-    const double hh = hypot(alpha, sqrt(x2));
-    beta = (alpha >= 0.0) ? -hh : hh;
-    tau = (beta - alpha) / beta;
-    scal = 1.0 / (alpha - beta);
-  }
-  if (tau != 0.0) {
-    if (i0 > k && i0 < rows) yk[i0] = x0 * scal;
-    if (i1 < rows) yk[i1] = x1 * scal;
-  }
-  if (lane == 0) {
-    yk[k] = beta;
-    *tau_out = tau;
-  }
+  for (int j = j0; j < ncol; j += 2) g[(long long)j * ld] = S[i + j * TQ_LS];
 }
 
 __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
-  __shared__ double Y[TQ_CR * TQ_FAST_COLS];
-  __shared__ double tauS[TQ_FAST_COLS];
+  __shared__ __attribute__((aligned(16))) double S[TQ_LS * TQ_FAST_COLS];
+  __shared__ __attribute__((aligned(16))) double vb[TQ_CR];  // column k below the diagonal
+  __shared__ double part[TQ_NW][64];
+  __shared__ double tauS[TQ_FAST_COLS], scalS[TQ_FAST_COLS], betaS[TQ_FAST_COLS];
+  __shared__ double alphaS;
   const QrItem it = items[blockIdx.x];
   const int rows = it.rows, nc = it.ncol, tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, w = tid >> 6, rw0 = w * TQ_RW;
   double *G = it.panel + it.r0;
   const long long ld = it.ld;
-  lds_load_panel(Y, G, ld, rows, nc);
+  lds_load_panel(S, G, ld, rows, nc);
   __syncthreads();
-  // Look-ahead: while reflector k is applied, the owner of column k + 1
-  // (wave (k + 1) % 4) updates that column first and forms reflector k + 1
-  // at once, so one barrier per reflector publishes both the updated
-  // columns and the next reflector.  Every column still receives the
-  // reflectors in order with the same arithmetic.
-  const int nref = min(rows, nc);
-  if (nref > 0 && wave == 0) lds_make_reflector(Y, 0, rows, lane, &tauS[0]);
-  __syncthreads();
-  for (int k = 0; k < nref; k++) {
-    const double *yk = Y + k * TQ_CR;
-    const double tau = tauS[k];
-    const int i0 = k + lane, i1 = i0 + 64;
-    const double a0 = yk[min(i0, TQ_CR - 1)], a1 = yk[min(i1, TQ_CR - 1)];
-    const double v0 = i0 < rows ? (i0 == k ? 1.0 : a0) : 0.0;
-    const double v1 = i1 < rows ? a1 : 0.0;
-    int jf = k + 1 + ((wave - (k + 1)) & 3);
-    if (k + 1 < nref && jf == k + 1) {
-      if (tau != 0.0) lds_reflect_v<1>(Y, v0, v1, k, rows, tau, k + 1, k + 2, lane);
-      lds_make_reflector(Y + (k + 1) * TQ_CR, k + 1, rows, lane, &tauS[k + 1]);
-      jf += 4;
+  double y[TQ_RW];
+  {
+    const double *sj = S + min(lane, nc - 1) * TQ_LS + rw0;
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) {
+      const double x = sj[r];
+      y[r] = (rw0 + r < rows && lane < nc) ? x : 0.0;
     }
-    if (tau != 0.0) lds_reflect_v<8>(Y, v0, v1, k, rows, tau, jf, nc, lane);
-    __syncthreads();
   }
+  // Reflector k (LAPACK dlarfg convention): column k's rows below the
+  // diagonal x, alpha on it; beta = -sign(alpha) |(alpha, x)|, tau =
+  // (beta - alpha) / beta, v = [1; x / (alpha - beta)]; tau = 0 when x = 0.
+  // Column k itself keeps x and alpha until the end (its scale and beta are
+  // applied when it is written back).
+  const int nref = min(rows, nc);
+  for (int k = 0; k < nref; k++) {
+    const int kw = k / TQ_RW, kr = k % TQ_RW;
+    if (lane == k && w >= kw) {
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) vb[rw0 + r] = y[r];
+      if (w == kw) {
+        alphaS = vb[k];
+        for (int r = 0; r <= kr; r++) vb[rw0 + r] = 0.0;
+      }
+    }
+    __syncthreads();
+    const double a0 = vb[lane], a1 = vb[lane + 64];
+    const double x0 = lane > k ? a0 : 0.0, x1 = lane + 64 > k ? a1 : 0.0;
+    const double x2 = wave_sum(x0 * x0 + x1 * x1);
+    const double alpha = alphaS;
+    double tau = 0.0, beta = alpha, scal = 1.0;
+    if (x2 > 0.0) {
+      const double hh = hypot(alpha, sqrt(x2));
+      beta = (alpha >= 0.0) ? -hh : hh;
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    if (tid == 0) {
+      tauS[k] = tau;
+      scalS[k] = scal;
+      betaS[k] = beta;
+    }
+    // v = scal * vb with the unit entry: vb[k] = alpha - beta = 1 / scal,
+    // written by the wave that reads it (its LDS accesses stay in order;
+    // the other waves' norms never look at row k)
+    if (w == kw && lane == 0) vb[k] = alpha - beta;
+    double v[TQ_RW];
+    double pd = 0.0;
+    if (w >= kw) {
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) {
+        v[r] = vb[rw0 + r];
+        pd += v[r] * y[r];
+      }
+    }
+    part[w][lane] = pd;
+    __syncthreads();
+    if (tau != 0.0 && lane > k && w >= kw) {
+      double dot = part[0][lane];
+#pragma unroll
+      for (int ww = 1; ww < TQ_NW; ww++) dot += part[ww][lane];
+      const double fs = tau * scal * scal * dot;
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) y[r] -= fs * v[r];
+    }
+  }
+  // back to LDS in factored form: R on and above the diagonal, the scaled
+  // reflectors below it
+#pragma unroll
+  for (int r = 0; r < TQ_RW; r++) {
+    const int i = rw0 + r;
+    double x = y[r];
+    if (lane < nref) {
+      if (i > lane) x *= scalS[lane];
+      else if (i == lane) x = betaS[lane];
+    }
+    S[i + lane * TQ_LS] = x;
+  }
+  __syncthreads();
   for (int k = tid; k < nc; k += TQ_NT) gmem(it.tau)[k] = k < nref ? tauS[k] : 0.0;
-  lds_store_panel(Y, G, ld, rows, nc);
+  lds_store_panel(S, G, ld, rows, nc);
   {
     // R (upper nc x nc, zero below the diagonal and past `rows`)
     const int i = tid & 63;
@@ -332,7 +330,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
     gdbl *mt = (it.is_top && it.mt) ? gmem(it.mt) + (long long)i * nc : nullptr;
     if (i < nc)
       for (int j = tid >> 6; j < nc; j += 4) {
-        const double r = (i <= j && i < rows) ? Y[i + j * TQ_CR] : 0.0;
+        const double r = (i <= j && i < rows) ? S[i + j * TQ_LS] : 0.0;
         ro[(long long)j * it.ldr] = r;
         if (mt) mt[j] = r;
       }
@@ -341,74 +339,86 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
     double mx = 0.0;
     for (int t = it.ell; t < nc; t++) {
       double ss = 0.0;
-      for (int i = it.ell; i <= t && i < rows; i++) ss += Y[i + t * TQ_CR] * Y[i + t * TQ_CR];
+      for (int i = it.ell; i <= t && i < rows; i++) ss += S[i + t * TQ_LS] * S[i + t * TQ_LS];
       mx = fmax(mx, sqrt(ss));
     }
     *gmem(it.est) = mx;
   }
 }
 
-// Q of one chunk: the reflectors are read from the factored panel in
-// global memory (each lane's two values of reflector k - 1 are loaded while
-// reflector k is applied), so LDS holds only the Q slice (64 KB: two
-// workgroups per CU).
+// Q of one chunk: [C; 0] (C the parent's ncol x nq slice, the identity at
+// the top) times the chunk's reflectors in reverse order, column per lane.
 __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
-  __shared__ double Qs[TQ_CR * TQ_FAST_COLS];
+  __shared__ __attribute__((aligned(16))) double S[TQ_LS * TQ_FAST_COLS];
+  __shared__ double part[2][TQ_NW][64];
   __shared__ double tauS[TQ_FAST_COLS];
   const QrItem it = items[blockIdx.x];
   const int rows = it.rows, nc = it.ncol, nq = it.nq, tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const double *G = it.panel + it.r0;
-  const long long ld = it.ld;
+  const int lane = tid & 63, w = tid >> 6, rw0 = w * TQ_RW;
   const int nref = min(rows, nc);
-  for (int k = tid; k < nref; k += TQ_NT) tauS[k] = gmem(it.tau)[k];
   {
-    // [C; 0]: C = the parent's nc x nq slice, or the identity at the top
+    // reflectors as LDS columns: unit diagonal, zero above it and past rows
     const int i = tid & 127, j0 = tid >> 7;
-    if (i < rows) {
-      if (i < nc && it.cin) {
-        const gdbl *c = gmem(it.cin) + it.crow0 + i;
-        for (int j = j0; j < nq; j += 16) {
-          double v[8];
+    const gdbl *g = gmem(it.panel + it.r0) + min(i, rows - 1);
+    const long long ld = it.ld;
+    for (int j = j0; j < nref; j += 16) {
+      double x[8];
 #pragma unroll
-          for (int u = 0; u < 8; u++) v[u] = c[(long long)min(j + 2 * u, nq - 1) * it.ldc];
+      for (int u = 0; u < 8; u++) x[u] = g[(long long)min(j + 2 * u, nref - 1) * ld];
 #pragma unroll
-          for (int u = 0; u < 8; u++)
-            if (j + 2 * u < nq) Qs[i + (j + 2 * u) * TQ_CR] = v[u];
-        }
-      } else {
-        for (int j = j0; j < nq; j += 2) Qs[i + j * TQ_CR] = (i < nc && i == j) ? 1.0 : 0.0;
+      for (int u = 0; u < 8; u++) {
+        const int jj = j + 2 * u;
+        if (jj < nref) S[i + jj * TQ_LS] = (i > jj && i < rows) ? x[u] : (i == jj ? 1.0 : 0.0);
       }
     }
   }
-  // reflector k's two values per lane (rows k + lane, k + lane + 64); the
-  // address is clamped into the panel so that the load is unconditional
-  const gdbl *Gg = gmem(G);
-  auto loadv = [&](int k, double &v0, double &v1) {
-    const int i0 = k + lane, i1 = i0 + 64;
-    const gdbl *gk = Gg + (long long)k * ld;
-    const double a0 = gk[min(i0, rows - 1)], a1 = gk[min(i1, rows - 1)];
-    v0 = i0 < rows ? (i0 == k ? 1.0 : a0) : 0.0;
-    v1 = i1 < rows ? a1 : 0.0;
-  };
-  double c0 = 0.0, c1 = 0.0;
-  if (nref > 0) loadv(nref - 1, c0, c1);
-  __syncthreads();
-  for (int k = nref - 1; k >= 0; k--) {
-    double n0, n1;
-    loadv(max(k - 1, 0), n0, n1);
-    const double tau = tauS[k];
-    if (tau != 0.0) {
-      if (nq > 32)
-        lds_reflect_v<16>(Qs, c0, c1, k, rows, tau, wave, nq, lane);
-      else
-        lds_reflect_v<4>(Qs, c0, c1, k, rows, tau, wave, nq, lane);
+  for (int k = tid; k < nref; k += TQ_NT) tauS[k] = gmem(it.tau)[k];
+  double q[TQ_RW];
+  if (it.cin) {
+    const gdbl *c = gmem(it.cin) + it.crow0 + (long long)min(lane, nq - 1) * it.ldc;
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) {
+      const int i = rw0 + r;
+      const double x = c[min(i, nc - 1)];
+      q[r] = (i < nc && lane < nq) ? x : 0.0;
     }
-    c0 = n0;
-    c1 = n1;
+  } else {
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) q[r] = (rw0 + r == lane && lane < nq) ? 1.0 : 0.0;
   }
   __syncthreads();
-  lds_store_panel(Qs, it.qout + it.r0, it.ldq, rows, nq);
+  int par = 0;
+  for (int k = nref - 1; k >= 0; k--) {
+    const double tau = tauS[k];
+    if (tau == 0.0) continue;
+    const int kw = k / TQ_RW;
+    double v[TQ_RW];
+    double pd = 0.0;
+    if (w >= kw) {
+      const double *vk = S + k * TQ_LS + rw0;
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) {
+        v[r] = vk[r];
+        pd += v[r] * q[r];
+      }
+    }
+    part[par][w][lane] = pd;
+    __syncthreads();
+    if (w >= kw) {
+      double dot = part[par][0][lane];
+#pragma unroll
+      for (int ww = 1; ww < TQ_NW; ww++) dot += part[par][ww][lane];
+      const double f = tau * dot;
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) q[r] -= f * v[r];
+    }
+    par ^= 1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < TQ_RW; r++) S[rw0 + r + lane * TQ_LS] = q[r];
+  __syncthreads();
+  lds_store_panel(S, it.qout + it.r0, it.ldq, rows, nq);
 }
 
 // chunking of one level: the LDS path spreads rows evenly over
@@ -864,4 +874,41 @@ extern "C" int hm_hip_audit_qr(int njobs, const int *m, const int *L, int with_r
   }
   if (msg && msglen > 0) snprintf(msg, (size_t)msglen, "%s", text.c_str());
   return rc;
+}
+
+// Thin QR of device panels: the batched Householder/TSQR + BCGS2 program the
+// compression uses.  Each Y_i (m_i x n_i, n_i <= m_i) is overwritten.
+extern "C" int hm_hip_qr_batch(int njobs, const int *m, const int *n, double *const *Y,
+                               const long *ldy, double *const *Q, const long *ldq,
+                               double *const *R, const long *ldr, void *stream) {
+  if (njobs <= 0) return SUCCESS;
+  if (!m || !n || !Y || !ldy || !Q || !ldq) {
+    set_error("hm_hip_qr_batch: NULL argument");
+    return INPUT_ERROR;
+  }
+  std::vector<QrJob> jobs;
+  std::vector<MemRange> inputs;
+  for (int i = 0; i < njobs; i++) {
+    if (n[i] < 1 || m[i] < n[i] || !Y[i] || !Q[i] || ldy[i] < m[i] || ldq[i] < m[i] ||
+        (R && R[i] && ldr[i] < n[i])) {
+      set_error("hm_hip_qr_batch: panel %d has m=%d n=%d (needs 1 <= n <= m, ld >= rows)", i,
+                m[i], n[i]);
+      return INPUT_ERROR;
+    }
+    QrJob J;
+    J.Y = Y[i];
+    J.m = m[i];
+    J.ldy = ldy[i];
+    J.L = n[i];
+    J.Q = Q[i];
+    J.ldq = ldq[i];
+    J.R = (R && R[i]) ? R[i] : nullptr;
+    J.ldr = J.R ? ldr[i] : 0;
+    jobs.push_back(J);
+    inputs.push_back({J.Y, ((size_t)(J.L - 1) * J.ldy + J.m) * sizeof(double)});
+    inputs.push_back({J.Q, ((size_t)(J.L - 1) * J.ldq + J.m) * sizeof(double)});
+    if (J.R) inputs.push_back({J.R, ((size_t)(J.L - 1) * J.ldr + J.L) * sizeof(double)});
+  }
+  return program_exec([&](Program &p, Arena &ar) { return qr_add(p, ar, jobs); }, inputs,
+                      stream_of(stream));
 }

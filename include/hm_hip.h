@@ -188,6 +188,18 @@ HM_HIP_API int hm_hip_recompress_batch(int nnodes, const int *m, const int *n,
                             double *const *d_v, double thr, int method,
                             int *k_out, void *stream);
 
+/* Thin QR of device panels: Y_i (m_i x n_i, 1 <= n_i <= m_i, leading
+ * dimension ldy[i]) = Q_i R_i with Q_i (m_i x n_i, ld ldq[i]) orthonormal
+ * and R_i (n_i x n_i, ld ldr[i], upper triangular; R or R[i] may be NULL)
+ * — the batched Householder TSQR with block Gram-Schmidt across 64-column
+ * panels that compression and recompression use (LAPACK dlarfg signs: the
+ * diagonal of R is -sign of the pivot).  Y_i is overwritten.  Pointer
+ * arrays are host-resident, the panels device-resident.  Plays the part of
+ * the dgeqrt_ / dgemqrt_ pair at src/hodlr_algebra.c:120-128, 177. */
+HM_HIP_API int hm_hip_qr_batch(int njobs, const int *m, const int *n, double *const *Y,
+                               const long *ldy, double *const *Q, const long *ldq,
+                               double *const *R, const long *ldr, void *stream);
+
 /* ---- binary files -------------------------------------------------------
  * The layout of hmat_amd_save_tree (include/hmat_lib_amd.h), written from /
  * read into the device pools directly, so a file saved from a host tree
