@@ -291,11 +291,13 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
     double v[TQ_RW];
     double pd = 0.0;
     if (w >= kw) {
+      double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int r = 0; r < TQ_RW; r++) {
         v[r] = vb[rw0 + r];
-        pd += v[r] * y[r];
+        p4[r & 3] += v[r] * y[r];
       }
+      pd = (p4[0] + p4[1]) + (p4[2] + p4[3]);
     }
     part[w][lane] = pd;
     __syncthreads();
