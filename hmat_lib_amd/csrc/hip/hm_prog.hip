@@ -179,9 +179,13 @@ int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
   }
   if (live.empty()) return SUCCESS;
   const size_t joff = p.push(jobs);
-  size_t toff[GEMM_VARIANTS];
-  int ntile[GEMM_VARIANTS];
+  // tile lists only for the variants that can run: all of them when the op
+  // is tunable, else the one asked for
+  const int fixed = variant < 0 ? -1 : std::min(variant, GEMM_VARIANTS - 1);
+  size_t toff[GEMM_VARIANTS] = {0, 0, 0};
+  int ntile[GEMM_VARIANTS] = {0, 0, 0};
   for (int v = 0; v < GEMM_VARIANTS; v++) {
+    if (fixed >= 0 && v != fixed) continue;
     std::vector<int4> tiles;
     for (int q : live) {
       const GemmJob &J = jobs[q];
@@ -194,7 +198,7 @@ int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
     toff[v] = p.push(tiles);
     ntile[v] = (int)tiles.size();
   }
-  auto choice = std::make_shared<int>(variant < 0 ? 0 : std::min(variant, GEMM_VARIANTS - 1));
+  auto choice = std::make_shared<int>(fixed < 0 ? 0 : fixed);
   if (variant < 0) p.tunables.push_back({p.ops.size(), GEMM_VARIANTS, choice});
   const size_t t0 = toff[0], t1 = toff[1], t2 = toff[2];
   const int n0 = ntile[0], n1 = ntile[1], n2 = ntile[2];
@@ -290,9 +294,13 @@ int gemm_chain_add(Program &p, const std::vector<GemmChain> &chains,
   }
   if (live.empty()) return SUCCESS;
   const size_t coff = p.push(chains), loff = p.push(links);
-  size_t toff[GEMM_VARIANTS];
-  int ntile[GEMM_VARIANTS];
+  // tile lists only for the variants that can run: all of them when the op
+  // is tunable, else the one asked for
+  const int fixed = variant < 0 ? -1 : std::min(variant, GEMM_VARIANTS - 1);
+  size_t toff[GEMM_VARIANTS] = {0, 0, 0};
+  int ntile[GEMM_VARIANTS] = {0, 0, 0};
   for (int v = 0; v < GEMM_VARIANTS; v++) {
+    if (fixed >= 0 && v != fixed) continue;
     std::vector<int4> tiles;
     for (int q : live) {
       const GemmChain &c = chains[q];
@@ -303,7 +311,7 @@ int gemm_chain_add(Program &p, const std::vector<GemmChain> &chains,
     toff[v] = p.push(tiles);
     ntile[v] = (int)tiles.size();
   }
-  auto choice = std::make_shared<int>(variant < 0 ? 0 : std::min(variant, GEMM_VARIANTS - 1));
+  auto choice = std::make_shared<int>(fixed < 0 ? 0 : fixed);
   if (variant < 0) p.tunables.push_back({p.ops.size(), GEMM_VARIANTS, choice});
   const size_t t0 = toff[0], t1 = toff[1], t2 = toff[2];
   const int n0 = ntile[0], n1 = ntile[1], n2 = ntile[2];

@@ -1,16 +1,17 @@
 // Batched tall-skinny Householder QR (TSQR) with explicit Q, FP64.
 //
-// A panel (m x ncol, column-major) is cut into chunks of CR rows (the last
-// chunk takes the remainder, so every chunk has >= ncol rows); each chunk is
-// factored by one workgroup (Householder, LAPACK dlarfg sign convention:
-// beta = -sign(alpha) ||x||, tau = 0 when the tail is zero), its R is
-// stacked into the next level's panel, and so on until one chunk remains.
-// The explicit Q (first nq columns) is then built top-down: each chunk
-// applies its reflectors to [C; 0] where C is its ncol x nq slice of the
-// parent level's Q.  All chunks of all panels of one level run in one
-// launch.  Extras of the top chunk: R (ncol x ncol), R^T, and the norms of
-// the trailing columns of R past column `ell` (for the randomized range
-// finder's posterior test: they equal |(I - Q_1 Q_1^T) y_t|).
+// A panel (m x ncol, ncol <= 64, column-major) is cut into chunks of at most
+// 128 rows (rows spread evenly, so every chunk of a multi-chunk level has
+// >= ncol rows); each chunk is factored by one workgroup (Householder,
+// LAPACK dlarfg sign convention: beta = -sign(alpha) ||x||, tau = 0 when the
+// tail is zero), its R is stacked into the next level's panel, and so on
+// until one chunk remains.  The explicit Q (first nq columns) is then built
+// top-down: each chunk applies its reflectors to [C; 0] where C is its
+// ncol x nq slice of the parent level's Q.  All chunks of all panels of one
+// level run in one launch.  Extras of the top chunk: R (ncol x ncol), R^T,
+// and the norms of the trailing columns of R past column `ell` (for the
+// randomized range finder's posterior test: they equal
+// |(I - Q_1 Q_1^T) y_t|).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -24,167 +25,40 @@ namespace hm {
 
 static constexpr int TQ_NT = 256;
 
-struct QrItem {
-  double *panel;
+// One reduction level of one panel.  Its chunks are work items of 8 bytes
+// (level index, chunk index); chunk i covers rows [i rows / nch,
+// (i + 1) rows / nch) of the level.
+struct QrLevel {
+  double *panel;        // the level's rows (the panel, or the stacked R's)
   long long ld;
-  int r0, rows, ncol, nq;
-  double *tau;
-  double *rout;       // up: R_i destination (rows rrow0.., ld ldr)
+  long long rows;
+  int nch, ncol, nq, is_top;
+  double *tau;          // nch x ncol
+  double *rout;         // up: R_i to rows i ncol.. of the next level (top: R)
   long long ldr;
-  int rrow0;
-  int is_top;
-  const double *cin;  // down: parent Q slice (NULL: identity)
+  const double *cin;    // down: the next level's Q (NULL at the top)
   long long ldc;
-  int crow0;
-  int ell;            // top: trailing-column norms past ell -> *est
-  double *qout;       // down: Q rows [r0, r0+rows) with ld ldq
+  double *qout;         // down: this level's Q (ld ldq)
   long long ldq;
-  double *est;
-  double *mt;         // top: R^T (ncol x ncol, ld ncol)
+  double *est;          // top: trailing-column norms of R past ell
+  double *mt;           // top: R^T (ncol x ncol, ld ncol)
+  int ell, pad;
 };
 
-template <int NV>
-__device__ __forceinline__ void bsum(double (&v)[NV], double *red, double *out) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  constexpr int NW = TQ_NT / 64;
-#pragma unroll
-  for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; i++) red[i * NW + w] = v[i];
-  }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double s = 0.0;
-    for (int ww = 0; ww < NW; ww++) s += red[threadIdx.x * NW + ww];
-    out[threadIdx.x] = s;
-  }
-  __syncthreads();
-}
-
-__device__ __forceinline__ int own_first(int tid, int r) {
-  return tid >= r ? tid : tid + ((r - tid + TQ_NT - 1) / TQ_NT) * TQ_NT;
-}
-
-__global__ __launch_bounds__(TQ_NT) void k_tsqr_up(const QrItem *items) {
-  __shared__ double red[8 * (TQ_NT / 64)], out[8];
-  const QrItem it = items[blockIdx.x];
-  double *Y = it.panel + it.r0;
-  const long long ld = it.ld;
-  const int rows = it.rows, nc = it.ncol, tid = threadIdx.x;
-  const int nref = min(rows, nc);
-  for (int k = 0; k < nref; k++) {
-    double p[1] = {0.0};
-    for (int i = own_first(tid, k + 1); i < rows; i += TQ_NT) {
-      const double x = Y[i + k * ld];
-      p[0] += x * x;
-    }
-    bsum<1>(p, red, out);
-    const double xn2 = out[0];
-    const double alpha = Y[k + k * ld];
-    double tau = 0.0, beta = alpha, scal = 1.0;
-    if (xn2 > 0.0) {
-      const double hh = hypot(alpha, sqrt(xn2));
-      beta = (alpha >= 0.0) ? -hh : hh;
-      tau = (beta - alpha) / beta;
-      scal = 1.0 / (alpha - beta);
-    }
-    __syncthreads();
-    for (int i = own_first(tid, k); i < rows; i += TQ_NT) {
-      if (i == k) Y[i + k * ld] = beta;
-      else if (tau != 0.0) Y[i + k * ld] *= scal;
-    }
-    if (tid == 0) it.tau[k] = tau;
-    if (tau == 0.0) continue;
-    for (int jb = k + 1; jb < nc; jb += 8) {
-      const int nv = min(8, nc - jb);
-      double q[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) q[v] = 0.0;
-      for (int i = own_first(tid, k); i < rows; i += TQ_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) q[v] += vi * Y[i + (jb + v) * ld];
-      }
-      bsum<8>(q, red, out);
-      for (int i = own_first(tid, k); i < rows; i += TQ_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) Y[i + (jb + v) * ld] -= tau * out[v] * vi;
-      }
-    }
-  }
-  if (tid < nc && nref < nc) {
-    for (int k = nref; k < nc; k++)
-      if (tid == 0) it.tau[k] = 0.0;
-  }
-  __syncthreads();
-  // R (upper ncol x ncol; rows past `rows` are zero) to the next level
-  for (int e = tid; e < nc * nc; e += TQ_NT) {
-    const int i = e % nc, j = e / nc;
-    const double r = (i <= j && i < rows) ? Y[i + j * ld] : 0.0;
-    it.rout[it.rrow0 + i + j * it.ldr] = r;
-    if (it.is_top && it.mt) it.mt[j + i * nc] = r;
-  }
-  if (it.is_top && it.est) {
-    __syncthreads();
-    if (tid == 0) {
-      double mx = 0.0;
-      for (int t = it.ell; t < nc; t++) {
-        double ss = 0.0;
-        for (int i = it.ell; i <= t && i < rows; i++) {
-          const double r = Y[i + t * ld];
-          ss += r * r;
-        }
-        mx = fmax(mx, sqrt(ss));
-      }
-      *it.est = mx;
-    }
-  }
-}
-
-__global__ __launch_bounds__(TQ_NT) void k_tsqr_down(const QrItem *items) {
-  __shared__ double red[8 * (TQ_NT / 64)], out[8];
-  const QrItem it = items[blockIdx.x];
-  const double *Y = it.panel + it.r0;
-  const long long ld = it.ld;
-  double *Q = it.qout + it.r0;
-  const long long ldq = it.ldq;
-  const int rows = it.rows, nc = it.ncol, nq = it.nq, tid = threadIdx.x;
-  for (int i = tid; i < rows; i += TQ_NT)
-    for (int j = 0; j < nq; j++) {
-      double v = 0.0;
-      if (i < nc) v = it.cin ? it.cin[it.crow0 + i + j * it.ldc] : (i == j ? 1.0 : 0.0);
-      Q[i + j * ldq] = v;
-    }
-  __syncthreads();
-  const int nref = min(rows, nc);
-  for (int k = nref - 1; k >= 0; k--) {
-    const double tau = it.tau[k];
-    if (tau == 0.0) continue;
-    for (int jb = 0; jb < nq; jb += 8) {
-      const int nv = min(8, nq - jb);
-      double q[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) q[v] = 0.0;
-      for (int i = own_first(tid, k); i < rows; i += TQ_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) q[v] += vi * Q[i + (jb + v) * ldq];
-      }
-      bsum<8>(q, red, out);
-      for (int i = own_first(tid, k); i < rows; i += TQ_NT) {
-        const double vi = (i == k) ? 1.0 : Y[i + k * ld];
-#pragma unroll
-        for (int v = 0; v < 8; v++)
-          if (v < nv) Q[i + (jb + v) * ldq] -= tau * out[v] * vi;
-      }
-    }
-  }
+// the chunk's share of its level
+struct QrChunk {
+  const QrLevel *L;
+  int i, r0, rows;
+};
+__device__ __forceinline__ QrChunk qr_chunk(const QrLevel *levels, const int2 *chunks) {
+  const int2 c = chunks[blockIdx.x];
+  QrChunk q;
+  q.L = levels + c.x;
+  q.i = c.y;
+  const long long rows = q.L->rows, nch = q.L->nch;
+  q.r0 = (int)((long long)c.y * rows / nch);
+  q.rows = (int)(((long long)c.y + 1) * rows / nch) - q.r0;
+  return q;
 }
 
 // ------------------------------------------------ LDS-resident chunks
@@ -229,16 +103,17 @@ __device__ __forceinline__ void lds_store_panel(const double *S, double *A, long
   for (int j = j0; j < ncol; j += 2) g[(long long)j * ld] = S[i + j * TQ_LS];
 }
 
-__global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
+__global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrLevel *levels, const int2 *chunks) {
   __shared__ __attribute__((aligned(16))) double S[TQ_LS * TQ_FAST_COLS];
   __shared__ __attribute__((aligned(16))) double vb[TQ_CR];  // column k below the diagonal
   __shared__ double part[TQ_NW][64];
   __shared__ double tauS[TQ_FAST_COLS], scalS[TQ_FAST_COLS], betaS[TQ_FAST_COLS];
   __shared__ double alphaS;
-  const QrItem it = items[blockIdx.x];
-  const int rows = it.rows, nc = it.ncol, tid = threadIdx.x;
+  const QrChunk ck = qr_chunk(levels, chunks);
+  const QrLevel &it = *ck.L;
+  const int rows = ck.rows, nc = it.ncol, tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, rw0 = w * TQ_RW;
-  double *G = it.panel + it.r0;
+  double *G = it.panel + ck.r0;
   const long long ld = it.ld;
   lds_load_panel(S, G, ld, rows, nc);
   __syncthreads();
@@ -323,12 +198,13 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
     S[i + lane * TQ_LS] = x;
   }
   __syncthreads();
-  for (int k = tid; k < nc; k += TQ_NT) gmem(it.tau)[k] = k < nref ? tauS[k] : 0.0;
+  double *tau_out = it.tau + (long long)ck.i * nc;
+  for (int k = tid; k < nc; k += TQ_NT) gmem(tau_out)[k] = k < nref ? tauS[k] : 0.0;
   lds_store_panel(S, G, ld, rows, nc);
   {
     // R (upper nc x nc, zero below the diagonal and past `rows`)
     const int i = tid & 63;
-    gdbl *ro = gmem(it.rout) + it.rrow0 + i;
+    gdbl *ro = gmem(it.rout) + (it.is_top ? 0 : ck.i * nc) + i;
     gdbl *mt = (it.is_top && it.mt) ? gmem(it.mt) + (long long)i * nc : nullptr;
     if (i < nc)
       for (int j = tid >> 6; j < nc; j += 4) {
@@ -350,18 +226,20 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrItem *items) {
 
 // Q of one chunk: [C; 0] (C the parent's ncol x nq slice, the identity at
 // the top) times the chunk's reflectors in reverse order, column per lane.
-__global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
+__global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrLevel *levels,
+                                                         const int2 *chunks) {
   __shared__ __attribute__((aligned(16))) double S[TQ_LS * TQ_FAST_COLS];
   __shared__ double part[2][TQ_NW][64];
   __shared__ double tauS[TQ_FAST_COLS];
-  const QrItem it = items[blockIdx.x];
-  const int rows = it.rows, nc = it.ncol, nq = it.nq, tid = threadIdx.x;
+  const QrChunk ck = qr_chunk(levels, chunks);
+  const QrLevel &it = *ck.L;
+  const int rows = ck.rows, nc = it.ncol, nq = it.nq, tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, rw0 = w * TQ_RW;
   const int nref = min(rows, nc);
   {
     // reflectors as LDS columns: unit diagonal, zero above it and past rows
     const int i = tid & 127, j0 = tid >> 7;
-    const gdbl *g = gmem(it.panel + it.r0) + min(i, rows - 1);
+    const gdbl *g = gmem(it.panel + ck.r0) + min(i, rows - 1);
     const long long ld = it.ld;
     for (int j = j0; j < nref; j += 16) {
       double x[8];
@@ -374,10 +252,10 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
       }
     }
   }
-  for (int k = tid; k < nref; k += TQ_NT) tauS[k] = gmem(it.tau)[k];
+  for (int k = tid; k < nref; k += TQ_NT) tauS[k] = gmem(it.tau + (long long)ck.i * nc)[k];
   double q[TQ_RW];
   if (it.cin) {
-    const gdbl *c = gmem(it.cin) + it.crow0 + (long long)min(lane, nq - 1) * it.ldc;
+    const gdbl *c = gmem(it.cin) + (long long)ck.i * nc + (long long)min(lane, nq - 1) * it.ldc;
 #pragma unroll
     for (int r = 0; r < TQ_RW; r++) {
       const int i = rw0 + r;
@@ -420,26 +298,12 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrItem *items) {
 #pragma unroll
   for (int r = 0; r < TQ_RW; r++) S[rw0 + r + lane * TQ_LS] = q[r];
   __syncthreads();
-  lds_store_panel(S, it.qout + it.r0, it.ldq, rows, nq);
+  lds_store_panel(S, it.qout + ck.r0, it.ldq, rows, nq);
 }
 
-// chunking of one level: the LDS path spreads rows evenly over
-// ceil(rows / TQ_CR) chunks; the general path uses chunks of CR rows with
-// the remainder in the last
-static long long tq_nchunks(long long rows, int ncol) {
-  if (ncol <= TQ_FAST_COLS) return std::max<long long>(1, (rows + TQ_CR - 1) / TQ_CR);
-  return std::max<long long>(1, rows / std::max(128, 2 * ncol));
-}
-static void tq_chunk(long long rows, long long nch, long long i, int ncol, long long &r0,
-                     long long &rr) {
-  if (ncol <= TQ_FAST_COLS) {
-    r0 = i * rows / nch;
-    rr = (i + 1) * rows / nch - r0;
-  } else {
-    const int CR = std::max(128, 2 * ncol);
-    r0 = i * CR;
-    rr = (i == nch - 1) ? rows - i * CR : CR;
-  }
+// chunks of one level: rows spread evenly over ceil(rows / TQ_CR) chunks
+static long long tq_nchunks(long long rows) {
+  return std::max<long long>(1, (rows + TQ_CR - 1) / TQ_CR);
 }
 
 int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
@@ -448,34 +312,32 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
   std::vector<std::vector<long long>> rows_per_level(panels.size());
   for (size_t p = 0; p < panels.size(); p++) {
     const TsqrPanel &P = panels[p];
-    if (P.m < P.ncol || P.ncol < 1 || P.nq > P.ncol) {
-      set_error("tsqr: panel %zu has m=%d ncol=%d nq=%d", p, P.m, P.ncol, P.nq);
+    if (P.m < P.ncol || P.ncol < 1 || P.ncol > TQ_FAST_COLS || P.nq > P.ncol) {
+      set_error("tsqr: panel %zu has m=%d ncol=%d nq=%d (needs ncol <= %d)", p, P.m, P.ncol,
+                P.nq, TQ_FAST_COLS);
       return INPUT_ERROR;
     }
     long long rows = P.m;
     for (;;) {
       rows_per_level[p].push_back(rows);
-      const long long nch = tq_nchunks(rows, P.ncol);
+      const long long nch = tq_nchunks(rows);
       if (nch == 1) break;
       rows = nch * P.ncol;
     }
   }
-  struct Level {
-    std::vector<QrItem> up, down, up_fast, down_fast;
-  };
   size_t maxL = 0;
   for (auto &r : rows_per_level) maxL = std::max(maxL, r.size());
-  std::vector<Level> levels(maxL);
+  std::vector<QrLevel> levels;
+  std::vector<std::vector<int2>> up(maxL), down(maxL);
   for (size_t p = 0; p < panels.size(); p++) {
     const TsqrPanel &P = panels[p];
-    const bool fast = P.ncol <= TQ_FAST_COLS;
     const size_t nl = rows_per_level[p].size();
     // scratch of this panel: tau per level, stacked R and level Q for
     // levels >= 1, the top R when the caller wants none
     std::vector<double *> tau(nl), st(nl, nullptr), qs(nl, nullptr);
     for (size_t L = 0; L < nl; L++) {
       const long long rows = rows_per_level[p][L];
-      const long long nch = tq_nchunks(rows, P.ncol);
+      const long long nch = tq_nchunks(rows);
       tau[L] = ar.take<double>((size_t)nch * P.ncol);
       prog.touch(tau[L], (size_t)nch * P.ncol * sizeof(double), "tsqr tau");
       if (L > 0) {
@@ -496,83 +358,65 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
     if (P.est) prog.touch(P.est, sizeof(double), "tsqr est");
     for (size_t L = 0; L < nl; L++) {
       const long long rows = rows_per_level[p][L];
-      const long long nch = tq_nchunks(rows, P.ncol);
+      const long long nch = tq_nchunks(rows);
       const bool top = (L + 1 == nl);
-      double *store = L == 0 ? P.A : st[L];
-      const long long ld = L == 0 ? P.lda : rows;
-      double *qstore = L == 0 ? P.Q : qs[L];
-      const long long ldq = L == 0 ? P.ldq : rows;
-      const long long rows_next = top ? 0 : rows_per_level[p][L + 1];
+      QrLevel lv;
+      memset(&lv, 0, sizeof(lv));
+      lv.panel = L == 0 ? P.A : st[L];
+      lv.ld = L == 0 ? P.lda : rows;
+      lv.rows = rows;
+      lv.nch = (int)nch;
+      lv.ncol = P.ncol;
+      lv.nq = P.nq;
+      lv.is_top = top ? 1 : 0;
+      lv.tau = tau[L];
+      if (top) {
+        lv.rout = rtop;
+        lv.ldr = ldr_top;
+        lv.mt = P.Mt;
+        lv.est = P.est;
+        lv.ell = P.ell;
+      } else {
+        const long long rows_next = rows_per_level[p][L + 1];
+        lv.rout = st[L + 1];
+        lv.ldr = rows_next;
+        lv.cin = qs[L + 1];
+        lv.ldc = rows_next;
+      }
+      lv.qout = L == 0 ? P.Q : qs[L];
+      lv.ldq = L == 0 ? P.ldq : rows;
+      const int li = (int)levels.size();
+      levels.push_back(lv);
       for (long long i = 0; i < nch; i++) {
-        QrItem it;
-        memset(&it, 0, sizeof(it));
-        long long cr0, crr;
-        tq_chunk(rows, nch, i, P.ncol, cr0, crr);
-        it.panel = store;
-        it.ld = ld;
-        it.r0 = (int)cr0;
-        it.rows = (int)crr;
-        it.ncol = P.ncol;
-        it.nq = P.nq;
-        it.tau = tau[L] + i * P.ncol;
-        it.is_top = top ? 1 : 0;
-        if (top) {
-          it.rout = rtop;
-          it.ldr = ldr_top;
-          it.rrow0 = 0;
-          it.mt = P.Mt;
-          it.est = P.est;
-          it.ell = P.ell;
-          it.cin = nullptr;
-        } else {
-          it.rout = st[L + 1];
-          it.ldr = rows_next;
-          it.rrow0 = (int)(i * P.ncol);
-          it.cin = qs[L + 1];
-          it.ldc = rows_next;
-          it.crow0 = (int)(i * P.ncol);
-        }
-        it.qout = qstore;
-        it.ldq = ldq;
-        (fast ? levels[L].up_fast : levels[L].up).push_back(it);
-        if (P.Q) (fast ? levels[L].down_fast : levels[L].down).push_back(it);
+        up[L].push_back(make_int2(li, (int)i));
+        if (P.Q) down[L].push_back(make_int2(li, (int)i));
       }
     }
   }
-  std::vector<size_t> up_off(maxL), down_off(maxL), upf_off(maxL), downf_off(maxL);
+  const size_t lev_off = prog.push(levels);
+  std::vector<size_t> up_off(maxL), down_off(maxL);
   for (size_t L = 0; L < maxL; L++) {
-    up_off[L] = prog.push(levels[L].up);
-    down_off[L] = prog.push(levels[L].down);
-    upf_off[L] = prog.push(levels[L].up_fast);
-    downf_off[L] = prog.push(levels[L].down_fast);
+    up_off[L] = prog.push(up[L]);
+    down_off[L] = prog.push(down[L]);
   }
   for (size_t L = 0; L < maxL; L++) {
-    const unsigned n = (unsigned)levels[L].up.size(), nf = (unsigned)levels[L].up_fast.size();
-    const size_t off = up_off[L], offf = upf_off[L];
+    const unsigned n = (unsigned)up[L].size();
+    const size_t off = up_off[L];
     if (n)
       prog.add("tsqr_up", [=](hipStream_t s, const unsigned char *b) {
-        hipLaunchKernelGGL(k_tsqr_up, dim3(n), dim3(TQ_NT), 0, s,
-                           reinterpret_cast<const QrItem *>(b + off));
-      });
-    if (nf)
-      prog.add("tsqr_up", [=](hipStream_t s, const unsigned char *b) {
-        hipLaunchKernelGGL(k_tsqr_up_lds, dim3(nf), dim3(TQ_NT), 0, s,
-                           reinterpret_cast<const QrItem *>(b + offf));
+        hipLaunchKernelGGL(k_tsqr_up_lds, dim3(n), dim3(TQ_NT), 0, s,
+                           reinterpret_cast<const QrLevel *>(b + lev_off),
+                           reinterpret_cast<const int2 *>(b + off));
       });
   }
   for (size_t L = maxL; L-- > 0;) {
-    const unsigned n = (unsigned)levels[L].down.size(),
-                   nf = (unsigned)levels[L].down_fast.size();
-    const size_t off = down_off[L], offf = downf_off[L];
+    const unsigned n = (unsigned)down[L].size();
+    const size_t off = down_off[L];
     if (n)
       prog.add("tsqr_down", [=](hipStream_t s, const unsigned char *b) {
-        hipLaunchKernelGGL(k_tsqr_down, dim3(n), dim3(TQ_NT), 0, s,
-                           reinterpret_cast<const QrItem *>(b + off));
-      });
-    if (nf)
-      prog.add("tsqr_down", [=](hipStream_t s, const unsigned char *b) {
-        hipLaunchKernelGGL(k_tsqr_down_lds, dim3(nf), dim3(TQ_NT), 0, s,
-                           reinterpret_cast<const QrItem *>(b + offf));
+        hipLaunchKernelGGL(k_tsqr_down_lds, dim3(n), dim3(TQ_NT), 0, s,
+                           reinterpret_cast<const QrLevel *>(b + lev_off),
+                           reinterpret_cast<const int2 *>(b + off));
       });
   }
   return SUCCESS;
