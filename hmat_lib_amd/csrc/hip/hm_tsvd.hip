@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 
 #include "hm_tsqr.h"
 #include "hm_tsvd.h"
@@ -636,6 +637,59 @@ __global__ __launch_bounds__(256) void k_outer(const OuterJob *jobs, const int4 
 // ------------------------------------------------------------ driver
 
 namespace {
+// The round's work buffers (sketch, Omega, Q, B^T, Q2, cores, sigma) carved
+// from one device buffer that is kept between rounds and calls: each round
+// ends with its stream synchronized, so the buffer is idle in between.  A
+// round that finds it taken (another host thread), runs on another device
+// or needs more than kRoundKeep bytes allocates its own.
+struct RoundCache {
+  std::mutex mu;
+  bool busy = false;
+  int device = -1;
+  char *buf = nullptr;
+  size_t cap = 0;
+};
+RoundCache g_round;
+constexpr size_t kRoundKeep = (size_t)8 << 30;
+
+// Holds the cached buffer for one round (or owns a buffer of its own).
+struct RoundBuffer {
+  char *base = nullptr;
+  bool cached = false, own = false;
+  int get(size_t need) {
+    int dev = -1;
+    if (need <= kRoundKeep && hipGetDevice(&dev) == hipSuccess) {
+      std::lock_guard<std::mutex> lock(g_round.mu);
+      if (!g_round.busy && (g_round.device < 0 || g_round.device == dev)) {
+        g_round.busy = cached = true;
+        g_round.device = dev;
+      }
+    }
+    if (cached) {
+      if (g_round.cap < need) {
+        const size_t want = std::max(need, g_round.cap + g_round.cap / 2);
+        dfree(g_round.buf);
+        g_round.buf = nullptr;
+        g_round.cap = 0;
+        HM_OK(dalloc(&g_round.buf, want));
+        g_round.cap = want;
+      }
+      base = g_round.buf;
+      return SUCCESS;
+    }
+    HM_OK(dalloc(&base, need));
+    own = true;
+    return SUCCESS;
+  }
+  ~RoundBuffer() {
+    if (own) dfree(base);
+    if (cached) {
+      std::lock_guard<std::mutex> lock(g_round.mu);
+      g_round.busy = false;
+    }
+  }
+};
+
 struct Scoped {
   std::vector<void *> ptrs;
   ~Scoped() {
@@ -702,16 +756,19 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       st.max_sketch = std::max(st.max_sketch, b.ell);
     }
     TsvdBufs bf;
-    HM_OK(dalloc(&bf.Y, ytot)); mem.keep(bf.Y);
-    HM_OK(dalloc(&bf.Om, otot)); mem.keep(bf.Om);
-    HM_OK(dalloc(&bf.Q, qtot)); mem.keep(bf.Q);
-    HM_OK(dalloc(&bf.Bt, btot)); mem.keep(bf.Bt);
-    HM_OK(dalloc(&bf.Q2, btot)); mem.keep(bf.Q2);
-    HM_OK(dalloc(&bf.M, mtot)); mem.keep(bf.M);
-    HM_OK(dalloc(&bf.Vj, mtot)); mem.keep(bf.Vj);
-    HM_OK(dalloc(&bf.G, mtot)); mem.keep(bf.G);
-    HM_OK(dalloc(&bf.Z, mtot)); mem.keep(bf.Z);
-    HM_OK(dalloc(&bf.Sig, stot)); mem.keep(bf.Sig);
+    RoundBuffer rbuf;
+    {
+      const long long sizes[10] = {ytot, otot, qtot, btot, btot, mtot, mtot, mtot, mtot, stot};
+      double **dst[10] = {&bf.Y, &bf.Om, &bf.Q, &bf.Bt, &bf.Q2,
+                          &bf.M, &bf.Vj, &bf.G, &bf.Z, &bf.Sig};
+      size_t offs[10], need = 0;
+      for (int q = 0; q < 10; q++) {
+        offs[q] = need;
+        need += ((size_t)sizes[q] * sizeof(double) + 16 + 255) & ~(size_t)255;
+      }
+      HM_OK(rbuf.get(need));
+      for (int q = 0; q < 10; q++) *dst[q] = reinterpret_cast<double *>(rbuf.base + offs[q]);
+    }
     TsvdBlock *d_blocks = nullptr;
     HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
     // work lists, biggest K first for balance
