@@ -50,8 +50,9 @@ struct QrChunk {
   const QrLevel *L;
   int i, r0, rows;
 };
-__device__ __forceinline__ QrChunk qr_chunk(const QrLevel *levels, const int2 *chunks) {
-  const int2 c = chunks[blockIdx.x];
+__device__ __forceinline__ QrChunk qr_chunk(const QrLevel *levels, const int2 *chunks,
+                                            int idx) {
+  const int2 c = chunks[idx];
   QrChunk q;
   q.L = levels + c.x;
   q.i = c.y;
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrLevel *levels, co
   __shared__ double part[TQ_NW][64];
   __shared__ double tauS[TQ_FAST_COLS], scalS[TQ_FAST_COLS], betaS[TQ_FAST_COLS];
   __shared__ double alphaS;
-  const QrChunk ck = qr_chunk(levels, chunks);
+  const QrChunk ck = qr_chunk(levels, chunks, blockIdx.x);
   const QrLevel &it = *ck.L;
   const int rows = ck.rows, nc = it.ncol, tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, rw0 = w * TQ_RW;
@@ -224,6 +225,137 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_lds(const QrLevel *levels, co
   }
 }
 
+// Panels of at most 16 columns: one wave per chunk, four chunks per
+// workgroup and no workgroup barrier.  Lane 16 q + j keeps rows
+// 32 q .. 32 q + 31 of column j in registers; the four row quarters of a
+// column are summed with lane shuffles.  Same reflectors, same output
+// layout as k_tsqr_up_lds (whose comments give the conventions).
+static constexpr int TQ_NARROW = 16;
+static constexpr int TQ_NLS = TQ_CR + 1;  // LDS column stride of the narrow staging
+__global__ __launch_bounds__(TQ_NT) void k_tsqr_up_narrow(const QrLevel *levels,
+                                                          const int2 *chunks, int nchunks) {
+  __shared__ double Sw[TQ_NW][TQ_NLS * TQ_NARROW];
+  __shared__ __attribute__((aligned(16))) double vbw[TQ_NW][TQ_CR];
+  __shared__ double tauW[TQ_NW][TQ_NARROW], scalW[TQ_NW][TQ_NARROW], betaW[TQ_NW][TQ_NARROW];
+  __shared__ double alphaW[TQ_NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int idx = blockIdx.x * TQ_NW + w;
+  if (idx >= nchunks) return;  // no workgroup barriers below
+  const QrChunk ck = qr_chunk(levels, chunks, idx);
+  const QrLevel &it = *ck.L;
+  const int rows = ck.rows, nc = it.ncol;
+  const int q = lane >> 4, j = lane & 15, rq0 = q * TQ_RW;
+  double *S = Sw[w], *vb = vbw[w];
+  double *G = it.panel + ck.r0;
+  const long long ld = it.ld;
+  {
+    const gdbl *g = gmem(G);
+    for (int jj = 0; jj < nc; jj++) {
+      const double a = g[min(lane, rows - 1) + jj * ld], b = g[min(lane + 64, rows - 1) + jj * ld];
+      if (lane < rows) S[lane + jj * TQ_NLS] = a;
+      if (lane + 64 < rows) S[lane + 64 + jj * TQ_NLS] = b;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  double y[TQ_RW];
+  {
+    const double *sj = S + min(j, nc - 1) * TQ_NLS + rq0;
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) {
+      const double x = sj[r];
+      y[r] = (rq0 + r < rows && j < nc) ? x : 0.0;
+    }
+  }
+  const int nref = min(rows, nc);
+  for (int k = 0; k < nref; k++) {
+    const int kq = k / TQ_RW, kr = k % TQ_RW;
+    __builtin_amdgcn_wave_barrier();
+    if (j == k && q >= kq) {
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) vb[rq0 + r] = y[r];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (j == k && q == kq) {
+      alphaW[w] = vb[k];
+      for (int r = 0; r <= kr; r++) vb[rq0 + r] = 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const double a0 = vb[lane], a1 = vb[lane + 64];
+    const double x0 = lane > k ? a0 : 0.0, x1 = lane + 64 > k ? a1 : 0.0;
+    const double x2 = wave_sum(x0 * x0 + x1 * x1);
+    const double alpha = alphaW[w];
+    double tau = 0.0, beta = alpha, scal = 1.0;
+    if (x2 > 0.0) {
+      const double hh = hypot(alpha, sqrt(x2));
+      beta = (alpha >= 0.0) ? -hh : hh;
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      tauW[w][k] = tau;
+      scalW[w][k] = scal;
+      betaW[w][k] = beta;
+      vb[k] = alpha - beta;  // the unit entry of v = scal * vb
+    }
+    __builtin_amdgcn_wave_barrier();
+    double v[TQ_RW];
+    double p4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) {
+      v[r] = vb[rq0 + r];
+      p4[r & 3] += v[r] * y[r];
+    }
+    double pd = q >= kq ? (p4[0] + p4[1]) + (p4[2] + p4[3]) : 0.0;
+    pd += __shfl_xor(pd, 16, 64);
+    pd += __shfl_xor(pd, 32, 64);
+    if (tau != 0.0 && j > k && q >= kq) {
+      const double fs = tau * scal * scal * pd;
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) y[r] -= fs * v[r];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int r = 0; r < TQ_RW; r++) {
+    const int i = rq0 + r;
+    double x = y[r];
+    if (j < nref) {
+      if (i > j) x *= scalW[w][j];
+      else if (i == j) x = betaW[w][j];
+    }
+    if (i < TQ_CR) S[i + j * TQ_NLS] = x;
+  }
+  __builtin_amdgcn_wave_barrier();
+  {
+    double *tau_out = it.tau + (long long)ck.i * nc;
+    if (lane < nc) gmem(tau_out)[lane] = lane < nref ? tauW[w][lane] : 0.0;
+    gdbl *g = gmem(G);
+    for (int jj = 0; jj < nc; jj++) {
+      if (lane < rows) g[lane + jj * ld] = S[lane + jj * TQ_NLS];
+      if (lane + 64 < rows) g[lane + 64 + jj * ld] = S[lane + 64 + jj * TQ_NLS];
+    }
+    // R (upper nc x nc, zero below the diagonal and past `rows`)
+    gdbl *ro = gmem(it.rout) + (it.is_top ? 0 : ck.i * nc);
+    gdbl *mt = (it.is_top && it.mt) ? gmem(it.mt) : nullptr;
+    for (int e = lane; e < nc * nc; e += 64) {
+      const int i = e % nc, jj = e / nc;
+      const double r = (i <= jj && i < rows) ? S[i + jj * TQ_NLS] : 0.0;
+      ro[i + (long long)jj * it.ldr] = r;
+      if (mt) mt[jj + (long long)i * nc] = r;
+    }
+    if (it.is_top && it.est && lane == 0) {
+      double mx = 0.0;
+      for (int t = it.ell; t < nc; t++) {
+        double ss = 0.0;
+        for (int i = it.ell; i <= t && i < rows; i++) ss += S[i + t * TQ_NLS] * S[i + t * TQ_NLS];
+        mx = fmax(mx, sqrt(ss));
+      }
+      *gmem(it.est) = mx;
+    }
+  }
+}
+
 // Q of one chunk: [C; 0] (C the parent's ncol x nq slice, the identity at
 // the top) times the chunk's reflectors in reverse order, column per lane.
 __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrLevel *levels,
@@ -231,7 +363,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrLevel *levels,
   __shared__ __attribute__((aligned(16))) double S[TQ_LS * TQ_FAST_COLS];
   __shared__ double part[2][TQ_NW][64];
   __shared__ double tauS[TQ_FAST_COLS];
-  const QrChunk ck = qr_chunk(levels, chunks);
+  const QrChunk ck = qr_chunk(levels, chunks, blockIdx.x);
   const QrLevel &it = *ck.L;
   const int rows = ck.rows, nc = it.ncol, nq = it.nq, tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, rw0 = w * TQ_RW;
@@ -328,7 +460,7 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
   size_t maxL = 0;
   for (auto &r : rows_per_level) maxL = std::max(maxL, r.size());
   std::vector<QrLevel> levels;
-  std::vector<std::vector<int2>> up(maxL), down(maxL);
+  std::vector<std::vector<int2>> up(maxL), up_narrow(maxL), down(maxL);
   for (size_t p = 0; p < panels.size(); p++) {
     const TsqrPanel &P = panels[p];
     const size_t nl = rows_per_level[p].size();
@@ -388,25 +520,31 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
       const int li = (int)levels.size();
       levels.push_back(lv);
       for (long long i = 0; i < nch; i++) {
-        up[L].push_back(make_int2(li, (int)i));
+        (P.ncol <= TQ_NARROW ? up_narrow[L] : up[L]).push_back(make_int2(li, (int)i));
         if (P.Q) down[L].push_back(make_int2(li, (int)i));
       }
     }
   }
   const size_t lev_off = prog.push(levels);
-  std::vector<size_t> up_off(maxL), down_off(maxL);
+  std::vector<size_t> up_off(maxL), upn_off(maxL), down_off(maxL);
   for (size_t L = 0; L < maxL; L++) {
     up_off[L] = prog.push(up[L]);
+    upn_off[L] = prog.push(up_narrow[L]);
     down_off[L] = prog.push(down[L]);
   }
   for (size_t L = 0; L < maxL; L++) {
-    const unsigned n = (unsigned)up[L].size();
-    const size_t off = up_off[L];
-    if (n)
+    const unsigned n = (unsigned)up[L].size(), nn = (unsigned)up_narrow[L].size();
+    const size_t off = up_off[L], offn = upn_off[L];
+    if (n || nn)
       prog.add("tsqr_up", [=](hipStream_t s, const unsigned char *b) {
-        hipLaunchKernelGGL(k_tsqr_up_lds, dim3(n), dim3(TQ_NT), 0, s,
-                           reinterpret_cast<const QrLevel *>(b + lev_off),
-                           reinterpret_cast<const int2 *>(b + off));
+        if (n)
+          hipLaunchKernelGGL(k_tsqr_up_lds, dim3(n), dim3(TQ_NT), 0, s,
+                             reinterpret_cast<const QrLevel *>(b + lev_off),
+                             reinterpret_cast<const int2 *>(b + off));
+        if (nn)
+          hipLaunchKernelGGL(k_tsqr_up_narrow, dim3((nn + TQ_NW - 1) / TQ_NW), dim3(TQ_NT), 0, s,
+                             reinterpret_cast<const QrLevel *>(b + lev_off),
+                             reinterpret_cast<const int2 *>(b + offn), (int)nn);
       });
   }
   for (size_t L = maxL; L-- > 0;) {
