@@ -433,6 +433,81 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_lds(const QrLevel *levels,
   lds_store_panel(S, it.qout + ck.r0, it.ldq, rows, nq);
 }
 
+// Q of a chunk of a panel of at most 16 columns: one wave per chunk, four
+// chunks per workgroup, lane 16 q + j holding rows 32 q .. 32 q + 31 of
+// column j of Q (as in k_tsqr_up_narrow); no workgroup barriers.
+__global__ __launch_bounds__(TQ_NT) void k_tsqr_down_narrow(const QrLevel *levels,
+                                                            const int2 *chunks, int nchunks) {
+  __shared__ __attribute__((aligned(16))) double Sw[TQ_NW][TQ_CR * TQ_NARROW];
+  __shared__ double tauW[TQ_NW][TQ_NARROW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int idx = blockIdx.x * TQ_NW + w;
+  if (idx >= nchunks) return;  // no workgroup barriers below
+  const QrChunk ck = qr_chunk(levels, chunks, idx);
+  const QrLevel &it = *ck.L;
+  const int rows = ck.rows, nc = it.ncol, nq = it.nq;
+  const int q = lane >> 4, j = lane & 15, rq0 = q * TQ_RW;
+  const int nref = min(rows, nc);
+  double *S = Sw[w];
+  {
+    // reflectors as LDS columns: unit diagonal, zero above it and past rows
+    const gdbl *g = gmem(it.panel + ck.r0);
+    const long long ld = it.ld;
+    for (int jj = 0; jj < nref; jj++) {
+      const double a = g[min(lane, rows - 1) + jj * ld], b = g[min(lane + 64, rows - 1) + jj * ld];
+      S[lane + jj * TQ_CR] = (lane > jj && lane < rows) ? a : (lane == jj ? 1.0 : 0.0);
+      S[lane + 64 + jj * TQ_CR] = (lane + 64 < rows) ? b : 0.0;
+    }
+  }
+  if (lane < nref) tauW[w][lane] = gmem(it.tau + (long long)ck.i * nc)[lane];
+  double qv[TQ_RW];
+  if (it.cin) {
+    const gdbl *c = gmem(it.cin) + (long long)ck.i * nc + (long long)min(j, nq - 1) * it.ldc;
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) {
+      const int i = rq0 + r;
+      const double x = c[min(i, nc - 1)];
+      qv[r] = (i < nc && j < nq) ? x : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) qv[r] = (rq0 + r == j && j < nq) ? 1.0 : 0.0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int k = nref - 1; k >= 0; k--) {
+    const double tau = tauW[w][k];
+    if (tau == 0.0) continue;
+    const int kq = k / TQ_RW;
+    const double *vk = S + k * TQ_CR + rq0;
+    double v[TQ_RW];
+    double p4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < TQ_RW; r++) {
+      v[r] = vk[r];
+      p4[r & 3] += v[r] * qv[r];
+    }
+    double pd = q >= kq ? (p4[0] + p4[1]) + (p4[2] + p4[3]) : 0.0;
+    pd += __shfl_xor(pd, 16, 64);
+    pd += __shfl_xor(pd, 32, 64);
+    if (q >= kq) {
+      const double f = tau * pd;
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++) qv[r] -= f * v[r];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  {
+    // Q rows of this chunk, one column per 16 lanes' group: straight from
+    // registers (lane (q, j) writes its 32 rows of column j)
+    gdbl *Q = gmem(it.qout + ck.r0) + (long long)j * it.ldq;
+    if (j < nq) {
+#pragma unroll
+      for (int r = 0; r < TQ_RW; r++)
+        if (rq0 + r < rows) Q[rq0 + r] = qv[r];
+    }
+  }
+}
+
 // chunks of one level: rows spread evenly over ceil(rows / TQ_CR) chunks
 static long long tq_nchunks(long long rows) {
   return std::max<long long>(1, (rows + TQ_CR - 1) / TQ_CR);
@@ -460,7 +535,7 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
   size_t maxL = 0;
   for (auto &r : rows_per_level) maxL = std::max(maxL, r.size());
   std::vector<QrLevel> levels;
-  std::vector<std::vector<int2>> up(maxL), up_narrow(maxL), down(maxL);
+  std::vector<std::vector<int2>> up(maxL), up_narrow(maxL), down(maxL), down_narrow(maxL);
   for (size_t p = 0; p < panels.size(); p++) {
     const TsqrPanel &P = panels[p];
     const size_t nl = rows_per_level[p].size();
@@ -521,16 +596,17 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
       levels.push_back(lv);
       for (long long i = 0; i < nch; i++) {
         (P.ncol <= TQ_NARROW ? up_narrow[L] : up[L]).push_back(make_int2(li, (int)i));
-        if (P.Q) down[L].push_back(make_int2(li, (int)i));
+        if (P.Q) (P.ncol <= TQ_NARROW ? down_narrow[L] : down[L]).push_back(make_int2(li, (int)i));
       }
     }
   }
   const size_t lev_off = prog.push(levels);
-  std::vector<size_t> up_off(maxL), upn_off(maxL), down_off(maxL);
+  std::vector<size_t> up_off(maxL), upn_off(maxL), down_off(maxL), downn_off(maxL);
   for (size_t L = 0; L < maxL; L++) {
     up_off[L] = prog.push(up[L]);
     upn_off[L] = prog.push(up_narrow[L]);
     down_off[L] = prog.push(down[L]);
+    downn_off[L] = prog.push(down_narrow[L]);
   }
   for (size_t L = 0; L < maxL; L++) {
     const unsigned n = (unsigned)up[L].size(), nn = (unsigned)up_narrow[L].size();
@@ -548,13 +624,18 @@ int tsqr_add(Program &prog, Arena &ar, const std::vector<TsqrPanel> &panels) {
       });
   }
   for (size_t L = maxL; L-- > 0;) {
-    const unsigned n = (unsigned)down[L].size();
-    const size_t off = down_off[L];
-    if (n)
+    const unsigned n = (unsigned)down[L].size(), nn = (unsigned)down_narrow[L].size();
+    const size_t off = down_off[L], offn = downn_off[L];
+    if (n || nn)
       prog.add("tsqr_down", [=](hipStream_t s, const unsigned char *b) {
-        hipLaunchKernelGGL(k_tsqr_down_lds, dim3(n), dim3(TQ_NT), 0, s,
-                           reinterpret_cast<const QrLevel *>(b + lev_off),
-                           reinterpret_cast<const int2 *>(b + off));
+        if (n)
+          hipLaunchKernelGGL(k_tsqr_down_lds, dim3(n), dim3(TQ_NT), 0, s,
+                             reinterpret_cast<const QrLevel *>(b + lev_off),
+                             reinterpret_cast<const int2 *>(b + off));
+        if (nn)
+          hipLaunchKernelGGL(k_tsqr_down_narrow, dim3((nn + TQ_NW - 1) / TQ_NW), dim3(TQ_NT), 0,
+                             s, reinterpret_cast<const QrLevel *>(b + lev_off),
+                             reinterpret_cast<const int2 *>(b + offn), (int)nn);
       });
   }
   return SUCCESS;
