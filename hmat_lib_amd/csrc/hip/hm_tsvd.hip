@@ -105,6 +105,7 @@ __global__ __launch_bounds__(256) void k_sketch(TsvdBlock *blocks,
                                                 const int4 *work, TsvdBufs bf) {
   __shared__ double lds[SketchTile::LDS_DOUBLES];
   const int4 w = work[blockIdx.x];
+  if (w.x < 0) return;  // padding of a shorter XCD sequence
   const TsvdBlock B = blocks[w.x];
   const int m = B.m, n = B.n, L = B.ell + B.ntest;
   const int i0 = w.y * SketchTile::TM, j0 = w.z * SketchTile::TN;
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(256) void k_project(TsvdBlock *blocks,
                                                  const int4 *work, TsvdBufs bf) {
   __shared__ double lds[Tile::LDS_DOUBLES];
   const int4 w = work[blockIdx.x];
+  if (w.x < 0) return;  // padding of a shorter XCD sequence
   const TsvdBlock B = blocks[w.x];
   const int m = B.m, n = B.n, ell = B.ell;
   const int i0 = w.y * Tile::TM, j0 = w.z * Tile::TN;
@@ -637,6 +639,32 @@ __global__ __launch_bounds__(256) void k_outer(const OuterJob *jobs, const int4 
 // ------------------------------------------------------------ driver
 
 namespace {
+// Work order for the sketch and projection passes: the items of one (block,
+// row tile) — its column tiles, which all read the same rows of A — go to
+// one XCD (workgroup b runs on XCD b % 8), so that the A tile is fetched into
+// that XCD's L2 once instead of once per XCD.  Groups are dealt greedily to
+// the XCD with the least work so far; the shorter XCD sequences are padded
+// with empty items (x = -1).
+std::vector<int4> xcd_grouped(const std::vector<int4> &items) {
+  constexpr int NX = 8;
+  std::vector<std::vector<int4>> seq(NX);
+  for (size_t a = 0; a < items.size();) {
+    size_t b = a + 1;
+    while (b < items.size() && items[b].x == items[a].x && items[b].y == items[a].y) b++;
+    int x = 0;
+    for (int k = 1; k < NX; k++)
+      if (seq[k].size() < seq[x].size()) x = k;
+    seq[x].insert(seq[x].end(), items.begin() + a, items.begin() + b);
+    a = b;
+  }
+  size_t len = 0;
+  for (auto &q : seq) len = std::max(len, q.size());
+  std::vector<int4> out(len * NX, make_int4(-1, 0, 0, 0));
+  for (int x = 0; x < NX; x++)
+    for (size_t t = 0; t < seq[x].size(); t++) out[t * NX + x] = seq[x][t];
+  return out;
+}
+
 // The round's work buffers (sketch, Omega, Q, B^T, Q2, cores, sigma) carved
 // from one device buffer that is kept between rounds and calls: each round
 // ends with its stream synchronized, so the buffer is idle in between.  A
@@ -799,6 +827,9 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       for (int a = 0; a < mt; a++)
         for (int c = 0; c < ntl; c++) (narrow ? wp16 : wp).push_back(make_int4(i, a, c, 0));
     }
+    ws = xcd_grouped(ws);
+    wp = xcd_grouped(wp);
+    wp16 = xcd_grouped(wp16);
     int2 *d_wo;
     int4 *d_ws, *d_wp;
     int *d_list;
