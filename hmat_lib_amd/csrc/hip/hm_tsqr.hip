@@ -947,7 +947,7 @@ extern "C" int hm_hip_qr_batch(int njobs, const int *m, const int *n, double *co
                                const long *ldy, double *const *Q, const long *ldq,
                                double *const *R, const long *ldr, void *stream) {
   if (njobs <= 0) return SUCCESS;
-  if (!m || !n || !Y || !ldy || !Q || !ldq) {
+  if (!m || !n || !Y || !ldy || !Q || !ldq || (R && !ldr)) {
     set_error("hm_hip_qr_batch: NULL argument");
     return INPUT_ERROR;
   }
