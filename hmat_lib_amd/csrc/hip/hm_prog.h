@@ -102,10 +102,13 @@ struct GemmJob {
 };
 
 // Tile variants (TM x TN): 0 = 64 x 64, 1 = 64 x 16 (narrow RHS),
-// 2 = 16 x 64 (few rows, e.g. rank-1 factors).  variant < 0 registers the
-// op as tunable (starts at 0); the kernel is persistent: at most
-// GEMM_MAX_GRID workgroups loop over the tiles.
+// 2 = 16 x 64 (few rows, e.g. rank-1 factors).  variant -1 registers the
+// op as tunable (starts at 0); GEMM_BY_SHAPE picks from the largest M and N
+// of the jobs (N <= 16: 64 x 16, or 16 x 64 when M <= 16 too; M <= 16:
+// 16 x 64; else 64 x 64).  The kernel is persistent: at most GEMM_MAX_GRID
+// workgroups loop over the tiles.
 static constexpr int GEMM_VARIANTS = 3;
+static constexpr int GEMM_BY_SHAPE = -2;
 static constexpr int GEMM_MAX_GRID = 4096;
 int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
              const char *name, int variant = 0);

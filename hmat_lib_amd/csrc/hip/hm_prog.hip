@@ -150,6 +150,14 @@ static const int kTileTN[GEMM_VARIANTS] = {GTile0::TN, GTile1::TN, GTile2::TN};
 
 int gemm_add(Program &p, Arena &ar, std::vector<GemmJob> jobs, bool ta, bool tb,
              const char *name, int variant) {
+  if (variant == GEMM_BY_SHAPE) {
+    int maxm = 0, maxn = 0;
+    for (const GemmJob &J : jobs) {
+      maxm = std::max(maxm, J.M);
+      maxn = std::max(maxn, J.N);
+    }
+    variant = maxn <= 16 ? (maxm <= 16 ? 2 : 1) : (maxm <= 16 ? 2 : 0);
+  }
   std::vector<int> split, live;
   long long max_e = 0;
   for (size_t q = 0; q < jobs.size(); q++) {

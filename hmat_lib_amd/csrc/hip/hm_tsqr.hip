@@ -737,18 +737,18 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
         }
       }
     }
-    HM_OK(gemm_add(prog, ar, g1, true, false, "qr proj1"));
-    HM_OK(gemm_add(prog, ar, g2, false, false, "qr upd1"));
+    HM_OK(gemm_add(prog, ar, g1, true, false, "qr proj1", GEMM_BY_SHAPE));
+    HM_OK(gemm_add(prog, ar, g2, false, false, "qr upd1", GEMM_BY_SHAPE));
     HM_OK(tsqr_add(prog, ar, t1));
-    HM_OK(gemm_add(prog, ar, g3, true, false, "qr proj2"));
-    HM_OK(gemm_add(prog, ar, g4, false, false, "qr upd2"));
-    HM_OK(gemm_add(prog, ar, g3b, true, false, "qr proj3"));
-    HM_OK(gemm_add(prog, ar, g4b, false, false, "qr upd3"));
+    HM_OK(gemm_add(prog, ar, g3, true, false, "qr proj2", GEMM_BY_SHAPE));
+    HM_OK(gemm_add(prog, ar, g4, false, false, "qr upd2", GEMM_BY_SHAPE));
+    HM_OK(gemm_add(prog, ar, g3b, true, false, "qr proj3", GEMM_BY_SHAPE));
+    HM_OK(gemm_add(prog, ar, g4b, false, false, "qr upd3", GEMM_BY_SHAPE));
     HM_OK(tsqr_add(prog, ar, t2));
     HM_OK(copy2d_add(prog, cp));
-    HM_OK(gemm_add(prog, ar, g5, false, false, "qr r12"));
-    HM_OK(gemm_add(prog, ar, g5b, false, false, "qr r12"));  // same output: own launch
-    HM_OK(gemm_add(prog, ar, g6, false, false, "qr r22"));
+    HM_OK(gemm_add(prog, ar, g5, false, false, "qr r12", GEMM_BY_SHAPE));
+    HM_OK(gemm_add(prog, ar, g5b, false, false, "qr r12", GEMM_BY_SHAPE));  // same output: own launch
+    HM_OK(gemm_add(prog, ar, g6, false, false, "qr r22", GEMM_BY_SHAPE));
   }
   return SUCCESS;
 }

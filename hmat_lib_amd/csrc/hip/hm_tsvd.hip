@@ -882,8 +882,8 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       }
       HM_OK(qr_add(pg, ar, qy));
       for (int rep = 0; rep < 2; rep++) {  // residual of the test columns, twice
-        HM_OK(gemm_add(pg, ar, t1, true, false, "test proj"));
-        HM_OK(gemm_add(pg, ar, t2, false, false, "test upd"));
+        HM_OK(gemm_add(pg, ar, t1, true, false, "test proj", GEMM_BY_SHAPE));
+        HM_OK(gemm_add(pg, ar, t2, false, false, "test upd", GEMM_BY_SHAPE));
       }
       HM_OK(colnorm_add(pg, cn));
       pg.add("project", [=](hipStream_t st, const unsigned char *) {
@@ -960,11 +960,11 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
                   reinterpret_cast<char *>(d_blocks + i) + offsetof(TsvdBlock, perr));
               pn.push_back({Ut, b.m, b.m, b.ntest, perr, Wt, b.n, b.n, 0});
             }
-            HM_OK(gemm_add(pg, ar, p1, true, false, "power At"));
-            HM_OK(gemm_add(pg, ar, p2, false, false, "power A"));
+            HM_OK(gemm_add(pg, ar, p1, true, false, "power At", GEMM_BY_SHAPE));
+            HM_OK(gemm_add(pg, ar, p2, false, false, "power A", GEMM_BY_SHAPE));
             for (int rep = 0; rep < 2; rep++) {
-              HM_OK(gemm_add(pg, ar, p3, true, false, "power proj"));
-              HM_OK(gemm_add(pg, ar, p4, false, false, "power upd"));
+              HM_OK(gemm_add(pg, ar, p3, true, false, "power proj", GEMM_BY_SHAPE));
+              HM_OK(gemm_add(pg, ar, p4, false, false, "power upd", GEMM_BY_SHAPE));
             }
             return colnorm_add(pg, pn);
           },
@@ -1049,7 +1049,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         HM_OK(program_exec(
             [&](Program &pg, Arena &ar) {
               pg.add("form", [](hipStream_t st, const unsigned char *) {});
-              return gemm_add(pg, ar, gu, false, false, "tsvd form");
+              return gemm_add(pg, ar, gu, false, false, "tsvd form", GEMM_BY_SHAPE);
             },
             fin, s));
       }
@@ -1253,7 +1253,7 @@ static int recompress_qr(int nn, const int *m, const int *n, const int *sr,
                                  {bf.G, (size_t)mtot * 8 + 16}, {bf.Z, (size_t)mtot * 8 + 16},
                                  {uo, (size_t)ut * 8 + 16}, {vo, (size_t)vt * 8 + 16}};
     HM_OK(program_exec([&](Program &pg, Arena &ar) {
-      return gemm_add(pg, ar, gu, false, false, "recompress form");
+      return gemm_add(pg, ar, gu, false, false, "recompress form", GEMM_BY_SHAPE);
     }, fin, s));
   }
   hipLaunchKernelGGL(k_copy_list, dim3((unsigned)outc.size(), 8), dim3(256), 0, s, d_out);
