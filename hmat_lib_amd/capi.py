@@ -153,6 +153,7 @@ def _declare_hip_api(lib):
         "hm_hip_device_count": ([PInt], c_int),
         "hm_hip_last_error": ([], C.c_char_p),
         "hm_hip_synchronize": ([V], c_int),
+        "hm_hip_release_buffers": ([], c_int),
         "hm_hip_malloc": ([POINTER(V), C.c_size_t], c_int),
         "hm_hip_free": ([V], c_int),
         "hm_hip_memcpy_h2d": ([V, V, C.c_size_t, V], c_int),

@@ -196,6 +196,16 @@ int products_diag(hm_dtree *t, int R, const double *X, long x_cs, double *Y, lon
 // for the global load in flight.  Accesses through these helpers are
 // global ones and leave LDS waits alone.
 typedef __attribute__((address_space(1))) double gdbl;
+// Orders one wavefront's LDS writes before its lanes' later LDS reads of
+// them: the wave barrier keeps the scheduler from moving instructions across
+// it, the wavefront-scope release / acquire fences keep the compiler from
+// moving the memory operations themselves.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ gdbl *gmem(double *p) { return (gdbl *)p; }
 __device__ __forceinline__ const gdbl *gmem(const double *p) { return (const gdbl *)p; }
 __device__ __forceinline__ double ldg(const double *p) { return *gmem(p); }

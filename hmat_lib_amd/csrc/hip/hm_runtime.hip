@@ -7,6 +7,7 @@
 #include <string>
 
 #include "hm_common.h"
+#include "hm_tsqr.h"
 
 namespace hm {
 
@@ -147,6 +148,16 @@ int hm_hip_init(int device) {
   if (!g_stream) HM_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_device = device;
   g_inited = 1;
+  return SUCCESS;
+}
+
+int hm_hip_release_buffers(void) {
+  const int a = release_exec_buffers();
+  const int b = release_round_buffer();
+  if (a != SUCCESS || b != SUCCESS) {
+    set_error("hm_hip_release_buffers: a call on another thread holds a kept buffer");
+    return INPUT_ERROR;
+  }
   return SUCCESS;
 }
 

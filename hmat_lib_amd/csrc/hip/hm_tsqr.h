@@ -51,4 +51,11 @@ struct QrJob {
 int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw = 64);
 
 
+// Free the process-wide buffers kept between calls: the launch programs'
+// scratch arena, blob and pinned staging (up to 1 GiB + 256 MiB) and the
+// compression rounds' work buffer (up to 8 GiB).  INPUT_ERROR when one of
+// them is in use by a running call (it is then kept).
+int release_exec_buffers();
+int release_round_buffer();
+
 }  // namespace hm

@@ -256,7 +256,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_narrow(const QrLevel *levels,
       if (lane + 64 < rows) S[lane + 64 + jj * TQ_NLS] = b;
     }
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   double y[TQ_RW];
   {
     const double *sj = S + min(j, nc - 1) * TQ_NLS + rq0;
@@ -269,17 +269,17 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_narrow(const QrLevel *levels,
   const int nref = min(rows, nc);
   for (int k = 0; k < nref; k++) {
     const int kq = k / TQ_RW, kr = k % TQ_RW;
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     if (j == k && q >= kq) {
 #pragma unroll
       for (int r = 0; r < TQ_RW; r++) vb[rq0 + r] = y[r];
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     if (j == k && q == kq) {
       alphaW[w] = vb[k];
       for (int r = 0; r <= kr; r++) vb[rq0 + r] = 0.0;
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     const double a0 = vb[lane], a1 = vb[lane + 64];
     const double x0 = lane > k ? a0 : 0.0, x1 = lane + 64 > k ? a1 : 0.0;
     const double x2 = wave_sum(x0 * x0 + x1 * x1);
@@ -291,14 +291,14 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_narrow(const QrLevel *levels,
       tau = (beta - alpha) / beta;
       scal = 1.0 / (alpha - beta);
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     if (lane == 0) {
       tauW[w][k] = tau;
       scalW[w][k] = scal;
       betaW[w][k] = beta;
       vb[k] = alpha - beta;  // the unit entry of v = scal * vb
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     double v[TQ_RW];
     double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_narrow(const QrLevel *levels,
       for (int r = 0; r < TQ_RW; r++) y[r] -= fs * v[r];
     }
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
 #pragma unroll
   for (int r = 0; r < TQ_RW; r++) {
     const int i = rq0 + r;
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_up_narrow(const QrLevel *levels,
     }
     if (i < TQ_CR) S[i + j * TQ_NLS] = x;
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   {
     double *tau_out = it.tau + (long long)ck.i * nc;
     if (lane < nc) gmem(tau_out)[lane] = lane < nref ? tauW[w][lane] : 0.0;
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_narrow(const QrLevel *level
 #pragma unroll
     for (int r = 0; r < TQ_RW; r++) qv[r] = (rq0 + r == j && j < nq) ? 1.0 : 0.0;
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   for (int k = nref - 1; k >= 0; k--) {
     const double tau = tauW[w][k];
     if (tau == 0.0) continue;
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(TQ_NT) void k_tsqr_down_narrow(const QrLevel *level
       for (int r = 0; r < TQ_RW; r++) qv[r] -= f * v[r];
     }
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   {
     // Q rows of this chunk, one column per 16 lanes' group: straight from
     // registers (lane (q, j) writes its 32 rows of column j)
@@ -788,6 +788,20 @@ int grow_device(T **p, size_t *cap, size_t need) {
   return SUCCESS;
 }
 }  // namespace
+
+int release_exec_buffers() {
+  std::lock_guard<std::mutex> lock(g_exec.mu);
+  if (g_exec.busy) return INPUT_ERROR;
+  dfree(g_exec.arena);
+  dfree(g_exec.blob);
+  if (g_exec.host) (void)hipHostFree(g_exec.host);
+  g_exec.arena = nullptr;
+  g_exec.blob = nullptr;
+  g_exec.host = nullptr;
+  g_exec.arena_cap = g_exec.blob_cap = g_exec.host_cap = 0;
+  g_exec.device = -1;
+  return SUCCESS;
+}
 
 int program_exec(const std::function<int(Program &, Arena &)> &build,
                  const std::vector<MemRange> &inputs, hipStream_t s) {

@@ -680,11 +680,16 @@ struct RoundCache {
 RoundCache g_round;
 constexpr size_t kRoundKeep = (size_t)8 << 30;
 
-// Holds the cached buffer for one round (or owns a buffer of its own).
+// Holds the cached buffer for one round (or owns a buffer of its own).  The
+// destructor waits for the round's stream before the buffer is freed or
+// handed back, so a round left early on an error path (kernels still in
+// flight) never lets another thread reuse memory that is being written.
 struct RoundBuffer {
   char *base = nullptr;
   bool cached = false, own = false;
-  int get(size_t need) {
+  hipStream_t stream = nullptr;
+  int get(size_t need, hipStream_t s) {
+    stream = s;
     int dev = -1;
     if (need <= kRoundKeep && hipGetDevice(&dev) == hipSuccess) {
       std::lock_guard<std::mutex> lock(g_round.mu);
@@ -710,6 +715,7 @@ struct RoundBuffer {
     return SUCCESS;
   }
   ~RoundBuffer() {
+    if (own || cached) (void)hipStreamSynchronize(stream);
     if (own) dfree(base);
     if (cached) {
       std::lock_guard<std::mutex> lock(g_round.mu);
@@ -718,6 +724,21 @@ struct RoundBuffer {
   }
 };
 
+}  // namespace
+
+// Frees the kept round buffer unless a round holds it (then BUSY: returns
+// INPUT_ERROR and keeps it).
+int release_round_buffer() {
+  std::lock_guard<std::mutex> lock(g_round.mu);
+  if (g_round.busy) return INPUT_ERROR;
+  dfree(g_round.buf);
+  g_round.buf = nullptr;
+  g_round.cap = 0;
+  g_round.device = -1;
+  return SUCCESS;
+}
+
+namespace {
 struct Scoped {
   std::vector<void *> ptrs;
   ~Scoped() {
@@ -794,7 +815,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
         offs[q] = need;
         need += ((size_t)sizes[q] * sizeof(double) + 16 + 255) & ~(size_t)255;
       }
-      HM_OK(rbuf.get(need));
+      HM_OK(rbuf.get(need, s));
       for (int q = 0; q < 10; q++) *dst[q] = reinterpret_cast<double *>(rbuf.base + offs[q]);
     }
     TsvdBlock *d_blocks = nullptr;

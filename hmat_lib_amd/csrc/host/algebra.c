@@ -249,6 +249,26 @@ HM_EXPORT int recompress_large_s(struct NodeOffDiagonal *node, const int m_small
   return recompress_host(node, svd_threshold, 1, ierr);
 }
 
+/* Reference: src/hodlr_algebra.c:1356-1374 (non-static there, documented in
+ * docs/source/api/hodlr_algebra.rst:7, no header declaration).  The sizes of
+ * the two workspaces the reference's multiply_hodlr_hodlr allocates:
+ * size1 = s1 * s2 and size2 = s1 * (largest diagonal leaf of hodlr1), s1, s2
+ * the largest ranks of the two trees.  The device product needs no caller
+ * workspace; the query is kept so callers that size buffers with it link. */
+HM_EXPORT void compute_workspace_multiply_hodlr_hodlr(
+    const struct TreeHODLR *const hodlr1, const struct TreeHODLR *const hodlr2,
+    unsigned int *restrict const size1, unsigned int *restrict const size2) {
+  const int s1 = get_highest_s(hodlr1);
+  const int s2 = get_highest_s(hodlr2);
+  *size1 = (unsigned int)(s1 * s2);
+  int largest_m = 0;
+  for (long c = 0; c < 2 * hodlr1->len_work_queue; c++) {
+    const int m = hodlr1->innermost_leaves[c]->data.diagonal.m;
+    if (m > largest_m) largest_m = m;
+  }
+  *size2 = (unsigned int)(s1 * largest_m);
+}
+
 /* Reference: src/hodlr_algebra.c:1452-1553.  Both inputs come from the side
  * table (uploaded on first use), the product is formed and recompressed on
  * the device, then copied into malloc'd arrays of `out` (which must hold no

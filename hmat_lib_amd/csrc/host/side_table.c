@@ -4,6 +4,7 @@
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "hm_host.h"
 #include "hmat_lib_amd.h"
@@ -43,6 +44,19 @@ static uint64_t fnv(uint64_t h, uint64_t v) {
   return h;
 }
 
+/* Eight entries of an array of `count` doubles, spread over it (first and
+ * last included): a refilled array at a reused address changes them. */
+static uint64_t fnv_samples(uint64_t h, const double *p, long count) {
+  if (!p || count <= 0) return h;
+  for (int k = 0; k < 8; k++) {
+    const long i = (long)((double)(count - 1) * k / 7.0);
+    uint64_t bits;
+    memcpy(&bits, p + i, sizeof(bits));
+    h = fnv(h, bits);
+  }
+  return h;
+}
+
 uint64_t hm_fingerprint(const struct HODLRInternalNode *node, int height) {
   struct HODLRInternalNode **all = hm_levels(node, height);
   if (!all) return 0;
@@ -56,6 +70,8 @@ uint64_t hm_fingerprint(const struct HODLRInternalNode *node, int height) {
       h = fnv(h, (uint64_t)(uintptr_t)o->v);
       h = fnv(h, ((uint64_t)(uint32_t)o->m << 32) | (uint32_t)o->n);
       h = fnv(h, (uint64_t)(uint32_t)o->s);
+      h = fnv_samples(h, o->u, (long)o->m * o->s);
+      h = fnv_samples(h, o->v, (long)o->n * o->s);
     }
   }
   long first_bottom = (1L << (height - 1)) - 1;
@@ -64,6 +80,7 @@ uint64_t hm_fingerprint(const struct HODLRInternalNode *node, int height) {
       const struct NodeDiagonal *d = &all[k]->children[c].leaf->data.diagonal;
       h = fnv(h, (uint64_t)(uintptr_t)d->data);
       h = fnv(h, (uint64_t)(uint32_t)d->m);
+      h = fnv_samples(h, d->data, (long)d->m * d->m);
     }
   }
   free(all);

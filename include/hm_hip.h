@@ -20,7 +20,10 @@
  *     column-major at a per-block offset;
  *   - int metadata (leaf offsets, block dims, ranks, offsets) on device.
  * One tree must not be used by two concurrent calls (its product scratch
- * is owned by the tree).
+ * is owned by the tree): that includes asynchronous calls on two different
+ * streams from one thread, which overlap on the device and race on the
+ * scratch.  Issue every product of one tree on one stream, or order the
+ * streams yourself (an event between them).
  */
 #ifndef HM_HIP_H
 #define HM_HIP_H
@@ -45,6 +48,15 @@ HM_HIP_API int hm_hip_init(int device);
 HM_HIP_API int hm_hip_device_count(int *count);
 HM_HIP_API const char *hm_hip_last_error(void);
 HM_HIP_API int hm_hip_synchronize(void *stream);
+
+/* Free the device and pinned buffers the library keeps between calls so
+ * that later calls need no allocation: the compression rounds' work buffer
+ * (up to 8 GiB, sized by the largest compression so far) and the launch
+ * programs' scratch, blob and pinned staging (up to 1 GiB + 256 MiB).  The
+ * next call that needs them allocates them again.  Product scratch belongs
+ * to each tree and goes with hm_hip_tree_destroy.  INPUT_ERROR (buffers kept)
+ * while another thread's call is using one of them. */
+HM_HIP_API int hm_hip_release_buffers(void);
 
 /* Raw device memory helpers (for callers that bring no allocator). */
 HM_HIP_API int hm_hip_malloc(void **d_ptr, size_t bytes);

@@ -35,6 +35,15 @@ int multiply_hodlr_hodlr(const struct TreeHODLR *hodlr1,
                          struct TreeHODLR *out, const double svd_threshold,
                          int *ierr);
 
+/* Workspace sizes of the reference's multiply_hodlr_hodlr: *size1 =
+ * s1 * s2, *size2 = s1 * (largest diagonal leaf of hodlr1), with s1, s2 the
+ * largest ranks of hodlr1 and hodlr2.  Replaces src/hodlr_algebra.c:
+ * 1356-1374 (exported there without a header declaration). */
+void compute_workspace_multiply_hodlr_hodlr(const struct TreeHODLR *const hodlr1,
+                                            const struct TreeHODLR *const hodlr2,
+                                            unsigned int *const size1,
+                                            unsigned int *const size2);
+
 /* QR method: U = Q_U R_U, V = Q_V R_V, SVD of R_U R_V^T (s x s).
  * Requires node->s < min(node->m, node->n).
  * Replaces src/hodlr_algebra.c:97-211. */

@@ -4,10 +4,14 @@
  * The reference has no device: every call of its API hands over host
  * arrays.  This build keeps that API unchanged (the hmat_lib headers) and
  * keeps, per tree, a device copy in a side table keyed by the node pointer
- * and height.  An entry is reused while the tree's structure fingerprint
- * (node sizes, ranks and the u/v/data pointer values) is unchanged and is
- * dropped by free_tree_data / free_tree_hodlr.  Callers that overwrite
- * u/v/data contents in place must call hmat_amd_invalidate.
+ * and height.  An entry is reused while the tree's fingerprint is unchanged
+ * and is dropped by free_tree_data / free_tree_hodlr.  The fingerprint
+ * covers node sizes, ranks, the u/v/data pointer values and eight sampled
+ * entries of every u/v/data array (first and last included), so replacing
+ * an array (free + malloc, even at the same address) and refilling it is
+ * detected.  Callers that overwrite entries of u/v/data in place must call
+ * hmat_amd_invalidate: an edit that misses the sampled entries is not seen
+ * and the products would use the old device copy.
  *
  * No counterpart in the reference; these are for callers who want the
  * device-resident path (inputs already in HBM).

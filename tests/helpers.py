@@ -179,3 +179,67 @@ def random_flat_tree(rng, leaf_sizes, ranks_fn):
             ft.blocks[(l, j)] = (rng.standard_normal((r1 - r0, s)),
                                  rng.standard_normal((c1 - c0, s)))
     return ft
+
+
+def kernel_rows(kind, pts, ell, shift, rows):
+    """Rows `rows` of the FP64 kernel matrix A(i, j) = k(|p_i - p_j|) +
+    shift * delta_ij, straight from the formula (kind 0 exp(-r/ell),
+    1 1/(r+ell), 2 Matern-3/2)."""
+    p = pts if pts.ndim > 1 else pts[:, None]
+    d = np.sqrt(((p[rows][:, None, :] - p[None, :, :]) ** 2).sum(-1))
+    if kind == 0:
+        a = np.exp(-d / ell)
+    elif kind == 1:
+        a = 1.0 / (d + ell)
+    else:
+        t = np.sqrt(3.0) * d / ell
+        a = (1.0 + t) * np.exp(-t)
+    a[np.arange(len(rows)), rows] += shift
+    return a
+
+
+class Checker:
+    """The reference hmat_lib built from its sources (oracle/_ref) when it
+    was built, else the oracle restatement (bitwise equal to it on every
+    case of test_oracle.py): allocate a host tree, run its products."""
+
+    def __init__(self):
+        if O.have_ref():
+            self.lib, self.kind = O.ref(), "reference"
+        else:
+            self.lib, self.kind = O.oracle(), "oracle"
+
+    def allocate(self, height):
+        ierr = C.c_int(0)
+        if self.kind == "reference":
+            t = self.lib.allocate_tree_monolithic(height, C.byref(ierr))
+        else:
+            t = self.lib.orc_allocate_tree(height, C.byref(ierr))
+        assert ierr.value == 0
+        return t
+
+    def free(self, t):
+        if self.kind == "reference":
+            self.lib.free_tree_hodlr(C.byref(t))
+        else:
+            self.lib.orc_free_tree(t)
+
+    def matmat(self, t, x):
+        x = np.asfortranarray(x, dtype=np.float64)
+        n, r = x.shape
+        out = np.zeros((n, r), order="F")
+        ierr = C.c_int(0)
+        f = (self.lib.multiply_hodlr_dense if self.kind == "reference"
+             else self.lib.orc_multiply_hodlr_dense)
+        f(t, capi.dptr(x), r, n, capi.dptr(out), n, C.byref(ierr))
+        assert ierr.value == 0
+        return out
+
+    def matvec(self, t, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(x.shape[0])
+        ierr = C.c_int(0)
+        f = self.lib.multiply_vector if self.kind == "reference" else self.lib.orc_multiply_vector
+        f(t, capi.dptr(x), capi.dptr(out), C.byref(ierr))
+        assert ierr.value == 0
+        return out

@@ -82,7 +82,9 @@ def test_qr_batch_shapes(gpu):
 @pytest.mark.parametrize("kind", ["zero_col", "repeat_col"])
 def test_qr_degenerate_columns(gpu, kind):
     rng = np.random.default_rng(12)
-    shapes = [(64, 8), (300, 40), (3000, 64)]
+    # (3000, 16) and (1000, 12): narrow panels over many chunks and levels of
+    # the one-wave TSQR kernels (panels of at most 16 columns)
+    shapes = [(64, 8), (300, 40), (3000, 64), (3000, 16), (1000, 12)]
     panels = [_panel(rng, m, n, kind) for m, n in shapes]
     _say(f"qr {kind}")
     for (q, r), y, s in zip(_run(panels), panels, shapes):
