@@ -168,6 +168,8 @@ def _declare_hip_api(lib):
         "hm_hip_tree_ranks": ([V, PInt], c_int),
         "hm_hip_matvec": ([V, V, V, V], c_int),
         "hm_hip_matmat": ([V, c_int, c_int, V, c_long, c_long, V, c_long, c_long, V], c_int),
+        "hm_hip_matmat_noalloc": ([V, c_int, c_int, V, c_long, c_long, V, c_long, c_long, V],
+                                  c_int),
         "hm_hip_compress_opts_default": ([POINTER(CompressOpts)], None),
         "hm_hip_compress_dense": ([V, c_long, c_int, c_int, PInt, c_double,
                                    POINTER(CompressOpts), POINTER(V),

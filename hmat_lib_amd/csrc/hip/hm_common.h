@@ -179,6 +179,8 @@ __host__ __device__ inline int blk_index(int level, int j) { return (1 << level)
 int tree_finalize_meta(hm_dtree *t, hipStream_t s);
 int tree_alloc_pools(hm_dtree *t);
 int ensure_scratch(hm_dtree *t, int R);
+// Doubles of product scratch (partials, W, segment sums) for R columns.
+long long scratch_need(const hm_dtree *t, int R);
 void tree_free(hm_dtree *t);
 void mixed_free(MixedFactors *m);
 // Y = H X with sliced BF16 factors (t->slices > 0); X, Y column-major.

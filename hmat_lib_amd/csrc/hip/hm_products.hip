@@ -18,6 +18,7 @@
 // MFMA tiles (v_mfma_f64_16x16x4_f64) staged through LDS.  The apply tile
 // shape is chosen per tree and RHS width by a small timing autotuner.
 #include <cstdlib>
+#include <mutex>
 
 #include "hm_common.h"
 
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles, int nt
 
 static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
                           long x_rs, long x_cs, double *Y, long y_rs,
-                          long y_cs) {
+                          long y_cs, double *scratch_base = nullptr) {
   ProdArgs a;
   a.h = t->height;
   a.nleaf = t->nleaf;
@@ -429,8 +430,8 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   a.wofs = t->plan.d_wofs;
   long long pmax = std::max(std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back()),
                             std::max(t->plan.wpofs[0].back(), t->plan.wpofs[1].back()));
-  a.P = t->d_scratch;
-  a.W = t->d_scratch + pmax * R;
+  a.P = scratch_base ? scratch_base : t->d_scratch;
+  a.W = a.P + pmax * R;
   a.Wp = a.W + t->plan.wofs[t->nblock] * R;
   a.X = X;
   a.x_rs = x_rs;
@@ -970,8 +971,11 @@ static int r_bucket(int R) { return R <= 16 ? 0 : (R <= 128 ? 1 : 2); }
 
 // Pick the fastest apply variant for this tree, transpose mode and RHS
 // bucket by timing each once (HIP events) on the live operands.  HM_TUNE=0
-// disables tuning (variant 1), HM_APPLY_VARIANT=v forces one.
-static int choose_variant(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_t s) {
+// disables tuning (variant 1), HM_APPLY_VARIANT=v forces one.  With
+// no_tune (the allocation-free product) an untuned tree runs variant 1
+// and nothing is recorded.
+static int choose_variant(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_t s,
+                          bool no_tune = false) {
   int &slot = t->plan.apply_variant[a.trans][r_bucket(a.R)];
   if (slot >= 0) return slot;
   const char *force = getenv("HM_APPLY_VARIANT");
@@ -979,6 +983,7 @@ static int choose_variant(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_
     slot = std::max(0, std::min(kNumVariants - 1, atoi(force)));
     return slot;
   }
+  if (no_tune) return 1;
   const char *tune = getenv("HM_TUNE");
   if (tune && tune[0] == '0') {
     slot = 1;
@@ -1074,8 +1079,20 @@ int products_diag(hm_dtree *t, int R, const double *X, long x_cs, double *Y, lon
   return SUCCESS;
 }
 
+// ext_scratch: product scratch supplied by the caller (the allocation-free
+// entry point), else the tree's own, grown on demand.
+static int products_run_ex(hm_dtree *t, int trans, int R, const double *X, long x_rs,
+                           long x_cs, double *Y, long y_rs, long y_cs, hipStream_t s,
+                           double *ext_scratch);
+
 int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
                  long x_cs, double *Y, long y_rs, long y_cs, hipStream_t s) {
+  return products_run_ex(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs, s, nullptr);
+}
+
+static int products_run_ex(hm_dtree *t, int trans, int R, const double *X, long x_rs,
+                           long x_cs, double *Y, long y_rs, long y_cs, hipStream_t s,
+                           double *ext_scratch) {
   if (!t) {
     set_error("product: NULL device tree");
     return INPUT_ERROR;
@@ -1085,10 +1102,16 @@ int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
     set_error("product: NULL operand");
     return INPUT_ERROR;
   }
-  if (t->slices > 0 && !trans && R > 1 && x_rs == 1 && y_rs == 1)
+  if (t->slices > 0 && !trans && R > 1 && x_rs == 1 && y_rs == 1) {
+    if (ext_scratch) {
+      set_error("allocation-free product: trees with sliced (mixed-precision) factors "
+                "are not supported (hm_hip_tree_set_precision(tree, 0, ...) first)");
+      return INPUT_ERROR;
+    }
     return mixed_matmat(t, R, X, x_cs, Y, y_cs, s);
-  HM_OK(ensure_scratch(t, R));
-  ProdArgs a = make_args(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs);
+  }
+  if (!ext_scratch) HM_OK(ensure_scratch(t, R));
+  ProdArgs a = make_args(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs, ext_scratch);
   const int nchunk = t->plan.nchunk;
   if (R == 1 && !trans) {
     if (nchunk > 0) {
@@ -1122,7 +1145,7 @@ int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
     HM_LAUNCH_CHECK();
     HM_OK(run_reduce(t, a, s));
   }
-  const int v = choose_variant(t, a, nfast, s);
+  const int v = choose_variant(t, a, nfast, s, ext_scratch != nullptr);
   Timed ta = timing_begin("mm_apply", s);
   run_apply(t, a, v, nfast, s);
   timing_end(ta, s);
@@ -1130,11 +1153,74 @@ int products_run(hm_dtree *t, int trans, int R, const double *X, long x_rs,
   return SUCCESS;
 }
 
+// --------------------------------------------- allocation-free products
+// Product scratch of hm_hip_matmat_noalloc: one fixed array per device,
+// 64 MiB, held from the load of this code object until the process ends.
+// Host code never uses its name as an address: the device address comes
+// from hipGetSymbolAddress, looked up once per device with that device
+// current (on the entry point's first call there) and kept per device.
+static constexpr long long kNoAllocScratch = 8LL << 20;  // doubles
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+double *g_scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+static int noalloc_scratch(double **out) {
+  int dev = -1;
+  HM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("allocation-free product: device %d beyond the supported %d", dev, kMaxDevices);
+    return INPUT_ERROR;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("allocation-free product: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                hipGetErrorString(e));
+      return ALLOCATION_FAILURE;
+    }
+    g_scratch_addr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return SUCCESS;
+}
+
+int products_run_noalloc(hm_dtree *t, int trans, int R, const double *X, long x_rs,
+                         long x_cs, double *Y, long y_rs, long y_cs, hipStream_t s) {
+  if (!t) {
+    set_error("product: NULL device tree");
+    return INPUT_ERROR;
+  }
+  if (R <= 0) return SUCCESS;
+  const long long need = scratch_need(t, R);
+  if (need > kNoAllocScratch) {
+    set_error("allocation-free product: needs %lld doubles of scratch for R = %d, more than "
+              "the fixed %lld (64 MiB); use hm_hip_matmat or fewer right-hand sides",
+              need, R, kNoAllocScratch);
+    return INPUT_ERROR;
+  }
+  double *base = nullptr;
+  HM_OK(noalloc_scratch(&base));
+  return products_run_ex(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs, s, base);
+}
+
 }  // namespace hm
 
 using namespace hm;
 
 extern "C" {
+
+int hm_hip_matmat_noalloc(hm_dtree *t, int trans, int R, const double *d_X, long x_rs,
+                          long x_cs, double *d_Y, long y_rs, long y_cs, void *stream) {
+  HM_OK(ensure_init());
+  return products_run_noalloc(t, trans ? 1 : 0, R, d_X, x_rs, x_cs, d_Y, y_rs, y_cs,
+                              stream_of(stream));
+}
 
 int hm_hip_matvec(hm_dtree *t, const double *d_x, double *d_y, void *stream) {
   HM_OK(ensure_init());

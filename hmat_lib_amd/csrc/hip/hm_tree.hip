@@ -267,11 +267,15 @@ int tree_alloc_pools(hm_dtree *t) {
   return SUCCESS;
 }
 
-int ensure_scratch(hm_dtree *t, int R) {
+long long scratch_need(const hm_dtree *t, int R) {
   const ProductPlan &p = t->plan;
   long long need_p = std::max(std::max(p.pofs[0].back(), p.pofs[1].back()),
                               std::max(p.wpofs[0].back(), p.wpofs[1].back()));
-  long long need = (need_p + p.wofs[t->nblock] + p.part_ofs[t->nblock]) * (long long)R + 64;
+  return (need_p + p.wofs[t->nblock] + p.part_ofs[t->nblock]) * (long long)R + 64;
+}
+
+int ensure_scratch(hm_dtree *t, int R) {
+  const long long need = scratch_need(t, R);
   if ((size_t)need <= t->scratch_elems) return SUCCESS;
   dfree(t->d_scratch);
   t->d_scratch = nullptr;

@@ -110,6 +110,26 @@ HM_HIP_API int hm_hip_matmat(hm_dtree *tree, int trans, int R, const double *d_X
                   long x_rs, long x_cs, double *d_Y, long y_rs, long y_cs,
                   void *stream);
 
+/* hm_hip_matmat without any allocation: the same product (same kernels,
+ * same operands and result) with its partial products and W in a fixed
+ * 64 MiB device array of the library (one per device, held until the
+ * process ends) instead of the tree's scratch, which hm_hip_matmat grows
+ * with hipMalloc.  No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Call hm_hip_init first.  It makes
+ * no hipMalloc / hipFree or other allocating call: a tree whose apply
+ * variant for this transpose mode and RHS width is not tuned yet runs
+ * variant 1 (no timing runs); trees with sliced factors
+ * (hm_hip_tree_set_precision > 0) give INPUT_ERROR, and so does a
+ * product whose scratch (partials + W, about R * (sum over levels and
+ * 64-row chunks of the input-side ranks + sum of ranks) doubles) exceeds
+ * 8 Mi doubles.  Every call on one device shares the array and it keeps
+ * the last call's contents (each product writes its scratch before reading
+ * it): calls that overlap on one device, from any thread or stream, race. */
+HM_HIP_API int hm_hip_matmat_noalloc(hm_dtree *tree, int trans, int R, const double *d_X,
+                                     long x_rs, long x_cs, double *d_Y, long y_rs,
+                                     long y_cs, void *stream);
+
 /* Precision of the off-diagonal products of this tree: 0 = FP64 (default);
  * 1..4 = the factors split into that many BF16 slices (integers with a
  * shared power-of-two scale per row of 128-entry chunks), products of
