@@ -51,11 +51,136 @@ def log(msg):
 
 
 def cpu_cores():
+    """Host threads the CPU baselines use: the cores this process may run
+    on, at most 16 (the host share a one-GPU box allots per GPU; the
+    affinity and nproc are reported beside it)."""
     try:
         n = len(os.sched_getaffinity(0))
     except Exception:
         n = os.cpu_count() or 1
     return max(1, min(16, n))
+
+
+def host_info(cores):
+    """nproc, affinity, CPU model and the BLAS/LAPACK the CPU baselines link."""
+    info = {"nproc": os.cpu_count(), "threads_used": cores}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except Exception:
+        pass
+    try:
+        mkl = C.CDLL("libmkl_rt.so.1")
+        buf = C.create_string_buffer(256)
+        mkl.mkl_get_version_string(buf, 256)
+        info["blas"] = buf.value.decode(errors="replace").strip()
+    except Exception as e:
+        info["blas"] = f"unknown ({e})"
+    return info
+
+
+def _set_cpu_threads(n):
+    """Thread count of the host BLAS (MKL) and OpenMP regions."""
+    try:
+        C.CDLL("libmkl_rt.so.1").MKL_Set_Num_Threads(C.c_int(n))
+    except Exception:
+        pass
+    try:
+        C.CDLL("libgomp.so.1").omp_set_num_threads(C.c_int(n))
+    except Exception:
+        pass
+
+
+def compress_cpu_leg(np, cores, budget_s):
+    """The reference's dense_to_tree_hodlr (built from its sources, MKL) on
+    the exp1d kernel (ell 0.05, thr 1e-10, leaf 256) over an N ladder, the
+    power law t = a N^b fitted on it, and its extrapolation to N = 65536
+    (labelled: the reference overflows int for N >= 32768 and would need
+    hours); plus one 1-thread point."""
+    from hmat_lib_amd import capi, workloads as WL
+    from oracle import oracle as O
+    if not O.have_ref():
+        return None
+    lib = O.ref()
+
+    def one(n, threads):
+        h = int(np.log2(n // 256))
+        pts = WL.points_1d_sorted(2002, n)
+        a = np.exp(-np.abs(pts[:, None] - pts[None, :]) / 0.05)
+        a = np.asfortranarray(a)
+        ierr = C.c_int(0)
+        t = lib.allocate_tree_monolithic(h, C.byref(ierr))
+        _set_cpu_threads(threads)
+        t0 = time.perf_counter()
+        lib.dense_to_tree_hodlr(t, n, None, capi.dptr(a), 1e-10, C.byref(ierr))
+        el = time.perf_counter() - t0
+        lib.free_tree_hodlr(C.byref(t))
+        _set_cpu_threads(cores)
+        return el
+
+    ladder, spent = [], 0.0
+    for n in (1024, 2048, 4096, 8192):
+        if n > 1024 and spent + (ladder[-1]["s"] * 10 if ladder else 0) > budget_s:
+            break
+        runs = [one(n, cores) for _ in range(3 if n <= 2048 else 1)]
+        el = float(np.median(runs))
+        spent += sum(runs)
+        ladder.append({"n": n, "height": int(np.log2(n // 256)), "s": el, "threads": cores})
+    x = np.log([r["n"] for r in ladder])
+    y = np.log([r["s"] for r in ladder])
+    b, loga = np.polyfit(x[-3:], y[-3:], 1)
+    one_thread = {"n": 2048, "s": one(2048, 1), "threads": 1}
+    out = {"kind": "reference", "cores": cores,
+           "ladder": ladder, "one_thread": one_thread,
+           "fit": {"a": float(np.exp(loga)), "b": float(b), "points": "last 3 of the ladder"},
+           "extrapolated_s_at_65536": float(np.exp(loga) * 65536.0 ** b),
+           "note": "extrapolated from the ladder by the fitted power law; the reference "
+                   "itself cannot compress N >= 32768 (int overflow) and its full dgesdd "
+                   "of 32768^2 blocks would take hours"}
+    log(f"cpu compression ladder: " + ", ".join(f"N={r['n']} {r['s']:.2f}s" for r in ladder)
+        + f"; fit b={b:.2f}, extrapolated N=65536 {out['extrapolated_s_at_65536']:.0f} s")
+    return out
+
+
+def compress_api_leg(D, np, w, h):
+    """Compression as the drop-in API runs it: the caller's host matrix (a
+    host copy of the device-generated exp1d_64k A) through libhmat_lib's
+    dense_to_tree_hodlr: device allocation, pageable host-to-device copy,
+    device compression, factors copied into the host tree."""
+    from hmat_lib_amd import capi
+    n = w.n
+    pts = w.points()
+    da = D.fill_kernel_matrix(w.kind, pts, w.ell, w.shift)
+    host = np.empty((n, n), order="F")
+    t0 = time.perf_counter()
+    capi.check(capi.hip().hm_hip_memcpy_d2h(host.ctypes.data, da.ptr, host.nbytes, None), "d2h")
+    d2h_s = time.perf_counter() - t0
+    da.free()
+    lib = capi.hmat()
+    ierr = C.c_int(0)
+    runs = []
+    for _ in range(2):
+        t = lib.allocate_tree_monolithic(h, C.byref(ierr))
+        t0 = time.perf_counter()
+        lib.dense_to_tree_hodlr(t, n, None, capi.dptr(host), w.thr, C.byref(ierr))
+        runs.append((time.perf_counter() - t0) * 1e3)
+        capi.check(ierr.value, "dense_to_tree_hodlr")
+        lib.free_tree_hodlr(C.byref(t))
+    del host
+    out = {"ms": runs[-1], "ms_runs": runs, "bytes_h2d": 8 * n * n,
+           "h2d_rate_bound_GBps": 8 * n * n / (runs[-1] * 1e-3) / 1e9,
+           "note": "host A (pageable, 34.4 GB) -> drop-in dense_to_tree_hodlr, H2D and "
+                   "host tree fill included; the host copy itself took "
+                   f"{d2h_s:.1f} s and is not counted"}
+    log(f"compress (API, host A): {runs[-1]:.0f} ms")
+    return out
 
 
 def _dense_rows(kind, pts, ell, shift, rows):
@@ -375,6 +500,8 @@ def main():
     ap.add_argument("--workload", default="exp1d_64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-api-compress", action="store_true",
+                    help="skip the drop-in compression of a host copy of A (34 GB host RAM)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, mixed-precision and learned-sketch legs")
     ap.add_argument("--legs", default="batched,precision,learned_sketch,hh",
@@ -430,6 +557,34 @@ def main():
     info = dt.info()
     log(f"compress {compress_ms:.1f} ms (warm-up run {warm_ms:.1f} ms), rounds "
         f"{st.sketch_rounds}, retried {st.blocks_retried}, ranks {sorted(set(ranks.tolist()))}")
+    # the compression's streaming passes over A: one more compression with a
+    # HIP event pair around every sketch and projection launch
+    D.kernel_timing_reset()
+    D.kernel_timing(True)
+    dt_i = D.DeviceTree.compress(da, n, h, w.thr)
+    D.synchronize()
+    D.kernel_timing(False)
+    kt = D.kernel_timing_dump()
+    dt_i.free()
+    off_bytes = 8 * (n * n - info.diag_elems)  # off-diagonal A, read once per pass
+    passes = {}
+    for key, kname in (("tsvd_sketch", "k_sketch"), ("tsvd_project", "k_project")):
+        if key in kt and kt[key][1]:
+            ms = kt[key][0] / kt[key][1]
+            passes[kname] = {"launches": kt[key][1], "avg_ms": ms,
+                             "algorithmic_bytes": off_bytes,
+                             "achieved": off_bytes / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                             "frac": off_bytes / (ms * 1e-3) / PEAK_HBM}
+    compress_roof = {
+        "bound": "hbm", "peak": PEAK_HBM / 1e9, "unit": "GB/s", "passes": passes,
+        "floor_ms_two_passes": 2 * off_bytes / PEAK_HBM * 1e3,
+        "floor_ms_one_read": off_bytes / PEAK_HBM * 1e3,
+        "frac_of_two_pass_floor": 2 * off_bytes / PEAK_HBM * 1e3 / compress_ms,
+        "frac_of_one_read_floor": off_bytes / PEAK_HBM * 1e3 / compress_ms,
+        "note": "algorithmic bytes per pass 8 (N^2 - sum of leaf m^2): the off-diagonal A "
+                "read once; times by HIP events around each launch"}
+    log("compress passes: " + ", ".join(f"{k} {v['avg_ms']:.2f} ms {v['achieved']:.0f} GB/s"
+                                        for k, v in passes.items()))
 
     # accuracy vs the FP64 dense product on sampled rows
     x = w.rhs_matrix()
@@ -531,6 +686,7 @@ def main():
 
     # CPU baseline: reference multiply_hodlr_dense on the same tree
     cpu = None
+    cpu_mv = None
     ref_same_tree_err = None
     if not args.no_cpu:
         try:
@@ -562,9 +718,41 @@ def main():
                              f"{cores} threads, {t_cpu:.1f} s"}
             log(f"cpu baseline: {cpu['value']:.3e} rows*RHS/s ({calls} calls), "
                 f"GPU vs reference on the same tree: {ref_same_tree_err:.2e}")
+            # matvec: the reference's multiply_vector on the same tree
+            if kind == "reference":
+                xv = np.ascontiguousarray(x[:, 0])
+                outv = np.zeros(n)
+                mv_calls, t_mv = 0, 0.0
+                while t_mv < min(5.0, args.cpu_seconds) and mv_calls < 2000:
+                    tc = time.perf_counter()
+                    lib.multiply_vector(ht, capi.dptr(xv), capi.dptr(outv), C.byref(ierr))
+                    t_mv += time.perf_counter() - tc
+                    mv_calls += 1
+                cpu_mv = {"us": t_mv / mv_calls * 1e6, "cores": cores, "kind": kind,
+                          "sample": f"{mv_calls} calls of hmat_lib multiply_vector on the same "
+                                    f"tree, MKL, {cores} threads",
+                          "gpu_speedup": t_mv / mv_calls * 1e6 / matvec_us}
+                log(f"cpu matvec: {cpu_mv['us']:.0f} us ({mv_calls} calls)")
             release(ht)
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {e}")
+
+    cpu_compress = None
+    if not args.no_cpu:
+        try:
+            cpu_compress = compress_cpu_leg(np, cores, 2 * args.cpu_seconds)
+            if cpu_compress:
+                cpu_compress["gpu_speedup_vs_extrapolated"] = (
+                    cpu_compress["extrapolated_s_at_65536"] * 1e3 / compress_ms
+                    if n == 65536 else None)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu compression baseline skipped: {e}")
+    compress_api = None
+    if not args.no_api_compress:
+        try:
+            compress_api = compress_api_leg(D, np, w, h)
+        except Exception as e:
+            log(f"API-faithful compression skipped: {e}")
 
     extras = {}
     if not args.no_extras and "hh" in args.legs.split(","):
@@ -605,6 +793,11 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "compress_ms": compress_ms,
+        "compress_roofline": compress_roof,
+        "compress_cpu_baseline": cpu_compress,
+        "compress_api": compress_api,
+        "matvec_cpu_baseline": cpu_mv,
+        "host": host_info(cores),
         "compress_stats": {"sketch_rounds": st.sketch_rounds, "retried": st.blocks_retried,
                            "max_sketch": st.max_sketch, "capped": st.fallback_blocks,
                            "max_rank": int(ranks.max()), "min_rank": int(ranks.min())},
