@@ -31,6 +31,7 @@
 //   Y += sum over levels U_b W_b   sliced GEMM with the levels concatenated
 //                                  along K, one pass over Y
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -125,16 +126,19 @@ struct MxItem {
 
 // MX_TM x MX_TN tile per workgroup (2 x 2 waves of 32 x 32), K-steps of
 // 32: each step's S slices of the A rows and B columns go global ->
-// registers (prefetched one step ahead) -> LDS, and every wave reads its
-// MFMA fragments from LDS.  LDS rows are padded to MX_RS entries.
+// registers (fetched one step ahead) -> one of two LDS stages, and every
+// wave reads its MFMA fragments from LDS; the stage written during a step is
+// read in the next, so one barrier per step orders both.  The row and
+// column scales of a chunk are loaded into registers when the chunk starts
+// and used by its fold.  LDS rows are padded to MX_RS entries.
 static constexpr int MX_RS = 40;
 static constexpr int MX_TM = 64, MX_TN = 64;
 
 template <int S>
 __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const MxSeg *segs) {
   constexpr int FM = 2, FN = 2;  // 16 x 16 blocks per wave
-  __shared__ __attribute__((aligned(16))) __bf16 As[S][MX_TM * MX_RS];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[S][MX_TN * MX_RS];
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][S][MX_TM * MX_RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][S][MX_TN * MX_RS];
   const MxItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
   const int L = tid & 63, w = tid >> 6;
@@ -163,6 +167,28 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
         rb[a] = ldB ? ldg8(gb + a * sd.b_slice + k0) : bf8{};
       }
     };
+    auto stage = [&](int buf) {
+#pragma unroll
+      for (int a = 0; a < S; a++) {
+        *reinterpret_cast<bf8 *>(&As[buf][a][lr * MX_RS + kq]) = ra0[a];
+        *reinterpret_cast<bf8 *>(&Bs[buf][a][lr * MX_RS + kq]) = rb[a];
+      }
+    };
+    double sa[FM][4], sb[FN];
+    auto fetch_scales = [&](int c) {
+#pragma unroll
+      for (int f = 0; f < FM; f++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int row = it.i0 + wm * 32 + f * 16 + 4 * lk + v;
+          sa[f][v] = ldg(sd.sA + (long long)(sd.a_c0 + c) * sd.ldsa + (row < it.M ? row : it.i0));
+        }
+#pragma unroll
+      for (int g = 0; g < FN; g++) {
+        const int col = it.j0 + wn * 32 + g * 16 + li;
+        sb[g] = ldg(sd.sB + (long long)(sd.b_c0 + c) * sd.ldsb + (col < it.N ? col : it.j0));
+      }
+    };
     const int kbeg = ldgi(sd.cs), kend = ldgi(sd.cs + sd.nch);
     int c = 0, cend = ldgi(sd.cs + 1);
     f4 acc[S][FM][FN];
@@ -173,27 +199,28 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
 #pragma unroll
         for (int g = 0; g < FN; g++) acc[d][f][g] = f4{0.f, 0.f, 0.f, 0.f};
     fetch(kbeg);
+    fetch_scales(0);
+    __syncthreads();  // the previous segment's last stage has been read
+    stage(0);
+    __syncthreads();
+    int buf = 0;
     for (int k0 = kbeg; k0 < kend; k0 += 32) {
-      __syncthreads();
-#pragma unroll
-      for (int a = 0; a < S; a++) {
-        *reinterpret_cast<bf8 *>(&As[a][lr * MX_RS + kq]) = ra0[a];
-        *reinterpret_cast<bf8 *>(&Bs[a][lr * MX_RS + kq]) = rb[a];
-      }
-      __syncthreads();
-      if (k0 + 32 < kend) fetch(k0 + 32);
+      const bool more = k0 + 32 < kend;
+      if (more) fetch(k0 + 32);
       bf8 fb[S][FN];
 #pragma unroll
       for (int a = 0; a < S; a++)
 #pragma unroll
         for (int g = 0; g < FN; g++)
-          fb[a][g] = *reinterpret_cast<const bf8 *>(&Bs[a][(wn * 32 + g * 16 + li) * MX_RS + 8 * lk]);
+          fb[a][g] =
+              *reinterpret_cast<const bf8 *>(&Bs[buf][a][(wn * 32 + g * 16 + li) * MX_RS + 8 * lk]);
 #pragma unroll
       for (int f = 0; f < FM; f++) {
         bf8 fa[S];
 #pragma unroll
         for (int a = 0; a < S; a++)
-          fa[a] = *reinterpret_cast<const bf8 *>(&As[a][(wm * 32 + f * 16 + li) * MX_RS + 8 * lk]);
+          fa[a] =
+              *reinterpret_cast<const bf8 *>(&As[buf][a][(wm * 32 + f * 16 + li) * MX_RS + 8 * lk]);
 #pragma unroll
         for (int a = 0; a < S; a++)
 #pragma unroll
@@ -210,14 +237,10 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
           for (int g = 0; g < FN; g++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-              const int row = it.i0 + wm * 32 + f * 16 + 4 * lk + v;
-              const int col = it.j0 + wn * 32 + g * 16 + li;
               double t = 0.0;
 #pragma unroll
               for (int d = S - 1; d >= 0; d--) t = t * 0x1p-8 + (double)acc[d][f][g][v];
-              if (row < it.M && col < it.N)
-                acc64[f][g][v] += ldg(sd.sA + (long long)(sd.a_c0 + c) * sd.ldsa + row) *
-                                  ldg(sd.sB + (long long)(sd.b_c0 + c) * sd.ldsb + col) * 0x1p-14 * t;
+              acc64[f][g][v] += sa[f][v] * sb[g] * 0x1p-14 * t;
             }
 #pragma unroll
         for (int d = 0; d < S; d++)
@@ -226,8 +249,14 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
 #pragma unroll
             for (int g = 0; g < FN; g++) acc[d][f][g] = f4{0.f, 0.f, 0.f, 0.f};
         c++;
-        cend = c < sd.nch ? ldgi(sd.cs + c + 1) : 0;
+        if (c < sd.nch) {
+          cend = ldgi(sd.cs + c + 1);
+          fetch_scales(c);
+        }
       }
+      if (more) stage(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
     }
   }
 #pragma unroll
@@ -441,6 +470,15 @@ static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
   return SUCCESS;
 }
 
+// The sliced GEMM over nitems work items.
+static void launch_sliced(int S, unsigned nitems, const MxItem *I, const MxSeg *G,
+                          hipStream_t st) {
+  if (S == 1) hipLaunchKernelGGL(k_sliced_gemm<1>, dim3(nitems), dim3(256), 0, st, I, G);
+  if (S == 2) hipLaunchKernelGGL(k_sliced_gemm<2>, dim3(nitems), dim3(256), 0, st, I, G);
+  if (S == 3) hipLaunchKernelGGL(k_sliced_gemm<3>, dim3(nitems), dim3(256), 0, st, I, G);
+  if (S == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nitems), dim3(256), 0, st, I, G);
+}
+
 // Chunk tables of program-built segments live in the program's own blob,
 // whose device address is known only at launch: fix[i] >= 0 is the byte
 // offset of segment i's table in the blob.
@@ -646,10 +684,7 @@ static int mixed_build(Program &p, Arena &ar, hm_dtree *t, int R, const double *
     if (!nwit) return;
     const MxItem *I = reinterpret_cast<const MxItem *>(blob + wit_off);
     const MxSeg *G = reinterpret_cast<const MxSeg *>(blob + seg_off);
-    if (S_ == 1) hipLaunchKernelGGL(k_sliced_gemm<1>, dim3(nwit), dim3(256), 0, st, I, G);
-    if (S_ == 2) hipLaunchKernelGGL(k_sliced_gemm<2>, dim3(nwit), dim3(256), 0, st, I, G);
-    if (S_ == 3) hipLaunchKernelGGL(k_sliced_gemm<3>, dim3(nwit), dim3(256), 0, st, I, G);
-    if (S_ == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nwit), dim3(256), 0, st, I, G);
+    launch_sliced(S_, nwit, I, G, st);
   });
   p.add("mixed W reduce", [=](hipStream_t st, const unsigned char *blob) {
     if (nred)
@@ -665,10 +700,7 @@ static int mixed_build(Program &p, Arena &ar, hm_dtree *t, int R, const double *
     if (!nait) return;
     const MxItem *I = reinterpret_cast<const MxItem *>(blob + ait_off);
     const MxSeg *G = reinterpret_cast<const MxSeg *>(blob + seg_off);
-    if (S_ == 1) hipLaunchKernelGGL(k_sliced_gemm<1>, dim3(nait), dim3(256), 0, st, I, G);
-    if (S_ == 2) hipLaunchKernelGGL(k_sliced_gemm<2>, dim3(nait), dim3(256), 0, st, I, G);
-    if (S_ == 3) hipLaunchKernelGGL(k_sliced_gemm<3>, dim3(nait), dim3(256), 0, st, I, G);
-    if (S_ == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nait), dim3(256), 0, st, I, G);
+    launch_sliced(S_, nait, I, G, st);
   });
   p.touch(X, ((size_t)(R - 1) * ldx + t->n) * sizeof(double), "mixed X");
   p.touch(Y, ((size_t)(R - 1) * ldy + t->n) * sizeof(double), "mixed Y");
