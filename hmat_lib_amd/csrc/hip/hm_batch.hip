@@ -24,6 +24,7 @@
 
 struct hm_batch {
   std::vector<hm_dtree *> trees;
+  std::vector<const MixedFactors *> mixed;  // each tree's sliced factors at creation
   int trans = 0;
   hm::Program prog;
   char *arena = nullptr;
@@ -102,19 +103,59 @@ void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, cons
   }
 }
 
+// Y_c = op(D_c) X_c for every leaf of H (chains of one link): the FP64
+// diagonal part of a tree whose off-diagonal products are sliced.
+static void diag_chains(const hm_dtree *H, int trans, const double *X, long long ldx, int R,
+                        double *Y, long long ldy, std::vector<GemmChain> &chains,
+                        std::vector<GemmLink> &links, bool rowmajor) {
+  if (R <= 0) return;
+  for (int c = 0; c < H->nleaf; c++) {
+    const int lo = H->leaf_off[c], mc = H->leaf_off[c + 1] - lo;
+    if (mc <= 0) continue;
+    GemmChain ch;
+    ch.C = rowmajor ? Y + (long long)lo * ldy : Y + lo;
+    ch.ldc = ldy;
+    ch.M = mc;
+    ch.N = R;
+    ch.tc = rowmajor ? 1 : 0;
+    ch.link0 = (int)links.size();
+    GemmLink dl{H->d_diag + H->diag_off[c], rowmajor ? X + (long long)lo * ldx : X + lo, mc,
+                ldx, mc, trans ? 1 : 0};
+    dl.tb = rowmajor ? 1 : 0;
+    links.push_back(dl);
+    ch.nlink = 1;
+    chains.push_back(ch);
+  }
+}
+
 // trans: 0 Y = H X, 1 Y = H^T X, 2 Y = X H (X, Y: R x n), taken as
-// Y^T = H^T X^T.
+// Y^T = H^T X^T.  Trees with sliced factors (hm_hip_tree_set_precision)
+// take their diagonal from the FP64 chains and their off-diagonal blocks
+// from the sliced stages, which run after the chains, one launch per
+// stage for all of them.
 static int build_batch(Program &p, Arena &ar, const std::vector<hm_dtree *> &trees, int trans,
                        const std::vector<BatchOperands> &ops) {
   std::vector<GemmJob> wj;
   std::vector<GemmChain> chains;
   std::vector<GemmLink> links;
+  std::vector<MixedJob> mj;
   const bool rowmajor = trans == 2;
-  for (size_t i = 0; i < trees.size(); i++)
-    subtree_product(ar, trees[i], 0, 0, trans ? 1 : 0, ops[i].X, ops[i].ldx, ops[i].R,
-                    ops[i].Y, ops[i].ldy, wj, chains, links, rowmajor);
+  for (size_t i = 0; i < trees.size(); i++) {
+    const BatchOperands &o = ops[i];
+    if (trees[i]->slices > 0) {
+      diag_chains(trees[i], trans ? 1 : 0, o.X, o.ldx, o.R, o.Y, o.ldy, chains, links,
+                  rowmajor);
+      if (o.R > 0)
+        mj.push_back({trees[i], trans ? 1 : 0, o.R, o.X, rowmajor ? o.ldx : 1,
+                      rowmajor ? 1 : o.ldx, o.Y, rowmajor ? o.ldy : 1, rowmajor ? 1 : o.ldy});
+    } else {
+      subtree_product(ar, trees[i], 0, 0, trans ? 1 : 0, o.X, o.ldx, o.R, o.Y, o.ldy, wj,
+                      chains, links, rowmajor);
+    }
+  }
   HM_OK(gemm_add(p, ar, wj, true, rowmajor, "batch W", -1));
   HM_OK(gemm_chain_add(p, chains, links, "batch apply", -1));
+  if (!mj.empty()) HM_OK(mixed_add(p, ar, mj));
   return SUCCESS;
 }
 
@@ -159,6 +200,7 @@ int hm_hip_batch_create(int ntrees, hm_dtree *const *trees, int trans, const int
     inputs.push_back({t->d_U, (size_t)u * sizeof(double)});
     inputs.push_back({t->d_V, (size_t)v * sizeof(double)});
     inputs.push_back({t->d_diag, (size_t)t->diag_off[t->nleaf] * sizeof(double)});
+    if (t->slices > 0) HM_OK(mixed_ensure_mode(tv[i], trans ? 1 : 0, stream_of(nullptr)));
   }
   // counting pass + audit, then the real program on its own arena
   Program dry;
@@ -174,6 +216,7 @@ int hm_hip_batch_create(int ntrees, hm_dtree *const *trees, int trans, const int
   }
   hm_batch *b = new hm_batch();
   b->trees = tv;
+  for (hm_dtree *t : tv) b->mixed.push_back(t->mixed);
   b->trans = trans;
   Arena ar;
   ar.counting = false;
@@ -218,6 +261,12 @@ int hm_hip_batch_tune(hm_batch *b, void *stream) {
 int hm_hip_batch_run(hm_batch *b, int use_graph, void *stream) {
   if (!b) return INPUT_ERROR;
   HM_OK(ensure_init());
+  for (size_t i = 0; i < b->trees.size(); i++)
+    if (b->trees[i]->mixed != b->mixed[i]) {
+      set_error("hm_hip_batch_run: tree %zu changed precision after the batch was created "
+                "(create a new batch)", i);
+      return INPUT_ERROR;
+    }
   hipStream_t s = stream_of(stream);
   if (!use_graph) return b->prog.run(s);
   if (!b->gexec) {

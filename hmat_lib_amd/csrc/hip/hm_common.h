@@ -183,12 +183,15 @@ int ensure_scratch(hm_dtree *t, int R);
 long long scratch_need(const hm_dtree *t, int R);
 void tree_free(hm_dtree *t);
 void mixed_free(MixedFactors *m);
-// Y = H X with sliced BF16 factors (t->slices > 0); X, Y column-major.
-int mixed_matmat(hm_dtree *t, int R, const double *X, long ldx, double *Y, long ldy,
-                 hipStream_t s);
-// the FP64 diagonal part only: Y = D X
-int products_diag(hm_dtree *t, int R, const double *X, long x_cs, double *Y, long y_cs,
-                  hipStream_t s);
+// Y = op(H) X with sliced BF16 factors (t->slices > 0; op = transpose
+// when trans); X, Y element (row, col) at [row * rs + col * cs].
+int mixed_matmat(hm_dtree *t, int trans, int R, const double *X, long x_rs, long x_cs,
+                 double *Y, long y_rs, long y_cs, hipStream_t s);
+// the sliced factors of product mode trans exist (sliced on first use)
+int mixed_ensure_mode(hm_dtree *t, int trans, hipStream_t s);
+// the FP64 diagonal part only: Y = op(D) X
+int products_diag(hm_dtree *t, int trans, int R, const double *X, long x_rs, long x_cs,
+                  double *Y, long y_rs, long y_cs, hipStream_t s);
 
 // ------------------------------------------------- global memory access
 // Kernels reach global memory through pointers read from work-item tables

@@ -119,8 +119,8 @@ struct MxSeg {
 // One work item: a 64 x 64 tile of C (rows i0.., cols j0.., within M x N)
 // accumulated over segments [s0, s1).  mode 0: C = acc; 1: C += acc.
 struct MxItem {
-  double *C;
-  long long ldc;
+  double *C;          // element (row, col) at C[row * crs + col * ldc]
+  long long ldc, crs;
   int M, N, i0, j0, s0, s1, mode, pad;
 };
 
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
         const int row = it.i0 + wm * 32 + f * 16 + 4 * lk + v;
         const int col = it.j0 + wn * 32 + g * 16 + li;
         if (row < it.M && col < it.N) {
-          double *cp = it.C + row + (long long)col * it.ldc;
+          double *cp = it.C + (long long)row * it.crs + (long long)col * it.ldc;
           stg(cp, it.mode ? ldg(cp) + acc64[f][g][v] : acc64[f][g][v]);
         }
       }
@@ -295,35 +295,43 @@ __global__ __launch_bounds__(256) void k_mx_reduce(const MxReduce *items) {
 }  // namespace hm
 
 // ---------------------------------------------------------- tree factors
+// Per tree: the K-chunk grid over the rows (chunks of <= MX_KCH rows inside
+// leaves, shared by both modes) and, per product mode (0: Y = H X, 1:
+// Y = H^T X), the sliced factors: the W side F_in^T (V^T for H X, U^T for
+// H^T X) sliced along the rows of its source node by that grid, the apply
+// side F_out (U for H X, V for H^T X) sliced along the rank by chunks of
+// MX_KCH.  Mode 1 is sliced on its first use.
 struct MixedFactors {
   int S = 0;
-  // global K-chunk grid over the rows: chunks of <= MX_KCH rows inside
-  // leaves; chunk q covers rows [row[q], row[q] + len[q]) and padded
-  // coordinates [pad[q], pad[q + 1])
+  // chunk q covers rows [row[q], row[q] + len[q]) and padded coordinates
+  // [pad[q], pad[q + 1])
   std::vector<int> row, len, leaf_chunk0;
   std::vector<long long> pad;
   long long npad = 0;
-  // per block: V^T slices (A of W = V^T X), U slices (A of Y += U W)
   struct Blk {
-    long long v_off = 0, v_kp = 0;  // Vs: [S][s][v_kp] at v_off; scales [nch_node][s]
-    long long v_soff = 0;
-    int q0 = 0, q1 = 0;             // chunk range of the column node
-    long long u_off = 0;            // Us: [S][m][s_pad]; scales [nrk][m]
-    long long u_soff = 0;
-    int s_pad = 0, nrk = 0;
-    long long rk_cs = 0;            // offset of the rank chunk table (nrk + 1 ints)
+    long long w_off = 0, w_kp = 0;  // F_in^T slices [S][s][w_kp] at w_off
+    long long w_soff = 0;           // their scales [node chunk][s]
+    int q0 = 0, q1 = 0;             // chunk range of the source node
+    long long a_off = 0;            // F_out slices [S][rows][s_pad]
+    long long a_soff = 0;           // their scales [rank chunk][rows]
+    int rows = 0;                   // rows of F_out
   };
-  std::vector<Blk> blk;
-  __bf16 *d_vs = nullptr, *d_us = nullptr;
-  double *d_sv = nullptr, *d_su = nullptr;
-  int *d_rk = nullptr;              // rank chunk tables
-  // cached product program (operands fixed)
+  struct Mode {
+    bool ready = false;
+    std::vector<Blk> blk;
+    __bf16 *d_ws = nullptr, *d_as = nullptr;
+    double *d_sw = nullptr, *d_sa = nullptr;
+  } mode[2];
+  std::vector<int> s_pad, nrk;  // per block: rank padded to 32, rank chunks
+  std::vector<long long> rk_cs;  // per block: offset of its rank chunk table
+  int *d_rk = nullptr;           // rank chunk tables (nrk + 1 ints per block)
+  // cached single-tree product program (operands fixed)
   hm::Program *prog = nullptr;
   char *arena = nullptr;
-  int R = 0;
+  int trans = -1, R = 0;
   const double *X = nullptr;
   double *Y = nullptr;
-  long ldx = 0, ldy = 0;
+  long x_rs = 0, x_cs = 0, y_rs = 0, y_cs = 0;
 };
 
 namespace hm {
@@ -331,10 +339,12 @@ namespace hm {
 void mixed_free(MixedFactors *m) {
   if (!m) return;
   (void)hipDeviceSynchronize();
-  dfree(m->d_vs);
-  dfree(m->d_us);
-  dfree(m->d_sv);
-  dfree(m->d_su);
+  for (auto &md : m->mode) {
+    dfree(md.d_ws);
+    dfree(md.d_as);
+    dfree(md.d_sw);
+    dfree(md.d_sa);
+  }
   dfree(m->d_rk);
   delete m->prog;
   dfree(m->arena);
@@ -342,6 +352,106 @@ void mixed_free(MixedFactors *m) {
 }
 
 static long long pad32(long long x) { return (x + 31) & ~31LL; }
+
+static int run_slices(const std::vector<SliceItem> &items, int S, hipStream_t s) {
+  if (items.empty()) return SUCCESS;
+  SliceItem *d_items = nullptr;
+  HM_OK(dalloc(&d_items, items.size()));
+  hipError_t e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(SliceItem),
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    const int n = (int)items.size();
+    hipLaunchKernelGGL(k_slice, dim3((n + 3) / 4), dim3(256), 0, s, d_items, n, S);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(d_items);
+  if (e != hipSuccess) {
+    set_error("mixed: slicing factors: %s", hipGetErrorString(e));
+    return ALLOCATION_FAILURE;
+  }
+  return SUCCESS;
+}
+
+// Slice the factors of product mode tr (0: H X, 1: H^T X).
+static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
+  MixedFactors::Mode &md = m->mode[tr];
+  const int S = m->S;
+  md.blk.assign(t->nblock, MixedFactors::Blk());
+  long long woff = 0, wsoff = 0, aoff = 0, asoff = 0;
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    MixedFactors::Blk &x = md.blk[g];
+    const int span = 1 << (t->height - b.level);
+    const int src = tr ? b.j : (b.j ^ 1);
+    x.q0 = m->leaf_chunk0[src * span];
+    x.q1 = m->leaf_chunk0[(src + 1) * span];
+    x.w_kp = m->pad[x.q1] - m->pad[x.q0];
+    x.w_off = woff;
+    woff += (long long)S * b.s * x.w_kp;
+    x.w_soff = wsoff;
+    wsoff += (long long)(x.q1 - x.q0) * b.s;
+    x.rows = tr ? b.n : b.m;
+    x.a_off = aoff;
+    aoff += (long long)S * x.rows * m->s_pad[g];
+    x.a_soff = asoff;
+    asoff += (long long)m->nrk[g] * x.rows;
+  }
+  HM_OK(dalloc(&md.d_ws, (size_t)woff + 8));
+  HM_OK(dalloc(&md.d_as, (size_t)aoff + 8));
+  HM_OK(dalloc(&md.d_sw, (size_t)wsoff + 1));
+  HM_OK(dalloc(&md.d_sa, (size_t)asoff + 1));
+  std::vector<SliceItem> items;
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    const MixedFactors::Blk &x = md.blk[g];
+    if (b.s <= 0) continue;
+    // W side: row r of F_in^T = column r of F_in (V: n x s, U: m x s),
+    // K = rows of the source node
+    const double *fin = tr ? t->d_U + b.uoff : t->d_V + b.voff;
+    const int fin_ld = tr ? b.m : b.n;
+    const int node_lo = tr ? b.r0 : b.c0;
+    for (int r = 0; r < b.s; r++)
+      for (int q = x.q0; q < x.q1; q++) {
+        SliceItem it;
+        it.src = fin;
+        it.sp = fin_ld;
+        it.sk = 1;
+        it.dst = md.d_ws + x.w_off;
+        it.dst_slice = (long long)b.s * x.w_kp;
+        it.dst_ld = x.w_kp;
+        it.dk0 = m->pad[q] - m->pad[x.q0];
+        it.scale = md.d_sw + x.w_soff + (long long)(q - x.q0) * b.s + r;
+        it.p = r;
+        it.k0 = m->row[q] - node_lo;
+        it.klen = m->len[q];
+        it.pad = 0;
+        items.push_back(it);
+      }
+    // apply side: row i of F_out, K = rank
+    const double *fout = tr ? t->d_V + b.voff : t->d_U + b.uoff;
+    for (int i = 0; i < x.rows; i++)
+      for (int c = 0; c < m->nrk[g]; c++) {
+        SliceItem it;
+        it.src = fout;
+        it.sp = 1;
+        it.sk = x.rows;
+        it.dst = md.d_as + x.a_off;
+        it.dst_slice = (long long)x.rows * m->s_pad[g];
+        it.dst_ld = m->s_pad[g];
+        it.dk0 = (long long)c * MX_KCH;
+        it.scale = md.d_sa + x.a_soff + (long long)c * x.rows + i;
+        it.p = i;
+        it.k0 = c * MX_KCH;
+        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
+        it.pad = 0;
+        items.push_back(it);
+      }
+  }
+  HM_OK(run_slices(items, S, s));
+  md.ready = true;
+  return SUCCESS;
+}
 
 static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
   MixedFactors *m = new MixedFactors();
@@ -360,36 +470,17 @@ static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
   }
   m->leaf_chunk0[nl] = (int)m->row.size();
   m->npad = m->pad.back();
-  m->blk.resize(t->nblock);
-  long long voff = 0, vsoff = 0, uoff = 0, usoff = 0, rk = 0;
   std::vector<int> rktab;
+  m->s_pad.assign(t->nblock, 0);
+  m->nrk.assign(t->nblock, 0);
+  m->rk_cs.assign(t->nblock, 0);
   for (int g = 0; g < t->nblock; g++) {
-    const BlockMeta &b = t->blk[g];
-    MixedFactors::Blk &x = m->blk[g];
-    const int span = 1 << (t->height - b.level);
-    const int src = b.j ^ 1;
-    x.q0 = m->leaf_chunk0[src * span];
-    x.q1 = m->leaf_chunk0[(src + 1) * span];
-    x.v_kp = m->pad[x.q1] - m->pad[x.q0];
-    x.v_off = voff;
-    voff += (long long)S * b.s * x.v_kp;
-    x.v_soff = vsoff;
-    vsoff += (long long)(x.q1 - x.q0) * b.s;
-    x.s_pad = (int)pad32(b.s);
-    x.nrk = (x.s_pad + MX_KCH - 1) / MX_KCH;
-    x.u_off = uoff;
-    uoff += (long long)S * b.m * x.s_pad;
-    x.u_soff = usoff;
-    usoff += (long long)x.nrk * b.m;
-    x.rk_cs = rk;
-    for (int c = 0; c <= x.nrk; c++) rktab.push_back(std::min(c * MX_KCH, x.s_pad));
-    rk += x.nrk + 1;
+    m->s_pad[g] = (int)pad32(t->blk[g].s);
+    m->nrk[g] = (m->s_pad[g] + MX_KCH - 1) / MX_KCH;
+    m->rk_cs[g] = (long long)rktab.size();
+    for (int c = 0; c <= m->nrk[g]; c++) rktab.push_back(std::min(c * MX_KCH, m->s_pad[g]));
   }
-  int rc = dalloc(&m->d_vs, (size_t)voff + 8);
-  if (rc == SUCCESS) rc = dalloc(&m->d_us, (size_t)uoff + 8);
-  if (rc == SUCCESS) rc = dalloc(&m->d_sv, (size_t)vsoff + 1);
-  if (rc == SUCCESS) rc = dalloc(&m->d_su, (size_t)usoff + 1);
-  if (rc == SUCCESS) rc = dalloc(&m->d_rk, rktab.size() + 1);
+  int rc = dalloc(&m->d_rk, rktab.size() + 1);
   if (rc == SUCCESS && !rktab.empty()) {
     hipError_t e = hipMemcpyAsync(m->d_rk, rktab.data(), rktab.size() * sizeof(int),
                                   hipMemcpyHostToDevice, s);
@@ -398,68 +489,7 @@ static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
       rc = ALLOCATION_FAILURE;
     }
   }
-  // slice V (rows of V^T = rank columns, K = node rows by chunk) and U
-  // (rows = block rows, K = rank by chunks of MX_KCH)
-  std::vector<SliceItem> items;
-  for (int g = 0; g < t->nblock && rc == SUCCESS; g++) {
-    const BlockMeta &b = t->blk[g];
-    const MixedFactors::Blk &x = m->blk[g];
-    if (b.s <= 0) continue;
-    const int node_lo = m->row.empty() ? 0 : (x.q0 < (int)m->row.size() ? m->row[x.q0] : 0);
-    for (int r = 0; r < b.s; r++)
-      for (int q = x.q0; q < x.q1; q++) {
-        SliceItem it;
-        it.src = t->d_V + b.voff;
-        it.sp = b.n;
-        it.sk = 1;
-        it.dst = m->d_vs + x.v_off;
-        it.dst_slice = (long long)b.s * x.v_kp;
-        it.dst_ld = x.v_kp;
-        it.dk0 = m->pad[q] - m->pad[x.q0];
-        it.scale = m->d_sv + x.v_soff + (long long)(q - x.q0) * b.s + r;
-        it.p = r;
-        it.k0 = m->row[q] - node_lo;
-        it.klen = m->len[q];
-        it.pad = 0;
-        items.push_back(it);
-      }
-    for (int i = 0; i < b.m; i++)
-      for (int c = 0; c < x.nrk; c++) {
-        SliceItem it;
-        it.src = t->d_U + b.uoff;
-        it.sp = 1;
-        it.sk = b.m;
-        it.dst = m->d_us + x.u_off;
-        it.dst_slice = (long long)b.m * x.s_pad;
-        it.dst_ld = x.s_pad;
-        it.dk0 = (long long)c * MX_KCH;
-        it.scale = m->d_su + x.u_soff + (long long)c * b.m + i;
-        it.p = i;
-        it.k0 = c * MX_KCH;
-        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
-        it.pad = 0;
-        items.push_back(it);
-      }
-  }
-  if (rc == SUCCESS && !items.empty()) {
-    SliceItem *d_items = nullptr;
-    rc = dalloc(&d_items, items.size());
-    if (rc == SUCCESS) {
-      hipError_t e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(SliceItem),
-                                    hipMemcpyHostToDevice, s);
-      if (e == hipSuccess) {
-        const int n = (int)items.size();
-        hipLaunchKernelGGL(k_slice, dim3((n + 3) / 4), dim3(256), 0, s, d_items, n, S);
-        e = hipGetLastError();
-      }
-      if (e == hipSuccess) e = hipStreamSynchronize(s);
-      if (e != hipSuccess) {
-        set_error("mixed: slicing factors: %s", hipGetErrorString(e));
-        rc = ALLOCATION_FAILURE;
-      }
-      dfree(d_items);
-    }
-  }
+  if (rc == SUCCESS) rc = prepare_mode(t, m, 0, s);
   if (rc != SUCCESS) {
     mixed_free(m);
     return rc;
@@ -467,6 +497,193 @@ static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
   mixed_free(t->mixed);
   t->mixed = m;
   t->slices = S;
+  return SUCCESS;
+}
+
+// Chunk tables of program-built segments live in the program's own blob,
+// whose device address is known only at launch: fix[i] >= 0 is the byte
+// offset of segment i's table in the blob.
+__global__ __launch_bounds__(256) void k_mx_fix_segs(MxSeg *segs, const long long *fix,
+                                                     const unsigned char *blob, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n && fix[i] >= 0) segs[i].cs = reinterpret_cast<const int *>(blob + fix[i]);
+}
+
+// Work items of the sliced stages of one or more products (trees of one
+// slice count), gathered tree by tree and launched together.
+struct MixedCollect {
+  std::vector<SliceItem> sl, wsl;
+  std::vector<MxSeg> segs;
+  std::vector<MxItem> wit, ait;
+  std::vector<MxReduce> red;
+  std::vector<int> ctab;           // chunk tables of the W segments
+  std::vector<long long> seg_ctab;  // per segment: its table in ctab, or -1
+};
+
+// Bytes spanned by an n x R operand with the given strides.
+static size_t operand_bytes(long long n, int R, long rs, long cs) {
+  if (n <= 0 || R <= 0) return 0;
+  return (size_t)((n - 1) * rs + (long long)(R - 1) * cs + 1) * sizeof(double);
+}
+
+// The sliced part of op(H) X for one tree (op = transpose when tr): slice
+// X, W_b = F_in^T X_src per block (split-K segments, then their sum), slice
+// W, Y += F_out W over all levels per leaf.  X, Y element (row, col) at
+// [row * rs + col * cs].  The diagonal FP64 part is not included.
+static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, int tr, int R,
+                         const double *X, long x_rs, long x_cs, double *Y, long y_rs,
+                         long y_cs) {
+  MixedFactors *m = t->mixed;
+  const MixedFactors::Mode &md = m->mode[tr];
+  const int S = m->S;
+  const int nq = (int)m->row.size();
+  __bf16 *xs = ar.take<__bf16>((size_t)S * R * m->npad + 8);
+  double *sx = ar.take<double>((size_t)nq * R + 1);
+  p.touch(xs, ((size_t)S * R * m->npad) * 2, "mixed X slices");
+  for (int q = 0; q < nq; q++)
+    for (int j = 0; j < R; j++) {
+      SliceItem it;
+      it.src = X;
+      it.sp = x_cs;
+      it.sk = x_rs;
+      it.dst = xs;
+      it.dst_slice = (long long)R * m->npad;
+      it.dst_ld = m->npad;
+      it.dk0 = m->pad[q];
+      it.scale = sx + (long long)q * R + j;
+      it.p = j;
+      it.k0 = m->row[q];
+      it.klen = m->len[q];
+      it.pad = 0;
+      mc.sl.push_back(it);
+    }
+  // W = F_in^T X per block and segment
+  long long wtot = 0;
+  std::vector<long long> woff(t->nblock, 0);
+  for (int g = 0; g < t->nblock; g++) {
+    woff[g] = wtot;
+    wtot += (long long)t->blk[g].s * R;
+  }
+  double *W = ar.take<double>((size_t)wtot + 1);
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    const MixedFactors::Blk &x = md.blk[g];
+    if (b.s <= 0) continue;
+    double *Wg = W + woff[g];
+    const int nch = x.q1 - x.q0;
+    const int nseg = std::max(1, (nch + MX_SEG_CHUNKS - 1) / MX_SEG_CHUNKS);
+    double *part = nseg > 1 ? ar.take<double>((size_t)nseg * b.s * R) : nullptr;
+    if (nch == 0) {  // empty source: W_b = 0
+      mc.segs.push_back(
+          {nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, 0});
+      mc.seg_ctab.push_back((long long)mc.ctab.size());
+      mc.ctab.push_back(0);
+      const int sgi = (int)mc.segs.size() - 1;
+      for (int i0 = 0; i0 < b.s; i0 += MX_TM)
+        for (int j0 = 0; j0 < R; j0 += MX_TN)
+          mc.wit.push_back({Wg, b.s, 1, b.s, R, i0, j0, sgi, sgi, 0, 0});
+      p.touch(Wg, (size_t)b.s * R * sizeof(double), "mixed W");
+      continue;
+    }
+    for (int sg = 0; sg < nseg; sg++) {
+      const int c0 = sg * MX_SEG_CHUNKS, c1 = std::min(nch, c0 + MX_SEG_CHUNKS);
+      const long long base = m->pad[x.q0 + c0] - m->pad[x.q0];
+      mc.seg_ctab.push_back((long long)mc.ctab.size());
+      for (int c = c0; c <= c1; c++)
+        mc.ctab.push_back((int)(m->pad[x.q0 + c] - m->pad[x.q0 + c0]));
+      MxSeg sd;
+      sd.A = md.d_ws + x.w_off + base;
+      sd.lda = x.w_kp;
+      sd.a_slice = (long long)b.s * x.w_kp;
+      sd.sA = md.d_sw + x.w_soff;
+      sd.ldsa = b.s;
+      sd.B = xs + m->pad[x.q0 + c0];
+      sd.ldb = m->npad;
+      sd.b_slice = (long long)R * m->npad;
+      sd.sB = sx;
+      sd.ldsb = R;
+      sd.cs = nullptr;
+      sd.nch = c1 - c0;
+      sd.a_c0 = c0;
+      sd.b_c0 = x.q0 + c0;
+      sd.pad = 0;
+      mc.segs.push_back(sd);
+      const int sgi = (int)mc.segs.size() - 1;
+      double *C = nseg > 1 ? part + (long long)sg * b.s * R : Wg;
+      for (int i0 = 0; i0 < b.s; i0 += MX_TM)
+        for (int j0 = 0; j0 < R; j0 += MX_TN)
+          mc.wit.push_back({C, b.s, 1, b.s, R, i0, j0, sgi, sgi + 1, 0, 0});
+      p.touch(C, (size_t)b.s * R * sizeof(double), "mixed W");
+    }
+    if (nseg > 1) mc.red.push_back({part, Wg, b.s, b.s, R, nseg, 0});
+  }
+  // slices of W, then the apply segments (one per level with a rank)
+  std::vector<__bf16 *> Ws(t->nblock, nullptr);
+  std::vector<double *> sW(t->nblock, nullptr);
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    if (b.s <= 0) continue;
+    Ws[g] = ar.take<__bf16>((size_t)S * R * m->s_pad[g] + 8);
+    sW[g] = ar.take<double>((size_t)m->nrk[g] * R + 1);
+    p.touch(Ws[g], (size_t)S * R * m->s_pad[g] * 2, "mixed W slices");
+    for (int c = 0; c < m->nrk[g]; c++)
+      for (int j = 0; j < R; j++) {
+        SliceItem it;
+        it.src = W + woff[g];
+        it.sp = b.s;
+        it.sk = 1;
+        it.dst = Ws[g];
+        it.dst_slice = (long long)R * m->s_pad[g];
+        it.dst_ld = m->s_pad[g];
+        it.dk0 = (long long)c * MX_KCH;
+        it.scale = sW[g] + (long long)c * R + j;
+        it.p = j;
+        it.k0 = c * MX_KCH;
+        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
+        it.pad = 0;
+        mc.wsl.push_back(it);
+      }
+  }
+  for (int c = 0; c < t->nleaf; c++) {
+    const int lo = t->leaf_off[c], rows = t->leaf_off[c + 1] - lo;
+    if (rows <= 0) continue;
+    const int s0 = (int)mc.segs.size();
+    for (int l = 1; l <= t->height; l++) {
+      const int q = c >> (t->height - l);
+      // H X: block (l, q) has the leaf's rows (U); H^T X: block (l, q^1)
+      // has them as columns (V)
+      const int g = blk_index(l, tr ? (q ^ 1) : q);
+      const BlockMeta &b = t->blk[g];
+      const MixedFactors::Blk &x = md.blk[g];
+      if (b.s <= 0) continue;
+      const int roff = lo - (tr ? b.c0 : b.r0);
+      MxSeg sd;
+      sd.A = md.d_as + x.a_off + (long long)roff * m->s_pad[g];
+      sd.lda = m->s_pad[g];
+      sd.a_slice = (long long)x.rows * m->s_pad[g];
+      sd.sA = md.d_sa + x.a_soff + roff;
+      sd.ldsa = x.rows;
+      sd.B = Ws[g];
+      sd.ldb = m->s_pad[g];
+      sd.b_slice = (long long)R * m->s_pad[g];
+      sd.sB = sW[g];
+      sd.ldsb = R;
+      sd.cs = m->d_rk + m->rk_cs[g];
+      sd.nch = m->nrk[g];
+      sd.a_c0 = 0;
+      sd.b_c0 = 0;
+      sd.pad = 0;
+      mc.seg_ctab.push_back(-1);  // device table already resident
+      mc.segs.push_back(sd);
+    }
+    const int s1 = (int)mc.segs.size();
+    if (s1 == s0) continue;
+    for (int i0 = 0; i0 < rows; i0 += MX_TM)
+      for (int j0 = 0; j0 < R; j0 += MX_TN)
+        mc.ait.push_back({Y + (long long)lo * y_rs, y_cs, y_rs, rows, R, i0, j0, s0, s1, 1, 0});
+  }
+  p.touch(X, operand_bytes(t->n, R, x_rs, x_cs), "mixed X");
+  p.touch(Y, operand_bytes(t->n, R, y_rs, y_cs), "mixed Y");
   return SUCCESS;
 }
 
@@ -479,201 +696,29 @@ static void launch_sliced(int S, unsigned nitems, const MxItem *I, const MxSeg *
   if (S == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nitems), dim3(256), 0, st, I, G);
 }
 
-// Chunk tables of program-built segments live in the program's own blob,
-// whose device address is known only at launch: fix[i] >= 0 is the byte
-// offset of segment i's table in the blob.
-__global__ __launch_bounds__(256) void k_mx_fix_segs(MxSeg *segs, const long long *fix,
-                                                     const unsigned char *blob, int n) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n && fix[i] >= 0) segs[i].cs = reinterpret_cast<const int *>(blob + fix[i]);
-}
-
-// The per-call program: slice X, W GEMMs (+ reduce), slice W, the U W
-// GEMM over all levels.  The diagonal FP64 part runs before it.
-static int mixed_build(Program &p, Arena &ar, hm_dtree *t, int R, const double *X, long ldx,
-                       double *Y, long ldy) {
-  MixedFactors *m = t->mixed;
-  const int S = m->S;
-  const int nq = (int)m->row.size();
-  __bf16 *xs = ar.take<__bf16>((size_t)S * R * m->npad + 8);
-  double *sx = ar.take<double>((size_t)nq * R + 1);
-  p.touch(xs, ((size_t)S * R * m->npad) * 2, "mixed X slices");
-  std::vector<SliceItem> sl;
-  for (int q = 0; q < nq; q++)
-    for (int j = 0; j < R; j++) {
-      SliceItem it;
-      it.src = X;
-      it.sp = ldx;
-      it.sk = 1;
-      it.dst = xs;
-      it.dst_slice = (long long)R * m->npad;
-      it.dst_ld = m->npad;
-      it.dk0 = m->pad[q];
-      it.scale = sx + (long long)q * R + j;
-      it.p = j;
-      it.k0 = m->row[q];
-      it.klen = m->len[q];
-      it.pad = 0;
-      sl.push_back(it);
-    }
-  // W = V^T X per block and segment
-  std::vector<MxSeg> segs;
-  std::vector<MxItem> wit;
-  std::vector<MxReduce> red;
-  std::vector<int> ctab;  // segment chunk tables
-  std::vector<size_t> seg_ctab;
-  std::vector<double *> Wp(t->nblock, nullptr);
-  long long wtot = 0;
-  for (int g = 0; g < t->nblock; g++) wtot += (long long)t->blk[g].s * R;
-  double *W = ar.take<double>((size_t)wtot + 1);
-  std::vector<long long> woff(t->nblock, 0);
-  {
-    long long o = 0;
-    for (int g = 0; g < t->nblock; g++) {
-      woff[g] = o;
-      o += (long long)t->blk[g].s * R;
-    }
-  }
-  for (int g = 0; g < t->nblock; g++) {
-    const BlockMeta &b = t->blk[g];
-    const MixedFactors::Blk &x = m->blk[g];
-    if (b.s <= 0) continue;
-    double *Wg = W + woff[g];
-    const int nch = x.q1 - x.q0;
-    const int nseg = std::max(1, (nch + MX_SEG_CHUNKS - 1) / MX_SEG_CHUNKS);
-    double *part = nseg > 1 ? ar.take<double>((size_t)nseg * b.s * R) : nullptr;
-    if (nch == 0) {  // empty source: W_b = 0
-      segs.push_back({nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, 0});
-      seg_ctab.push_back(ctab.size());
-      ctab.push_back(0);
-      for (int i0 = 0; i0 < b.s; i0 += MX_TM)
-        for (int j0 = 0; j0 < R; j0 += MX_TN)
-          wit.push_back({Wg, b.s, b.s, R, i0, j0, (int)segs.size() - 1, (int)segs.size() - 1, 0, 0});
-      continue;
-    }
-    for (int sg = 0; sg < nseg; sg++) {
-      const int c0 = sg * MX_SEG_CHUNKS, c1 = std::min(nch, c0 + MX_SEG_CHUNKS);
-      const long long base = m->pad[x.q0 + c0] - m->pad[x.q0];
-      seg_ctab.push_back(ctab.size());
-      for (int c = c0; c <= c1; c++) ctab.push_back((int)(m->pad[x.q0 + c] - m->pad[x.q0 + c0]));
-      MxSeg sd;
-      sd.A = m->d_vs + x.v_off + base;
-      sd.lda = x.v_kp;
-      sd.a_slice = (long long)b.s * x.v_kp;
-      sd.sA = m->d_sv + x.v_soff;
-      sd.ldsa = b.s;
-      sd.B = xs + m->pad[x.q0 + c0];
-      sd.ldb = m->npad;
-      sd.b_slice = (long long)R * m->npad;
-      sd.sB = sx;
-      sd.ldsb = R;
-      sd.cs = nullptr;
-      sd.nch = c1 - c0;
-      sd.a_c0 = c0;
-      sd.b_c0 = x.q0 + c0;
-      sd.pad = 0;
-      segs.push_back(sd);
-      double *C = nseg > 1 ? part + (long long)sg * b.s * R : Wg;
-      for (int i0 = 0; i0 < b.s; i0 += MX_TM)
-        for (int j0 = 0; j0 < R; j0 += MX_TN)
-          wit.push_back({C, b.s, b.s, R, i0, j0, (int)segs.size() - 1, (int)segs.size(), 0, 0});
-      p.touch(C, (size_t)b.s * R * sizeof(double), "mixed W");
-    }
-    if (nseg > 1) red.push_back({part, Wg, b.s, b.s, R, nseg, 0});
-  }
-  // slices of W and the apply segments
-  const int S_ = S;
-  std::vector<SliceItem> wsl;
-  std::vector<MxItem> ait;
-  std::vector<__bf16 *> Ws(t->nblock, nullptr);
-  std::vector<double *> sW(t->nblock, nullptr);
-  for (int g = 0; g < t->nblock; g++) {
-    const BlockMeta &b = t->blk[g];
-    const MixedFactors::Blk &x = m->blk[g];
-    if (b.s <= 0) continue;
-    Ws[g] = ar.take<__bf16>((size_t)S_ * R * x.s_pad + 8);
-    sW[g] = ar.take<double>((size_t)x.nrk * R + 1);
-    for (int c = 0; c < x.nrk; c++)
-      for (int j = 0; j < R; j++) {
-        SliceItem it;
-        it.src = W + woff[g];
-        it.sp = b.s;
-        it.sk = 1;
-        it.dst = Ws[g];
-        it.dst_slice = (long long)R * x.s_pad;
-        it.dst_ld = x.s_pad;
-        it.dk0 = (long long)c * MX_KCH;
-        it.scale = sW[g] + (long long)c * R + j;
-        it.p = j;
-        it.k0 = c * MX_KCH;
-        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
-        it.pad = 0;
-        wsl.push_back(it);
-      }
-  }
-  const size_t seg_apply0 = segs.size();
-  for (int c = 0; c < t->nleaf; c++) {
-    const int lo = t->leaf_off[c], mc = t->leaf_off[c + 1] - lo;
-    if (mc <= 0) continue;
-    const int s0 = (int)segs.size();
-    for (int l = 1; l <= t->height; l++) {
-      const int q = c >> (t->height - l);
-      const int g = blk_index(l, q);
-      const BlockMeta &b = t->blk[g];
-      const MixedFactors::Blk &x = m->blk[g];
-      if (b.s <= 0) continue;
-      const int roff = lo - b.r0;
-      MxSeg sd;
-      sd.A = m->d_us + x.u_off + (long long)roff * x.s_pad;
-      sd.lda = x.s_pad;
-      sd.a_slice = (long long)b.m * x.s_pad;
-      sd.sA = m->d_su + x.u_soff + roff;
-      sd.ldsa = b.m;
-      sd.B = Ws[g];
-      sd.ldb = x.s_pad;
-      sd.b_slice = (long long)R * x.s_pad;
-      sd.sB = sW[g];
-      sd.ldsb = R;
-      sd.cs = m->d_rk + x.rk_cs;
-      sd.nch = x.nrk;
-      sd.a_c0 = 0;
-      sd.b_c0 = 0;
-      sd.pad = 0;
-      seg_ctab.push_back((size_t)-1);  // device table already resident
-      segs.push_back(sd);
-    }
-    const int s1 = (int)segs.size();
-    if (s1 == s0) continue;
-    for (int i0 = 0; i0 < mc; i0 += MX_TM)
-      for (int j0 = 0; j0 < R; j0 += MX_TN)
-        ait.push_back({Y + lo, ldy, mc, R, i0, j0, s0, s1, 1, 0});
-  }
-  (void)seg_apply0;
-  // chunk tables of the W segments live in the program blob: patch the
-  // device pointers at launch (blob base + offset)
-  const size_t ctab_off = p.push(ctab);
-  std::vector<size_t> seg_tab_off(segs.size());
-  for (size_t i = 0; i < segs.size(); i++) seg_tab_off[i] = seg_ctab[i];
-  const size_t sl_off = p.push(sl), wsl_off = p.push(wsl);
-  const size_t red_off = p.push(red);
-  const size_t wit_off = p.push(wit), ait_off = p.push(ait);
-  // segments: stored with cs = NULL for W segments; fixed up by a tiny
-  // host pass before upload (the blob base is only known on the device)
-  const size_t seg_off = p.push(segs);
-  const int nsl = (int)sl.size(), nwsl = (int)wsl.size(), nred = (int)red.size();
-  const unsigned nwit = (unsigned)wit.size(), nait = (unsigned)ait.size();
-  const int nsegs = (int)segs.size();
+// The launches of the gathered stages: segment fix-up, slice X, W GEMMs,
+// their reduction, slice W, the apply GEMMs (Y += F_out W).
+static void mixed_emit(Program &p, const MixedCollect &mc, int S_) {
+  const size_t ctab_off = p.push(mc.ctab);
+  const size_t sl_off = p.push(mc.sl), wsl_off = p.push(mc.wsl);
+  const size_t red_off = p.push(mc.red);
+  const size_t wit_off = p.push(mc.wit), ait_off = p.push(mc.ait);
+  const size_t seg_off = p.push(mc.segs);
+  const int nsl = (int)mc.sl.size(), nwsl = (int)mc.wsl.size(), nred = (int)mc.red.size();
+  const unsigned nwit = (unsigned)mc.wit.size(), nait = (unsigned)mc.ait.size();
+  const int nsegs = (int)mc.segs.size();
   long long maxE = 1;
-  for (const auto &r : red) maxE = std::max(maxE, (long long)r.M * r.N);
-  // record the W-segment table offsets for the fix-up op
+  for (const auto &r : mc.red) maxE = std::max(maxE, (long long)r.M * r.N);
   std::vector<long long> fix;
   for (int i = 0; i < nsegs; i++)
-    fix.push_back(seg_ctab[i] == (size_t)-1 ? -1 : (long long)(ctab_off + seg_ctab[i] * sizeof(int)));
+    fix.push_back(mc.seg_ctab[i] < 0 ? -1
+                                     : (long long)(ctab_off + mc.seg_ctab[i] * sizeof(int)));
   const size_t fix_off = p.push(fix);
   p.add("mixed fixup", [=](hipStream_t st, const unsigned char *blob) {
-    hipLaunchKernelGGL(k_mx_fix_segs, dim3((nsegs + 255) / 256), dim3(256), 0, st,
-                       reinterpret_cast<MxSeg *>(const_cast<unsigned char *>(blob) + seg_off),
-                       reinterpret_cast<const long long *>(blob + fix_off), blob, nsegs);
+    if (nsegs)
+      hipLaunchKernelGGL(k_mx_fix_segs, dim3((nsegs + 255) / 256), dim3(256), 0, st,
+                         reinterpret_cast<MxSeg *>(const_cast<unsigned char *>(blob) + seg_off),
+                         reinterpret_cast<const long long *>(blob + fix_off), blob, nsegs);
   });
   p.add("slice X", [=](hipStream_t st, const unsigned char *blob) {
     if (nsl)
@@ -682,13 +727,13 @@ static int mixed_build(Program &p, Arena &ar, hm_dtree *t, int R, const double *
   });
   p.add("mixed W", [=](hipStream_t st, const unsigned char *blob) {
     if (!nwit) return;
-    const MxItem *I = reinterpret_cast<const MxItem *>(blob + wit_off);
-    const MxSeg *G = reinterpret_cast<const MxSeg *>(blob + seg_off);
-    launch_sliced(S_, nwit, I, G, st);
+    launch_sliced(S_, nwit, reinterpret_cast<const MxItem *>(blob + wit_off),
+                  reinterpret_cast<const MxSeg *>(blob + seg_off), st);
   });
   p.add("mixed W reduce", [=](hipStream_t st, const unsigned char *blob) {
     if (nred)
-      hipLaunchKernelGGL(k_mx_reduce, dim3(nred, (unsigned)std::min<long long>(64, (maxE + 255) / 256)),
+      hipLaunchKernelGGL(k_mx_reduce,
+                         dim3(nred, (unsigned)std::min<long long>(64, (maxE + 255) / 256)),
                          dim3(256), 0, st, reinterpret_cast<const MxReduce *>(blob + red_off));
   });
   p.add("slice W", [=](hipStream_t st, const unsigned char *blob) {
@@ -698,35 +743,55 @@ static int mixed_build(Program &p, Arena &ar, hm_dtree *t, int R, const double *
   });
   p.add("mixed apply", [=](hipStream_t st, const unsigned char *blob) {
     if (!nait) return;
-    const MxItem *I = reinterpret_cast<const MxItem *>(blob + ait_off);
-    const MxSeg *G = reinterpret_cast<const MxSeg *>(blob + seg_off);
-    launch_sliced(S_, nait, I, G, st);
+    launch_sliced(S_, nait, reinterpret_cast<const MxItem *>(blob + ait_off),
+                  reinterpret_cast<const MxSeg *>(blob + seg_off), st);
   });
-  p.touch(X, ((size_t)(R - 1) * ldx + t->n) * sizeof(double), "mixed X");
-  p.touch(Y, ((size_t)(R - 1) * ldy + t->n) * sizeof(double), "mixed Y");
-  return SUCCESS;
 }
 
-int mixed_matmat(hm_dtree *t, int R, const double *X, long ldx, double *Y, long ldy,
-                 hipStream_t s) {
+int mixed_ensure_mode(hm_dtree *t, int tr, hipStream_t s) {
   MixedFactors *m = t->mixed;
   if (!m) {
     set_error("mixed product: no sliced factors");
     return INPUT_ERROR;
   }
-  HM_OK(products_diag(t, R, X, ldx, Y, ldy, s));
-  if (!m->prog || m->R != R || m->X != X || m->Y != Y || m->ldx != ldx || m->ldy != ldy) {
+  if (!m->mode[tr].ready) HM_OK(prepare_mode(t, m, tr, s));
+  return SUCCESS;
+}
+
+int mixed_add(Program &p, Arena &ar, const std::vector<MixedJob> &jobs) {
+  for (int S = 1; S <= 4; S++) {
+    MixedCollect mc;
+    bool any = false;
+    for (const MixedJob &j : jobs) {
+      if (!j.t->mixed || j.t->mixed->S != S || j.R <= 0) continue;
+      HM_OK(mixed_collect(mc, p, ar, j.t, j.trans, j.R, j.X, j.x_rs, j.x_cs, j.Y, j.y_rs,
+                          j.y_cs));
+      any = true;
+    }
+    if (any) mixed_emit(p, mc, S);
+  }
+  return SUCCESS;
+}
+
+int mixed_matmat(hm_dtree *t, int trans, int R, const double *X, long x_rs, long x_cs,
+                 double *Y, long y_rs, long y_cs, hipStream_t s) {
+  MixedFactors *m = t->mixed;
+  HM_OK(mixed_ensure_mode(t, trans, s));
+  HM_OK(products_diag(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs, s));
+  if (!m->prog || m->trans != trans || m->R != R || m->X != X || m->Y != Y ||
+      m->x_rs != x_rs || m->x_cs != x_cs || m->y_rs != y_rs || m->y_cs != y_cs) {
     delete m->prog;
     m->prog = nullptr;
     dfree(m->arena);
     m->arena = nullptr;
+    const std::vector<MixedJob> jobs{{t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs}};
     Program dry;
     Arena count;
     dry.record = true;
-    HM_OK(mixed_build(dry, count, t, R, X, ldx, Y, ldy));
+    HM_OK(mixed_add(dry, count, jobs));
     dry.allow(Arena::fake_base(), count.off + 256);
-    dry.allow(X, ((size_t)(R - 1) * ldx + t->n) * sizeof(double));
-    dry.allow(Y, ((size_t)(R - 1) * ldy + t->n) * sizeof(double));
+    dry.allow(X, operand_bytes(t->n, R, x_rs, x_cs));
+    dry.allow(Y, operand_bytes(t->n, R, y_rs, y_cs));
     std::string msg;
     if (dry.audit(&msg) != SUCCESS) {
       set_error("mixed product: audit: %s", msg.c_str());
@@ -738,18 +803,21 @@ int mixed_matmat(hm_dtree *t, int R, const double *X, long ldx, double *Y, long 
     HM_OK(dalloc(&ar.base, ar.cap));
     m->arena = ar.base;
     m->prog = new Program();
-    int rc = mixed_build(*m->prog, ar, t, R, X, ldx, Y, ldy);
+    int rc = mixed_add(*m->prog, ar, jobs);
     if (rc == SUCCESS) rc = m->prog->upload(s);
     if (rc != SUCCESS) {
       delete m->prog;
       m->prog = nullptr;
       return rc;
     }
+    m->trans = trans;
     m->R = R;
     m->X = X;
     m->Y = Y;
-    m->ldx = ldx;
-    m->ldy = ldy;
+    m->x_rs = x_rs;
+    m->x_cs = x_cs;
+    m->y_rs = y_rs;
+    m->y_cs = y_cs;
   }
   return m->prog->run(s);
 }

@@ -1066,14 +1066,14 @@ static int run_reduce(hm_dtree *t, const ProdArgs &a, hipStream_t s) {
   return SUCCESS;
 }
 
-int products_diag(hm_dtree *t, int R, const double *X, long x_cs, double *Y, long y_cs,
-                  hipStream_t s) {
+int products_diag(hm_dtree *t, int trans, int R, const double *X, long x_rs, long x_cs,
+                  double *Y, long y_rs, long y_cs, hipStream_t s) {
   if (R <= 0) return SUCCESS;
   HM_OK(ensure_scratch(t, R));
-  ProdArgs a = make_args(t, 0, R, X, 1, x_cs, Y, 1, y_cs);
-  a.h = 0;  // no off-diagonal levels: the tile computes D_c X_c only
+  ProdArgs a = make_args(t, trans ? 1 : 0, R, X, x_rs, x_cs, Y, y_rs, y_cs);
+  a.h = 0;  // no off-diagonal levels: the tile computes op(D_c) X_c only
   Timed ta = timing_begin("mm_diag", s);
-  run_apply(t, a, 0, false, s);
+  run_apply(t, a, 0, x_rs != 1, s);
   timing_end(ta, s);
   HM_LAUNCH_CHECK();
   return SUCCESS;
@@ -1102,13 +1102,13 @@ static int products_run_ex(hm_dtree *t, int trans, int R, const double *X, long 
     set_error("product: NULL operand");
     return INPUT_ERROR;
   }
-  if (t->slices > 0 && !trans && R > 1 && x_rs == 1 && y_rs == 1) {
+  if (t->slices > 0) {
     if (ext_scratch) {
       set_error("allocation-free product: trees with sliced (mixed-precision) factors "
                 "are not supported (hm_hip_tree_set_precision(tree, 0, ...) first)");
       return INPUT_ERROR;
     }
-    return mixed_matmat(t, R, X, x_cs, Y, y_cs, s);
+    return mixed_matmat(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs, s);
   }
   if (!ext_scratch) HM_OK(ensure_scratch(t, R));
   ProdArgs a = make_args(t, trans, R, X, x_rs, x_cs, Y, y_rs, y_cs, ext_scratch);

@@ -180,4 +180,22 @@ void subtree_product(Arena &ar, const hm_dtree *H, int l, int q, int trans, cons
                      std::vector<GemmChain> &chains, std::vector<GemmLink> &links,
                      bool rowmajor = false);
 
+// ------------------------------------------ sliced (mixed-precision) items
+// One product op(H) X (op = transpose when trans) of a tree whose factors
+// are sliced (hm_hip_tree_set_precision > 0, mode sliced with
+// mixed_ensure_mode), X / Y element (row, col) at [row * rs + col * cs]:
+// only the off-diagonal part, Y += sum of the blocks; the FP64 diagonal is
+// added separately (and must run first).
+struct MixedJob {
+  hm_dtree *t;
+  int trans, R;
+  const double *X;
+  long x_rs, x_cs;
+  double *Y;
+  long y_rs, y_cs;
+};
+// Appends the sliced stages of all jobs (one launch per stage and slice
+// count, whatever the number of trees).
+int mixed_add(Program &p, Arena &ar, const std::vector<MixedJob> &jobs);
+
 }  // namespace hm

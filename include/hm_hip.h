@@ -136,8 +136,12 @@ HM_HIP_API int hm_hip_matmat_noalloc(hm_dtree *tree, int trans, int R, const dou
  * slice pairs summed exactly in FP32 on MFMA and folded in FP64 (pairs
  * beyond the slice count dropped).  Accuracy relative to the chunk maxima
  * is ~2^-8 per slice: 2^-8 (1), 2^-16 (2), 2^-24 (3), 2^-32 (4); the
- * diagonal leaves stay FP64.  Applies to Y = H X with column-major X, Y and R > 1; other forms
- * keep FP64.  Builds the sliced copies of U and V on the device. */
+ * diagonal leaves stay FP64.  Applies to every product of the tree: H X and
+ * H^T X with any operand strides (so X H as well), H x, and the tree's
+ * share of a batch (hm_hip_batch_create).  Builds the sliced copies of U
+ * and V on the device (those for H^T X on the first transposed product).
+ * A batch created with the tree refuses to run after its precision
+ * changes. */
 HM_HIP_API int hm_hip_tree_set_precision(hm_dtree *tree, int slices, void *stream);
 
 /* Apply-kernel variant the autotuner picked for this tree, transpose mode
@@ -293,8 +297,10 @@ HM_HIP_API int hm_hip_compress_batch(int nmat, const double *const *d_A, const l
  * Y_i = H_i X_i, 1: Y_i = H_i^T X_i (X_i, Y_i n_i x R[i]); 2: Y_i = X_i H_i
  * (X_i, Y_i R[i] x n_i); device arrays, column-major, leading dimensions
  * ldx[i], ldy[i].  One launch program of two batched-GEMM launches
- * whatever the number of trees or levels.  The batch keeps pointers to
- * the trees and operands (it owns neither).  Replaces per-tree loops of
+ * whatever the number of trees or levels; trees with sliced factors
+ * (hm_hip_tree_set_precision) add one launch per sliced stage for all of
+ * them.  The batch keeps pointers to the trees and operands (it owns
+ * neither).  Replaces per-tree loops of
  * multiply_hodlr_dense / multiply_hodlr_transpose_dense /
  * multiply_dense_hodlr (src/dense_algebra.c:361-648, 957-1026). */
 typedef struct hm_batch hm_batch;

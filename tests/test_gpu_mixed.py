@@ -1,12 +1,14 @@
 """Mixed-precision products: off-diagonal factors split into 1..4 BF16
 slices on MFMA with exact FP32 accumulation, diagonal leaves FP64, vs the
-FP64 dense product of the same tree.
+FP64 dense product of the same tree — for H X, H^T X, X H (strided
+operands), H x (R = 1), and for batches of trees with different slice
+counts beside FP64 trees.
 
 Tolerance per slice count (relative, Frobenius): each slice carries 8 bits
 of the chunk maximum, so 1 slice 3e-2, 2 slices 5e-4, 3 slices 2e-6, 4
-slices 1e-8 (the workload target).  Trees cover
-ragged and empty leaves, leaves longer than one 128-row chunk and ranks
-spanning two 128-rank chunks."""
+slices 1e-8 (the workload target).  Trees cover ragged and empty leaves,
+leaves longer than one 128-row chunk and ranks spanning two 128-rank
+chunks."""
 import sys
 
 import numpy as np
@@ -32,36 +34,108 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("name,leaves,ranks", CASES, ids=[c[0] for c in CASES])
-def test_mixed_products_vs_dense(gpu, name, leaves, ranks):
+def _upload(rng, leaves, ranks):
     from hmat_lib_amd import device as D
-    rng = np.random.default_rng(len(leaves) + sum(leaves))
     ft = H.random_flat_tree(rng, leaves, ranks)
     lib = capi.hmat()
     keep = T.Keep()
     t = T.fill_tree(lib, ft, keep)
-    try:
-        dense = ft.to_dense()
-        n = ft.n
-        dt = D.DeviceTree.upload(t.contents.root, ft.height)
-        for R in (3, 64, 100):
-            x = np.asfortranarray(rng.standard_normal((n, R)))
-            ref = dense @ x
-            dx = D.DeviceArray.from_numpy(x)
-            dy = D.DeviceArray((n, R))
-            for S in (1, 2, 3, 4):
-                _say(f"case: mixed {name} R={R} slices={S}")
-                dt.set_precision(S)
-                dt.matmat(dx, dy)
-                dt.matmat(dx, dy)  # cached program, same result
-                err = H.rel_err(dy.numpy(), ref)
-                assert err < TOLS[S], (S, err)
-                # transposed products stay FP64
-                dt.matmat(dx, dy, trans=1)
-                assert H.rel_err(dy.numpy(), dense.T @ x) < 1e-12
-            dt.set_precision(0)
+    dt = D.DeviceTree.upload(t.contents.root, ft.height)
+    T.detach_and_free(lib, t)
+    del keep
+    return ft, dt
+
+
+@pytest.mark.parametrize("name,leaves,ranks", CASES, ids=[c[0] for c in CASES])
+def test_mixed_products_vs_dense(gpu, name, leaves, ranks):
+    from hmat_lib_amd import device as D
+    rng = np.random.default_rng(len(leaves) + sum(leaves))
+    ft, dt = _upload(rng, leaves, ranks)
+    dense = ft.to_dense()
+    n = ft.n
+    for R in (3, 64, 100):
+        x = np.asfortranarray(rng.standard_normal((n, R)))
+        dx = D.DeviceArray.from_numpy(x)
+        dy = D.DeviceArray((n, R))
+        xr = np.asfortranarray(rng.standard_normal((R, n)))
+        dxr, dyr = D.DeviceArray.from_numpy(xr), D.DeviceArray((R, n))
+        for S in (1, 2, 3, 4):
+            _say(f"case: mixed {name} R={R} slices={S}")
+            dt.set_precision(S)
             dt.matmat(dx, dy)
-            assert H.rel_err(dy.numpy(), ref) < 1e-12
-        dt.free()
-    finally:
-        T.detach_and_free(lib, t)
+            dt.matmat(dx, dy)  # cached program, same result
+            err = H.rel_err(dy.numpy(), dense @ x)
+            assert err < TOLS[S], ("H X", S, err)
+            dt.matmat(dx, dy, trans=1)
+            err = H.rel_err(dy.numpy(), dense.T @ x)
+            assert err < TOLS[S], ("H^T X", S, err)
+            # X H through the strided transposed form
+            dt.matmat(dxr, dyr, trans=1, R=R, x_strides=(R, 1), y_strides=(R, 1))
+            err = H.rel_err(dyr.numpy(), xr @ dense)
+            assert err < TOLS[S], ("X H", S, err)
+        dt.set_precision(0)
+        dt.matmat(dx, dy)
+        assert H.rel_err(dy.numpy(), dense @ x) < 1e-12
+    # matvec (R = 1) on sliced factors
+    xv = rng.standard_normal(n)
+    dxv, dyv = D.DeviceArray.from_numpy(xv), D.DeviceArray((n,))
+    for S in (2, 4):
+        _say(f"case: mixed {name} matvec slices={S}")
+        dt.set_precision(S)
+        dt.matvec(dxv, dyv)
+        assert H.rel_err(dyv.numpy(), dense @ xv) < TOLS[S]
+    dt.set_precision(0)
+    dt.free()
+
+
+@pytest.mark.parametrize("trans", [0, 1, 2])
+def test_mixed_batch(gpu, trans):
+    """A batch of uneven trees, some with 4 or 2 BF16 slices and some FP64:
+    the FP64 diagonal chains plus one launch per sliced stage for all sliced
+    trees; vs dense per tree at each tree's tolerance, tuned and from HIP
+    graph replays; a tree whose precision changes afterwards is refused."""
+    from hmat_lib_amd import device as D
+    rng = np.random.default_rng(60 + trans)
+    specs = [([128] * 8, 4), ([90, 33, 1, 70, 41, 0, 17, 60], 2), ([200, 150, 0, 64], 0),
+             ([256] * 4, 4), ([64] * 16, 0), ([300, 129, 77, 256], 4)]
+    trees, dense, slices = [], [], []
+    for leaves, S in specs:
+        ft, dt = _upload(rng, leaves, lambda l, j: 1 + (7 * l + 3 * j) % 30)
+        if S:
+            dt.set_precision(S)
+        trees.append(dt)
+        dense.append(ft.to_dense())
+        slices.append(S)
+    xs, ys, xh = [], [], []
+    for i, d in enumerate(dense):
+        n = d.shape[0]
+        R = [1, 5, 64, 100, 17, 70][i]
+        shape = (R, n) if trans == 2 else (n, R)
+        x = np.asfortranarray(rng.standard_normal(shape))
+        xh.append(x)
+        xs.append(D.DeviceArray.from_numpy(x))
+        ys.append(D.DeviceArray(shape))
+    b = D.Batch(trees, xs, ys, trans=trans)
+
+    def check(tag):
+        for i, d in enumerate(dense):
+            ref = (xh[i] @ d if trans == 2 else (d.T if trans else d) @ xh[i])
+            tol = TOLS[slices[i]] if slices[i] else 1e-12
+            err = H.rel_err(ys[i].numpy(), ref)
+            assert err < tol, (tag, i, slices[i], err)
+
+    _say(f"case: mixed batch trans={trans}")
+    b.run()
+    D.synchronize()
+    check("eager")
+    b.tune()
+    for rep in range(3):
+        b.run(graph=True)
+        D.synchronize()
+        check(f"graph {rep}")
+    trees[0].set_precision(3)
+    with pytest.raises(RuntimeError, match="precision"):
+        b.run()
+    b.free()
+    for t in trees:
+        t.free()
