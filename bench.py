@@ -199,6 +199,95 @@ def _dense_rows(kind, pts, ell, shift, rows):
     return a
 
 
+def _time_batch(D, b, steps=20):
+    b.run()
+    b.tune()
+    for _ in range(3):
+        b.run(graph=True)
+    D.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(graph=True)
+    D.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def mixed_batch_table(D, np, trees, xs, fp64, t_plain, t_batched, trans=0):
+    """Batched products with every tree's off-diagonal factors in S = 1..4
+    BF16 slices (diagonal leaves FP64): time per product (HIP graph replay)
+    vs the per-tree FP64 calls (the plain form) and the batched FP64
+    program, and the largest error over the trees vs their FP64 product."""
+    table = {}
+    for S in (1, 2, 3, 4):
+        for t in trees:
+            t.set_precision(S)
+        ys = [D.DeviceArray(x.shape) for x in xs]
+        b = D.Batch(trees, xs, ys, trans=trans)
+        t_s = _time_batch(D, b)
+        err = max(float(np.linalg.norm(y.numpy() - f) / max(np.linalg.norm(f), 1e-300))
+                  for y, f in zip(ys, fp64))
+        b.free()
+        for y in ys:
+            y.free()
+        table[f"bf16x{S}"] = {"us_per_step": t_s * 1e6,
+                              "speedup_vs_per_tree_fp64": t_plain / t_s,
+                              "speedup_vs_batched_fp64": t_batched / t_s,
+                              "max_rel_err_vs_fp64": err}
+    for t in trees:
+        t.set_precision(0)
+    ok = [k for k, v in table.items()
+          if v["max_rel_err_vs_fp64"] <= 1e-8 and v["speedup_vs_per_tree_fp64"] >= 5.0]
+    log("mixed batch: " + ", ".join(f"{k} {v['us_per_step']:.0f} us "
+                                    f"{v['speedup_vs_per_tree_fp64']:.1f}x err "
+                                    f"{v['max_rel_err_vs_fp64']:.1e}" for k, v in table.items()))
+    return {"per_slices": table, "target_5x_at_1e-8_met_by": ok}
+
+
+def batch2d_leg(D, WL, np):
+    """32 trees of the 2-D inverse-distance kernel 1/(r + 1e-2) on Morton
+    points (N log-uniform in [2048, 16384], multiple of 128, leaf 128,
+    thr 1e-8, real ranks), R = 256: per-tree FP64 calls vs the batched FP64
+    program vs batched BF16 slices."""
+    u = WL.uniform(8080, 32)
+    ns = [int(128 * round(np.exp(np.log(2048) + v * np.log(8)) / 128)) for v in u]
+    trees, xs, ys = [], [], []
+    rng = np.random.default_rng(2)
+    for i, n in enumerate(ns):
+        pts = WL.points_2d_morton(7000 + i, n)
+        da = D.fill_kernel_matrix(1, pts, 1e-2, 0.0)
+        h = max(1, int(round(np.log2(n / 128))))
+        trees.append(D.DeviceTree.compress(da, n, h, 1e-8))
+        da.free()
+        x = np.asfortranarray(rng.standard_normal((n, 256)))
+        xs.append(D.DeviceArray.from_numpy(x))
+        ys.append(D.DeviceArray((n, 256)))
+
+    def loop():
+        for t, x, y in zip(trees, xs, ys):
+            t.matmat(x, y)
+    loop()
+    D.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        loop()
+    D.synchronize()
+    t_l = (time.perf_counter() - t0) / 5
+    b = D.Batch(trees, xs, ys)
+    t_b = _time_batch(D, b)
+    fp64 = [y.numpy() for y in ys]
+    b.free()
+    ranks = np.concatenate([t.ranks() for t in trees])
+    out = {"workload": "32 trees, 1/(r + 1e-2) on Morton U[0,1]^2 points, N log-uniform in "
+                       "[2048, 16384] (multiple of 128), leaf 128, thr 1e-8, R = 256",
+           "per_tree_fp64_us": t_l * 1e6, "batched_fp64_us": t_b * 1e6,
+           "batched_fp64_speedup": t_l / t_b, "max_rank": int(ranks.max()),
+           "mixed": mixed_batch_table(D, np, trees, xs, fp64, t_l, t_b)}
+    for t in trees:
+        t.free()
+    log(f"batch 2-D: per-tree FP64 {t_l * 1e3:.2f} ms, batched FP64 {t_b * 1e3:.2f} ms")
+    return out
+
+
 def batch_leg(D, WL, np):
     """512 uneven exp-kernel trees (leaf 128, thr 1e-8, R_i log-uniform in
     [1, 128]): batched compression and the batched product (HIP graph
@@ -279,7 +368,9 @@ def batch_leg(D, WL, np):
            "rank_histogram": {str(k): int(v) for k, v in enumerate(np.bincount(ranks)) if v},
            "max_rel_diff_batched_vs_per_tree": diff,
            "max_rel_err_vs_dense_sampled": max(errs)}
+    fp64 = [y.numpy() for y in ys]
     b.free()
+    out["mixed"] = mixed_batch_table(D, np, trees, xs, fp64, t_l, t_b, trans=0)
     for t in trees:
         t.free()
     log(f"batch: {out['rows_rhs_per_s']:.3e} rows*RHS/s ({out['speedup_vs_per_tree']:.1f}x per-tree), "
@@ -504,7 +595,7 @@ def main():
                     help="skip the drop-in compression of a host copy of A (34 GB host RAM)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, mixed-precision and learned-sketch legs")
-    ap.add_argument("--legs", default="batched,precision,learned_sketch,hh",
+    ap.add_argument("--legs", default="batched,batched_2d,precision,learned_sketch,hh",
                     help="comma-separated subset of the extra legs to run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 10 device-to-device copies of this many MiB "
@@ -763,8 +854,8 @@ def main():
             extras["hh"] = {"error": str(e)}
     dt.free()
     if not args.no_extras:
-        for name, leg in (("batched", batch_leg), ("precision", precision_leg),
-                          ("learned_sketch", learned_leg)):
+        for name, leg in (("batched", batch_leg), ("batched_2d", batch2d_leg),
+                          ("precision", precision_leg), ("learned_sketch", learned_leg)):
             if name not in args.legs.split(","):
                 continue
             try:
