@@ -433,11 +433,12 @@ def precision_leg(D, WL, np):
 
 def learned_leg(D, WL, np):
     """Matern-3/2 N=32768 (height 8, thr 1e-8) at held-out seeded length
-    scales: compression with the learned second sketch width vs the plain
-    doubling path (same residual test, same sketch cap)."""
+    scales (8 seeded draws of ell ~ U[0.01, 1]): compression with the
+    learned second sketch width vs the plain doubling path (same residual
+    test, same sketch cap)."""
     from hmat_lib_amd import learned as Lm
     model = Lm.RankModel.load()
-    u = WL.uniform(424242, 3)
+    u = WL.uniform(424242, 8)
     cases = []
     tot_p = tot_l = 0.0
     for i, uu in enumerate(u):
@@ -479,7 +480,7 @@ def learned_leg(D, WL, np):
     model.free()
     log(f"learned: plain {tot_p:.0f} ms vs learned {tot_l:.0f} ms ({tot_p / tot_l:.2f}x)")
     return {"workload": "Matern-3/2 N=32768 Morton U[0,1]^2, height 8, thr 1e-8, held-out "
-                        "ell ~ U[0.01, 1] (3 seeded), sketch cap 1024",
+                        "ell ~ U[0.01, 1] (8 seeded), sketch cap 1024",
             "plain_ms_total": tot_p, "learned_ms_total": tot_l, "speedup": tot_p / tot_l,
             "cases": cases}
 

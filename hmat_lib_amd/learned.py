@@ -222,21 +222,25 @@ def make_dataset(instances, thr=1e-8, kind=2, log=print):
         X = rank_features(block_features(pts, height, ell), sig, thr)
         Xs.append(X)
         ys.append(np.log2(tr.need + 1.0).astype(np.float32))
-        log(f"[learned] n={n} h={height} ell={ell:.4f} seed={seed}: "
-            f"{(time.perf_counter() - t0) * 1e3:.0f} ms, max rank {ranks.max()}, "
-            f"max width {tr.need.max()}")
+        if len(Xs) % 50 == 1:
+            log(f"[learned] {len(Xs)}: n={n} h={height} ell={ell:.4f} seed={seed}: "
+                f"{(time.perf_counter() - t0) * 1e3:.0f} ms, max rank {ranks.max()}, "
+                f"max width {tr.need.max()}")
         tr.free()
         da.free()
     return np.vstack(Xs), np.concatenate(ys)
 
 
 def training_instances(count, seed=5005):
-    """Seeded Matern-3/2 instances: ell ~ U[0.01, 1], N in {8192, 16384,
-    32768} (leaf 128)."""
+    """Seeded Matern-3/2 instances: ell ~ U[0.01, 1], leaf 128, N = 4096,
+    8192, 16384 or 32768 with probabilities 0.35, 0.35, 0.2, 0.1 (the
+    blocks of every level of a 32768 tree occur in the smaller trees too,
+    except the two largest levels)."""
     u = WL.uniform(seed, 2 * count).reshape(count, 2)
     out = []
     for i in range(count):
-        n = [8192, 16384, 32768][int(u[i, 1] * 3) % 3]
+        v = float(u[i, 1])
+        n = 4096 if v < 0.35 else (8192 if v < 0.7 else (16384 if v < 0.9 else 32768))
         h = int(np.log2(n // 128))
         out.append((n, h, 0.01 + 0.99 * float(u[i, 0]), seed + 11 * i))
     return out
@@ -246,8 +250,8 @@ def main():
     import argparse
     from . import device as D
     ap = argparse.ArgumentParser(description="train the packaged rank predictor on the GPU")
-    ap.add_argument("--instances", type=int, default=48)
-    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--instances", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--dataset", default=None, help="also save the dataset (npz)")
     args = ap.parse_args()

@@ -112,16 +112,33 @@ def test_trained_predictor_gains(gpu):
     point seeds not in its training set): the predicted second width
     replaces the doubling rounds, ranks agree with the plain path (a block
     may move by one where a singular value sits at the threshold), the
-    product error is unchanged, and most predictions pass the residual
-    test."""
+    product error is unchanged, most predictions pass the residual test, and
+    the compression takes less wall time than the plain path's (best of
+    three runs each)."""
     from hmat_lib_amd import learned as L
     from hmat_lib_amd import device as D
+    import time
     model = L.RankModel.load()
+    t_plain = t_learned = 0.0
     for (n, h, ell, seed) in [(16384, 7, 0.04, 777), (16384, 7, 0.45, 778)]:
         _say(f"case: trained predictor n={n} ell={ell}")
         pts, da = _matern(n, ell, seed)
-        plain, _ = L.compress(da, pts, h, 1e-8, ell, None)
-        learned, _ = L.compress(da, pts, h, 1e-8, ell, model)
+        best = {}
+        for mode in ("plain", "learned", "plain", "learned", "plain", "learned"):
+            D.synchronize()
+            t0 = time.perf_counter()
+            tr, _ = L.compress(da, pts, h, 1e-8, ell, model if mode == "learned" else None)
+            D.synchronize()
+            el = time.perf_counter() - t0
+            if mode not in best or el < best[mode][0]:
+                if mode in best:
+                    best[mode][1].free()
+                best[mode] = (el, tr)
+            else:
+                tr.free()
+        (tp, plain), (tl, learned) = best["plain"], best["learned"]
+        t_plain += tp
+        t_learned += tl
         sp, sl = plain.stats, learned.stats
         _say(f"  plain {sp.ms_total:.0f} ms rounds {sp.sketch_rounds}; learned "
              f"{sl.ms_total:.0f} ms rounds {sl.sketch_rounds}, predicted "
@@ -135,3 +152,7 @@ def test_trained_predictor_gains(gpu):
         assert el < max(2 * ep, 1e-8)
         plain.free(), learned.free(), da.free()
     model.free()
+    # the predictor saves sketch rounds, and with them time (best of three
+    # compressions each, over both matrices)
+    _say(f"  wall time plain {t_plain * 1e3:.0f} ms, learned {t_learned * 1e3:.0f} ms")
+    assert t_learned < 0.95 * t_plain
