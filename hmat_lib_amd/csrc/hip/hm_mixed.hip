@@ -696,16 +696,59 @@ static void launch_sliced(int S, unsigned nitems, const MxItem *I, const MxSeg *
   if (S == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nitems), dim3(256), 0, st, I, G);
 }
 
+// Work order of the sliced GEMM launches: workgroup b runs on XCD b % 8, and
+// each XCD has its own L2.  Items that share operands have the same first
+// segment (the rank and column tiles of one W segment read the same slices
+// of F_in^T and X; the tiles of one leaf's apply the same rows of F_out and
+// the same W slices), so each such group goes whole to one XCD (the least
+// loaded so far, load = items x chunks along K) and the eight XCD queues are
+// interleaved; queues shorter than the longest are padded with empty items
+// (no segments, no rows).
+static std::vector<MxItem> xcd_order(const std::vector<MxItem> &items,
+                                     const std::vector<MxSeg> &segs) {
+  constexpr int NX = 8;
+  std::vector<std::vector<MxItem>> q(NX);
+  // groups (runs of items with the same segments) and their loads
+  std::vector<std::pair<size_t, size_t>> grp;
+  std::vector<long long> gl;
+  for (size_t i = 0; i < items.size();) {
+    size_t j = i;
+    while (j < items.size() && items[j].s0 == items[i].s0 && items[j].s1 == items[i].s1) j++;
+    long long k = 0;
+    for (int sg = items[i].s0; sg < items[i].s1; sg++) k += std::max(1, segs[sg].nch);
+    grp.push_back({i, j});
+    gl.push_back((long long)(j - i) * std::max(1LL, k));
+    i = j;
+  }
+  long long load[NX] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t g = 0; g < grp.size(); g++) {
+    int x = 0;
+    for (int y = 1; y < NX; y++)
+      if (load[y] < load[x]) x = y;
+    for (size_t u = grp[g].first; u < grp[g].second; u++) q[x].push_back(items[u]);
+    load[x] += gl[g];
+  }
+  size_t len = 0;
+  for (const auto &v : q) len = std::max(len, v.size());
+  const MxItem none{nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<MxItem> out(len * NX, none);
+  for (int x = 0; x < NX; x++)
+    for (size_t u = 0; u < q[x].size(); u++) out[u * NX + x] = q[x][u];
+  return out;
+}
+
 // The launches of the gathered stages: segment fix-up, slice X, W GEMMs,
 // their reduction, slice W, the apply GEMMs (Y += F_out W).
 static void mixed_emit(Program &p, const MixedCollect &mc, int S_) {
   const size_t ctab_off = p.push(mc.ctab);
   const size_t sl_off = p.push(mc.sl), wsl_off = p.push(mc.wsl);
   const size_t red_off = p.push(mc.red);
-  const size_t wit_off = p.push(mc.wit), ait_off = p.push(mc.ait);
+  const std::vector<MxItem> wit = xcd_order(mc.wit, mc.segs);
+  const std::vector<MxItem> ait = xcd_order(mc.ait, mc.segs);
+  const size_t wit_off = p.push(wit), ait_off = p.push(ait);
   const size_t seg_off = p.push(mc.segs);
   const int nsl = (int)mc.sl.size(), nwsl = (int)mc.wsl.size(), nred = (int)mc.red.size();
-  const unsigned nwit = (unsigned)mc.wit.size(), nait = (unsigned)mc.ait.size();
+  const unsigned nwit = (unsigned)wit.size(), nait = (unsigned)ait.size();
   const int nsegs = (int)mc.segs.size();
   long long maxE = 1;
   for (const auto &r : mc.red) maxE = std::max(maxE, (long long)r.M * r.N);
