@@ -61,8 +61,9 @@ def cpu_cores():
     return max(1, min(16, n))
 
 
-def host_info(cores):
-    """nproc, affinity, CPU model and the BLAS/LAPACK the CPU baselines link."""
+def host_info(cores, cpu_ran=True):
+    """nproc, affinity, CPU model and the BLAS/LAPACK the CPU baselines link
+    (loaded by the reference library of those legs; not loaded when none ran)."""
     info = {"nproc": os.cpu_count(), "threads_used": cores}
     try:
         info["affinity"] = len(os.sched_getaffinity(0))
@@ -82,7 +83,8 @@ def host_info(cores):
         mkl.mkl_get_version_string(buf, 256)
         info["blas"] = buf.value.decode(errors="replace").strip()
     except Exception as e:
-        info["blas"] = f"unknown ({e})"
+        info["blas"] = (f"unknown ({e})" if cpu_ran else
+                        "not loaded (no CPU baseline leg ran in this process)")
     return info
 
 
@@ -889,7 +891,7 @@ def main():
         "compress_cpu_baseline": cpu_compress,
         "compress_api": compress_api,
         "matvec_cpu_baseline": cpu_mv,
-        "host": host_info(cores),
+        "host": host_info(cores, cpu_ran=not args.no_cpu),
         "compress_stats": {"sketch_rounds": st.sketch_rounds, "retried": st.blocks_retried,
                            "max_sketch": st.max_sketch, "capped": st.fallback_blocks,
                            "max_rank": int(ranks.max()), "min_rank": int(ranks.min())},
