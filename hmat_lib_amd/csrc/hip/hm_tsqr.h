@@ -47,6 +47,16 @@ struct QrJob {
   long long ldq;
   double *R;
   long long ldr;
+  // Continuing a QR whose first c_start columns (a multiple of the panel
+  // width) are done: their orthonormal columns are copied from Qprev (ld
+  // ldqp) into Q and, with R, R's leading c_start x c_start block from the
+  // transpose held in Rprev_t (ld ldrp); Y's first c_start columns are not
+  // read and the panel sweep starts at column c_start.
+  int c_start = 0;
+  const double *Qprev = nullptr;
+  long long ldqp = 0;
+  const double *Rprev_t = nullptr;
+  long long ldrp = 0;
 };
 int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw = 64);
 

@@ -648,6 +648,11 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
       set_error("qr: job with m=%d L=%d", J.m, J.L);
       return INPUT_ERROR;
     }
+    if (J.c_start < 0 || J.c_start % bw != 0 || (J.c_start > 0 && (J.c_start >= J.L || !J.Qprev ||
+                                                                 (J.R && !J.Rprev_t)))) {
+      set_error("qr: continued job with c_start=%d (L=%d, panel width %d)", J.c_start, J.L, bw);
+      return INPUT_ERROR;
+    }
     maxp = std::max(maxp, (J.L + bw - 1) / bw);
   }
   // per-job temporaries: C1 (when R is not wanted) and C2 (L x bw), R1, R2
@@ -666,7 +671,15 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
     tmp[q].r2 = ar.take<double>((size_t)bw * bw);
     if (J.R) zero.push_back({nullptr, J.R, 0, J.ldr, J.L, J.L, 0, 0});
   }
+  // the done columns of continued jobs (after R's zero fill)
+  std::vector<Copy2D> carried;
+  for (const QrJob &J : jobs) {
+    if (J.c_start <= 0 || J.L <= 0) continue;
+    carried.push_back({J.Qprev, J.Q, J.ldqp, J.ldq, J.m, J.c_start, 0, 0});
+    if (J.R) carried.push_back({J.Rprev_t, J.R, J.ldrp, J.ldr, J.c_start, J.c_start, 1, 0});
+  }
   HM_OK(copy2d_add(prog, zero));
+  HM_OK(copy2d_add(prog, carried));
   for (int p = 0; p < maxp; p++) {
     const int c0 = p * bw;
     std::vector<GemmJob> g1, g2, g3, g4, g3b, g4b, g5, g5b, g6;
@@ -674,7 +687,7 @@ int qr_add(Program &prog, Arena &ar, const std::vector<QrJob> &jobs, int bw) {
     std::vector<Copy2D> cp;
     for (size_t q = 0; q < jobs.size(); q++) {
       const QrJob &J = jobs[q];
-      if (J.L <= c0) continue;
+      if (J.L <= c0 || c0 < J.c_start) continue;
       const int w = std::min(bw, J.L - c0);
       double *Yp = J.Y + (long long)c0 * J.ldy;
       double *Qp = J.Q + (long long)c0 * J.ldq;

@@ -57,6 +57,7 @@ struct TsvdStats {
   int capped = 0;      // blocks that hit sketch_max without passing the test
   int widened = 0;     // blocks whose second width came from `widen`
   int widened_failed = 0;  // ... and failed the residual test at that width
+  int continued = 0;   // block rounds that continued the previous round's range finder
 };
 
 // Factor every block.  On entry blocks[i].{A,lda,m,n,ell,rule,thr,uid} are

@@ -783,6 +783,26 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
   std::vector<RoundOut> round_pools;
   Scoped round_mem;
   std::vector<long long> uoff_final(nb, 0), voff_final(nb, 0);
+  // Rounds after a failed one continue its range finder: column j of the
+  // sketch depends only on (block, j), so a retried block whose failed
+  // width w is a multiple of the 64-column QR panel keeps the first w
+  // columns of Q (the sketch's basis), of Q2 (B^T's) and R2 (carried out
+  // of the round buffer before the next round reuses it), and only the new
+  // columns are sketched, orthonormalised and projected; the continued
+  // QRs recompute exactly what a fresh round computes for those columns.
+  struct Carry {
+    int cols = 0;
+    const double *Q = nullptr, *Q2 = nullptr, *Mt = nullptr;
+  };
+  std::vector<Carry> carry(nb);
+  struct CarryBuf {
+    double *p = nullptr;
+    size_t bytes = 0;
+    ~CarryBuf() { dfree(p); }
+  } carry_cur;
+  auto can_carry = [&](const TsvdBlock &b) {
+    return b.ell >= 64 && b.ell % 64 == 0 && b.ell < std::min(std::min(b.m, b.n), cap);
+  };
   while (!active.empty()) {
     st.rounds++;
     const auto r0 = std::chrono::steady_clock::now();
@@ -806,17 +826,24 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     }
     TsvdBufs bf;
     RoundBuffer rbuf;
+    // R2^T of every block (M before the SVD overwrites it), kept when a
+    // block of this round could be continued
+    bool any_carry = false;
+    for (int i : active) any_carry |= can_carry(blocks[i]);
+    double *Ms = nullptr;
     {
-      const long long sizes[10] = {ytot, otot, qtot, btot, btot, mtot, mtot, mtot, mtot, stot};
-      double **dst[10] = {&bf.Y, &bf.Om, &bf.Q, &bf.Bt, &bf.Q2,
-                          &bf.M, &bf.Vj, &bf.G, &bf.Z, &bf.Sig};
-      size_t offs[10], need = 0;
-      for (int q = 0; q < 10; q++) {
+      const long long sizes[11] = {ytot, otot, qtot, btot, btot, mtot, mtot, mtot, mtot, stot,
+                                   any_carry ? mtot : 0};
+      double **dst[11] = {&bf.Y, &bf.Om, &bf.Q, &bf.Bt, &bf.Q2, &bf.M,
+                          &bf.Vj, &bf.G, &bf.Z, &bf.Sig, &Ms};
+      size_t offs[11], need = 0;
+      for (int q = 0; q < 11; q++) {
         offs[q] = need;
         need += ((size_t)sizes[q] * sizeof(double) + 16 + 255) & ~(size_t)255;
       }
       HM_OK(rbuf.get(need, s));
-      for (int q = 0; q < 10; q++) *dst[q] = reinterpret_cast<double *>(rbuf.base + offs[q]);
+      for (int q = 0; q < 11; q++) *dst[q] = reinterpret_cast<double *>(rbuf.base + offs[q]);
+      if (!any_carry) Ms = nullptr;  // no region was carved for it
     }
     TsvdBlock *d_blocks = nullptr;
     HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
@@ -830,11 +857,13 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     for (int i : order) {
       const TsvdBlock &b = blocks[i];
       const int L = b.ell + b.ntest;
-      for (int j = 0; j < L; j++) wo.push_back(make_int2(i, j));
+      // a continued block: only its new columns (and test columns)
+      const int c0 = carry[i].cols;
+      for (int j = c0; j < L; j++) wo.push_back(make_int2(i, j));
       const int mt = (b.m + SketchTile::TM - 1) / SketchTile::TM;
       const int ntl = (L + SketchTile::TN - 1) / SketchTile::TN;
       for (int a = 0; a < mt; a++)
-        for (int c = 0; c < ntl; c++) ws.push_back(make_int4(i, a, c, 0));
+        for (int c = c0 / SketchTile::TN; c < ntl; c++) ws.push_back(make_int4(i, a, c, 0));
     }
     std::vector<int> order_m = order;
     std::sort(order_m.begin(), order_m.end(), [&](int x, int y) {
@@ -845,8 +874,9 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       const bool narrow = b.ell <= ProjTile16::TN;
       const int mt = (b.n + SketchTile::TM - 1) / SketchTile::TM;
       const int ntl = narrow ? 1 : (b.ell + SketchTile::TN - 1) / SketchTile::TN;
+      const int c0 = narrow ? 0 : carry[i].cols / SketchTile::TN;
       for (int a = 0; a < mt; a++)
-        for (int c = 0; c < ntl; c++) (narrow ? wp16 : wp).push_back(make_int4(i, a, c, 0));
+        for (int c = c0; c < ntl; c++) (narrow ? wp16 : wp).push_back(make_int4(i, a, c, 0));
     }
     ws = xcd_grouped(ws);
     wp = xcd_grouped(wp);
@@ -873,6 +903,7 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       const TsvdBlock &b = blocks[i];
       inputs.push_back({b.A, ((size_t)(b.n - 1) * b.lda + b.m) * sizeof(double)});
     }
+    if (carry_cur.p) inputs.push_back({carry_cur.p, carry_cur.bytes});
     const unsigned long long seed = opt.seed;
     const unsigned nwo = (unsigned)wo.size(), nws = (unsigned)ws.size(),
                    nwp = (unsigned)wp.size(), nwp16 = (unsigned)wp16.size();
@@ -885,9 +916,22 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       for (int i : order) {
         const TsvdBlock &b = blocks[i];
         double *Y = bf.Y + b.yoff;
-        qy.push_back({Y, b.m, b.m, b.ell, bf.Q + b.qoff, b.m, nullptr, 0});
+        QrJob jy{Y, b.m, b.m, b.ell, bf.Q + b.qoff, b.m, nullptr, 0};
         double *R2 = ar.take<double>((size_t)b.ell * b.ell);
-        qb.push_back({bf.Bt + b.boff, b.n, b.n, b.ell, bf.Q2 + b.boff, b.n, R2, b.ell});
+        QrJob jb{bf.Bt + b.boff, b.n, b.n, b.ell, bf.Q2 + b.boff, b.n, R2, b.ell};
+        const Carry &cy = carry[i];
+        if (cy.cols > 0) {
+          jy.c_start = cy.cols;
+          jy.Qprev = cy.Q;
+          jy.ldqp = b.m;
+          jb.c_start = cy.cols;
+          jb.Qprev = cy.Q2;
+          jb.ldqp = b.n;
+          jb.Rprev_t = cy.Mt;
+          jb.ldrp = cy.cols;
+        }
+        qy.push_back(jy);
+        qb.push_back(jb);
         tr.push_back({R2, bf.M + b.moff, b.ell, b.ell, b.ell, b.ell, 1, 0});
         if (b.ntest > 0) {
           double *Yt = Y + (long long)b.ell * b.m;
@@ -935,6 +979,8 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       HM_HIP(hipGetLastError());
     }
     HM_OK(program_exec(build, inputs, s));
+    if (Ms && mtot > 0)
+      HM_HIP(hipMemcpyAsync(Ms, bf.M, (size_t)mtot * sizeof(double), hipMemcpyDeviceToDevice, s));
     {
       Timed t5 = timing_begin("tsvd_jacobi", s);
       HM_OK(svd_cores(d_blocks, blocks, order, bf, accept, s));
@@ -1101,6 +1147,57 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       // never narrower than the sketch that just failed
       b.ell = std::min(std::min(std::max(b.ell, prev_ell[r] + 1), mn), cap);
       b.ntest = (b.ell >= mn) ? 0 : std::min(opt.n_test, b.m - b.ell);
+    }
+    // carry the done columns of the continued blocks out of the round
+    // buffer (blocks[] already holds the next widths; the offsets are still
+    // this round's)
+    {
+      std::vector<Carry> next(nb);
+      size_t ctot = 0;
+      std::vector<int> cont;
+      for (size_t r = 0; r < retry.size(); r++) {
+        const int i = retry[r], w = prev_ell[r];
+        const TsvdBlock &b = blocks[i];
+        if (!Ms || w < 64 || w % 64 != 0 || b.ell <= w || w >= std::min(b.m, b.n)) continue;
+        cont.push_back((int)r);
+        ctot += ((size_t)b.m + b.n + w) * w;
+      }
+      double *cbuf = nullptr;
+      if (ctot > 0) {
+        HM_OK(dalloc(&cbuf, ctot));
+        size_t off = 0;
+        for (int r : cont) {
+          const int i = retry[r], w = prev_ell[r];
+          const TsvdBlock &b = blocks[i];
+          Carry c;
+          c.cols = w;
+          double *q = cbuf + off, *q2 = q + (size_t)b.m * w, *mt = q2 + (size_t)b.n * w;
+          off += ((size_t)b.m + b.n + w) * w;
+          hipError_t e = hipMemcpyAsync(q, bf.Q + b.qoff, (size_t)b.m * w * sizeof(double),
+                                        hipMemcpyDeviceToDevice, s);
+          if (e == hipSuccess)
+            e = hipMemcpyAsync(q2, bf.Q2 + b.boff, (size_t)b.n * w * sizeof(double),
+                               hipMemcpyDeviceToDevice, s);
+          if (e == hipSuccess)
+            e = hipMemcpyAsync(mt, Ms + b.moff, (size_t)w * w * sizeof(double),
+                               hipMemcpyDeviceToDevice, s);
+          if (e != hipSuccess) {
+            dfree(cbuf);
+            set_error("tsvd: carrying a round: %s", hipGetErrorString(e));
+            return ALLOCATION_FAILURE;
+          }
+          c.Q = q;
+          c.Q2 = q2;
+          c.Mt = mt;
+          next[i] = c;
+        }
+      }
+      HM_HIP(hipStreamSynchronize(s));
+      dfree(carry_cur.p);  // the round that used it has finished
+      carry_cur.p = cbuf;
+      carry_cur.bytes = ctot * sizeof(double);
+      carry.swap(next);
+      st.continued += (int)cont.size();
     }
     active = retry;
   }

@@ -56,29 +56,42 @@ def leaf_offsets(n, height):
 
 
 def block_features(pts, height, ell):
-    """(nblock, 7) geometric inputs, level-major blocks (level 1 first)."""
+    """(nblock, 7) geometric inputs, level-major blocks (level 1 first).
+    Every tree node is a contiguous range of the points: per-leaf sums and
+    bounding boxes, combined up the tree."""
     pts = np.asarray(pts, dtype=np.float64)
     if pts.ndim == 1:
         pts = pts[:, None]
-    n = pts.shape[0]
+    n, d = pts.shape
     off = leaf_offsets(n, height)
-    rows = []
+    nl = 1 << height
+    cnt = np.diff(off).astype(np.int64)
+    full = cnt > 0
+    lsum = np.zeros((nl, d))
+    llo = np.full((nl, d), np.inf)
+    lhi = np.full((nl, d), -np.inf)
+    if full.any():
+        starts = off[:-1][full]
+        lsum[full] = np.add.reduceat(pts, starts, axis=0)
+        llo[full] = np.minimum.reduceat(pts, starts, axis=0)
+        lhi[full] = np.maximum.reduceat(pts, starts, axis=0)
+    out = []
     for lvl in range(1, height + 1):
-        span = 1 << (height - lvl)
-        for j in range(1 << lvl):
-            r0, r1 = off[j * span], off[(j + 1) * span]
-            c0, c1 = off[(j ^ 1) * span], off[((j ^ 1) + 1) * span]
-            pr, pc = pts[r0:r1], pts[c0:c1]
-            m, k = max(r1 - r0, 1), max(c1 - c0, 1)
-            cr = pr.mean(0) if len(pr) else np.zeros(pts.shape[1])
-            cc = pc.mean(0) if len(pc) else np.zeros(pts.shape[1])
-            dr = np.linalg.norm(pr.max(0) - pr.min(0)) if len(pr) else 0.0
-            dc = np.linalg.norm(pc.max(0) - pc.min(0)) if len(pc) else 0.0
-            sep = np.linalg.norm(cr - cc)
-            rows.append([np.log2(m) / 16.0, np.log2(k) / 16.0, lvl / 16.0,
-                         np.log10(sep + 1e-3) / 3.0, np.log10(ell) / 2.0,
-                         np.log10(dr + 1e-6) / 3.0, np.log10(dc + 1e-6) / 3.0])
-    return np.asarray(rows, dtype=np.float32)
+        nn, span = 1 << lvl, 1 << (height - lvl)
+        c = cnt.reshape(nn, span).sum(1)
+        sm = lsum.reshape(nn, span, d).sum(1)
+        lo = llo.reshape(nn, span, d).min(1)
+        hi = lhi.reshape(nn, span, d).max(1)
+        cen = np.where(c[:, None] > 0, sm / np.maximum(c, 1)[:, None], 0.0)
+        dia = np.where(c > 0, np.linalg.norm(np.where(c[:, None] > 0, hi - lo, 0.0), axis=1), 0.0)
+        part = np.arange(nn) ^ 1
+        m, k = np.maximum(c, 1), np.maximum(c[part], 1)
+        sep = np.linalg.norm(cen - cen[part], axis=1)
+        out.append(np.stack([np.log2(m) / 16.0, np.log2(k) / 16.0, np.full(nn, lvl / 16.0),
+                             np.log10(sep + 1e-3) / 3.0, np.full(nn, np.log10(ell) / 2.0),
+                             np.log10(dia + 1e-6) / 3.0, np.log10(dia[part] + 1e-6) / 3.0],
+                            axis=1))
+    return np.concatenate(out).astype(np.float32)
 
 
 def rank_features(extra, sig16, thr):

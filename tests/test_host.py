@@ -392,3 +392,44 @@ def test_read_dense_text_reference_format(tmp_path):
         q.write_text(text)
         assert not lib.hmat_amd_read_dense_text(str(q).encode(), C.byref(n), C.byref(ierr))
         assert ierr.value == capi.INPUT_ERROR, name
+
+
+def _block_features_loop(pts, height, ell):
+    """The geometric inputs block by block, from their definition."""
+    from hmat_lib_amd import learned as L
+    pts = np.asarray(pts, dtype=np.float64)
+    if pts.ndim == 1:
+        pts = pts[:, None]
+    off = L.leaf_offsets(pts.shape[0], height)
+    rows = []
+    for lvl in range(1, height + 1):
+        span = 1 << (height - lvl)
+        for j in range(1 << lvl):
+            pr = pts[off[j * span]:off[(j + 1) * span]]
+            pc = pts[off[(j ^ 1) * span]:off[((j ^ 1) + 1) * span]]
+            cr = pr.mean(0) if len(pr) else np.zeros(pts.shape[1])
+            cc = pc.mean(0) if len(pc) else np.zeros(pts.shape[1])
+            dr = np.linalg.norm(pr.max(0) - pr.min(0)) if len(pr) else 0.0
+            dc = np.linalg.norm(pc.max(0) - pc.min(0)) if len(pc) else 0.0
+            rows.append([np.log2(max(len(pr), 1)) / 16, np.log2(max(len(pc), 1)) / 16, lvl / 16,
+                         np.log10(np.linalg.norm(cr - cc) + 1e-3) / 3, np.log10(ell) / 2,
+                         np.log10(dr + 1e-6) / 3, np.log10(dc + 1e-6) / 3])
+    return np.asarray(rows, dtype=np.float32)
+
+
+def test_block_features():
+    """The learned predictor's geometric inputs: a known answer (8 points
+    0..7 on a line, height 1: centroids 1.5 and 5.5, extents 3) and the
+    per-block definition on 1-D and 2-D points, with empty leaves."""
+    from hmat_lib_amd import learned as L
+    from hmat_lib_amd import workloads as WL
+    f = L.block_features(np.arange(8.0), 1, 0.1)
+    want = [np.log2(4) / 16, np.log2(4) / 16, 1 / 16, np.log10(4 + 1e-3) / 3, np.log10(0.1) / 2,
+            np.log10(3 + 1e-6) / 3, np.log10(3 + 1e-6) / 3]
+    np.testing.assert_allclose(f, np.array([want, want], dtype=np.float32), rtol=1e-6)
+    cases = [(np.sort(np.random.default_rng(1).uniform(size=300)), 5, 0.1),
+             (WL.points_2d_morton(4, 1000), 3, 0.3), (WL.points_2d_morton(5, 20), 5, 0.3),
+             (WL.points_2d_morton(6, 4096), 5, 0.05)]
+    for pts, h, ell in cases:
+        np.testing.assert_array_equal(L.block_features(pts, h, ell),
+                                      _block_features_loop(pts, h, ell))
