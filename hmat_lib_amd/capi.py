@@ -90,7 +90,7 @@ class CompressStats(C.Structure):
     _fields_ = [("sketch_rounds", c_int), ("blocks_retried", c_int),
                 ("max_sketch", c_int), ("fallback_blocks", c_int),
                 ("ms_total", c_double), ("predicted_blocks", c_int),
-                ("predicted_failed", c_int)]
+                ("predicted_failed", c_int), ("continued_blocks", c_int)]
 
 
 def dptr(a: np.ndarray):

@@ -225,6 +225,7 @@ static void fill_stats(hm_compress_stats *stats, const TsvdStats &ts, double ms)
   stats->ms_total = ms;
   stats->predicted_blocks = ts.widened;
   stats->predicted_failed = ts.widened_failed;
+  stats->continued_blocks = ts.continued;
 }
 
 static TsvdOptions tsvd_options(const hm_compress_opts &opts) {

@@ -194,6 +194,8 @@ typedef struct hm_compress_stats {
   double ms_total;
   int predicted_blocks; /* blocks whose second sketch width the rank model chose */
   int predicted_failed; /* ... whose residual test failed at that width (went on doubling) */
+  int continued_blocks; /* block-rounds that continued the failed round's sketch (kept its
+                           first columns' basis and sketched only the new columns) */
 } hm_compress_stats;
 
 HM_HIP_API void hm_hip_compress_opts_default(hm_compress_opts *opts);

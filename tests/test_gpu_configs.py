@@ -202,10 +202,11 @@ def test_batch_512_uneven(gpu):
 
 
 def test_inv2d_64k_fp64_and_slices(gpu):
-    """inv2d_64k, R=256: the FP64 product equals the reference's product on
-    the same tree (1e-12) and the FP64 dense rows to the threshold's
-    error; the sliced products' error vs the FP64 product of the same tree
-    per slice count, 4 slices within 1e-8."""
+    """inv2d_64k, R=256: the compression's later rounds continue the failed
+    rounds' sketches (blocks of ranks to 662); the FP64 product equals the
+    reference's product on the same tree (1e-12) and the FP64 dense rows to
+    the threshold's error; the sliced products' error vs the FP64 product
+    of the same tree per slice count, 4 slices within 1e-8."""
     from hmat_lib_amd import device as D
     w = WL.WORKLOADS["inv2d_64k"]
     _say(f"case: config {w.name} compress")
@@ -213,7 +214,9 @@ def test_inv2d_64k_fp64_and_slices(gpu):
     opts.sketch_max = 1024
     pts, dt = _compress_workload(D, w, opts)
     ranks = dt.ranks()
-    _say(f"  max rank {ranks.max()}, rounds {dt.stats.sketch_rounds}")
+    _say(f"  max rank {ranks.max()}, rounds {dt.stats.sketch_rounds}, "
+         f"continued block-rounds {dt.stats.continued_blocks}")
+    assert dt.stats.continued_blocks > 0
     x = w.rhs_matrix()
     dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((w.n, w.rhs))
     dt.matmat(dx, dy)
