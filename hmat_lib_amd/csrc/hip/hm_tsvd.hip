@@ -375,10 +375,28 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
     tile.template run<false, false>(lds, nc, la, lb);
     tile.store([&](int i, int j, double v) { G[i][j] = v; });
   }
+  const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
+  __syncthreads();
+  // A pair whose Gram passes every rotation test is left alone: with no
+  // rotation G never changes, so the sweep below would test exactly these
+  // entries and rotate nothing (later outer sweeps revisit mostly such pairs).
+  {
+    int mine = 0;
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int p = e >> 6, q = e & 63;
+      if (p < q && q < W) {
+        const double gpq = G[p][q];
+        mine |= (gpq != 0.0 && fabs(gpq) > tol * sqrt(G[p][p] * G[q][q]));
+      }
+    }
+    if (!__syncthreads_or(mine)) {
+      if (tid == 0) rotated[blockIdx.x] = 0;
+      return;
+    }
+  }
   for (int e = tid; e < 64 * 64; e += 256) J[e >> 6][e & 63] = ((e >> 6) == (e & 63)) ? 1.0 : 0.0;
   if (tid == 0) rot = 0;
   __syncthreads();
-  const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
   int rot_any = 0;
   // (2) cyclic Jacobi on G (round robin over 64 indices, 32 disjoint pairs)
   for (int sweep = 0; sweep < BJ_INNER; sweep++) {
