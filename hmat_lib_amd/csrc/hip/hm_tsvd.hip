@@ -313,22 +313,23 @@ static int to_device(T **d, const std::vector<T> &h, hipStream_t s) {
 // For cores wider than BJ_SCALAR_MAX columns: the columns of M (nc x nc)
 // are cut into blocks of BJ_W; each round of a sweep pairs the blocks
 // round robin and one workgroup per pair (a, b) (1) forms the Gram matrix
-// G = M_S^T M_S of the <= 64 columns S = a u b on MFMA, (2) diagonalises G
+// G = M_S^T M_S of the <= 2 BJ_W columns S = a u b on MFMA, (2) diagonalises G
 // in LDS by cyclic two-sided Jacobi (J accumulates the rotations); then
 // k_bjac_apply, one workgroup per 64-row tile, (3) applies J to the columns
 // S of M and of V (a tile loop inside the pair's workgroup was its longest
 // phase).  A pair whose first inner sweep rotates anything marks its matrix
-// as not converged.
-static constexpr int BJ_W = 32;
+// as not converged.  Blocks of 16 columns measured 3-10 % faster than
+// blocks of 32 on whole compressions and H x H products: twice the pair
+// visits per sweep, but each inner sweep has 31 rounds over a 32 x 32 Gram
+// instead of 63 over 64 x 64, so the LDS rotation work per sweep halves
+// while the Gram and apply flops stay the same.
+static constexpr int BJ_W = 16;
 static constexpr int BJ_SCALAR_MAX = 64;
 // inner cyclic sweeps per pair visit: the outer sweeps revisit every pair
 // until none rotates, so a partly diagonalised pair only costs another
-// visit, while each inner sweep costs 126 barriers (one sweep measured
-// faster than two or four)
+// visit, while each inner sweep costs 2 (2 BJ_W - 1) barriers (one sweep
+// measured faster than two or four)
 static constexpr int BJ_INNER = 1;
-typedef MfmaTile<2, 2, 2, 2, 16> BjTile;      // 64 x 64
-typedef MfmaTile<2, 2, 2, 2, 32> BjGramTile;  // 64 x 64, K-chunks of 32 (half the chunks)
-typedef MfmaTile<2, 2, 2, 2, 64> BjApplyTile; // 64 x 64, K = 64 in one chunk
 
 __global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list, TsvdBufs bf) {
   const TsvdBlock B = blocks[list[blockIdx.x]];
@@ -340,31 +341,46 @@ __global__ __launch_bounds__(256) void k_eye(TsvdBlock *blocks, const int *list,
     V[e] = ((e % nc) == (e / nc)) ? 1.0 : 0.0;
 }
 
+// Tiles of the block Jacobi for blocks of BW columns (pairs of PW = 2 BW):
+// the pair's Gram (PW x PW) and the application of J to 64-row tiles.
+template <int BW> struct BjTiles;
+template <> struct BjTiles<32> {
+  typedef MfmaTile<2, 2, 2, 2, 32> Gram;   // 64 x 64, K-chunks of 32
+  typedef MfmaTile<2, 2, 2, 2, 64> Apply;  // 64 x 64, K = 64 in one chunk
+};
+template <> struct BjTiles<16> {
+  typedef MfmaTile<2, 2, 1, 1, 32> Gram;   // 32 x 32, K-chunks of 32
+  typedef MfmaTile<2, 2, 2, 1, 32> Apply;  // 64 x 32, K = 32 in one chunk
+};
+
+template <int BW>
 __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4 *items,
                                                    TsvdBufs bf, int *flags, double *jout,
                                                    int *rotated) {
+  constexpr int PW = 2 * BW, NP = PW / 2, LG = PW + 1;
+  typedef typename BjTiles<BW>::Gram GramTile;
   // the MFMA staging area lives inside G: the Gram tile is stored to G only
-  // after its last K-chunk, and G is dead while J is applied; 67 KB of LDS
-  // per workgroup lets two share a CU
-  __shared__ double G[64][65];
-  __shared__ double J[64][65];
-  static_assert(BjGramTile::LDS_DOUBLES <= 64 * 65, "staging fits in G");
-  double *lds = &G[0][0];
-  __shared__ double cs[32][2];
-  __shared__ int pq[32][2];
+  // after its last K-chunk, and G is dead while J is applied
+  constexpr int GSZ = GramTile::LDS_DOUBLES > PW * LG ? GramTile::LDS_DOUBLES : PW * LG;
+  __shared__ double Gs[GSZ];
+  __shared__ double J[PW][LG];
+  double *lds = Gs;
+  auto G = [&](int r, int c) -> double & { return Gs[r * LG + c]; };
+  __shared__ double cs[NP][2];
+  __shared__ int pq[NP][2];
   __shared__ int rot;
   const int4 it = items[blockIdx.x];
   const TsvdBlock B = blocks[it.x];
   const int nc = B.ell;
-  const int ca = it.y * BJ_W, wa = min(BJ_W, nc - ca);
-  const int cb = it.z * BJ_W, wb = min(BJ_W, nc - cb);
+  const int ca = it.y * BW, wa = min(BW, nc - ca);
+  const int cb = it.z * BW, wb = min(BW, nc - cb);
   const int W = wa + wb;
   auto col = [&](int i) -> int { return i < wa ? ca + i : cb + (i - wa); };
   const double *M = bf.M + B.moff;
   const int tid = threadIdx.x;
   // (1) Gram matrix of the pair's columns
   {
-    BjGramTile tile;
+    GramTile tile;
     tile.zero();
     auto la = [&](int i, int k) -> double {
       return (i < W && k < nc) ? M[k + (long long)col(i) * nc] : 0.0;
@@ -373,7 +389,8 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
       return (j < W && k < nc) ? M[k + (long long)col(j) * nc] : 0.0;
     };
     tile.template run<false, false>(lds, nc, la, lb);
-    tile.store([&](int i, int j, double v) { G[i][j] = v; });
+    __syncthreads();  // the staging area is read before G overwrites it
+    tile.store([&](int i, int j, double v) { G(i, j) = v; });
   }
   const double tol = fmax(1.0, sqrt((double)nc)) * 2.220446049250313e-16;
   __syncthreads();
@@ -382,11 +399,11 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
   // entries and rotate nothing (later outer sweeps revisit mostly such pairs).
   {
     int mine = 0;
-    for (int e = tid; e < 64 * 64; e += 256) {
-      const int p = e >> 6, q = e & 63;
+    for (int e = tid; e < PW * PW; e += 256) {
+      const int p = e / PW, q = e % PW;
       if (p < q && q < W) {
-        const double gpq = G[p][q];
-        mine |= (gpq != 0.0 && fabs(gpq) > tol * sqrt(G[p][p] * G[q][q]));
+        const double gpq = G(p, q);
+        mine |= (gpq != 0.0 && fabs(gpq) > tol * sqrt(G(p, p) * G(q, q)));
       }
     }
     if (!__syncthreads_or(mine)) {
@@ -394,23 +411,23 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
       return;
     }
   }
-  for (int e = tid; e < 64 * 64; e += 256) J[e >> 6][e & 63] = ((e >> 6) == (e & 63)) ? 1.0 : 0.0;
+  for (int e = tid; e < PW * PW; e += 256) J[e / PW][e % PW] = ((e / PW) == (e % PW)) ? 1.0 : 0.0;
   if (tid == 0) rot = 0;
   __syncthreads();
   int rot_any = 0;
-  // (2) cyclic Jacobi on G (round robin over 64 indices, 32 disjoint pairs)
+  // (2) cyclic Jacobi on G (round robin over PW indices, NP disjoint pairs)
   for (int sweep = 0; sweep < BJ_INNER; sweep++) {
     __syncthreads();
     int any = 0;
-    for (int r = 0; r < 63; r++) {
-      if (tid < 32) {
-        const int pa = tid, pb = 63 - tid;
-        const int a = (pa == 0) ? 0 : ((pa - 1 + r) % 63) + 1;
-        const int b = (pb == 0) ? 0 : ((pb - 1 + r) % 63) + 1;
+    for (int r = 0; r < PW - 1; r++) {
+      if (tid < NP) {
+        const int pa = tid, pb = PW - 1 - tid;
+        const int a = (pa == 0) ? 0 : ((pa - 1 + r) % (PW - 1)) + 1;
+        const int b = (pb == 0) ? 0 : ((pb - 1 + r) % (PW - 1)) + 1;
         const int p = min(a, b), q = max(a, b);
         double c = 1.0, sn = 0.0;
         if (q < W) {
-          const double gpp = G[p][p], gqq = G[q][q], gpq = G[p][q];
+          const double gpp = G(p, p), gqq = G(q, q), gpq = G(p, q);
           if (gpq != 0.0 && fabs(gpq) > tol * sqrt(gpp * gqq)) {
             const double zeta = (gqq - gpp) / (2.0 * gpq);
             const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -428,23 +445,23 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
       // G <- R^T G R in one pass: thread owns whole 2 x 2 blocks (rows
       // p_i, q_i of pair i x columns p_j, q_j of pair j), so the update is
       // in place with no barrier between the row and the column rotation
-      for (int e = tid; e < 32 * 32; e += 256) {
-        const int ti = e >> 5, tj = e & 31;
+      for (int e = tid; e < NP * NP; e += 256) {
+        const int ti = e / NP, tj = e % NP;
         const double si = cs[ti][1], sj = cs[tj][1];
         if (si != 0.0 || sj != 0.0) {
           const double ci = cs[ti][0], cj = cs[tj][0];
           const int pi = pq[ti][0], qi = pq[ti][1], pj = pq[tj][0], qj = pq[tj][1];
-          const double b00 = G[pi][pj], b01 = G[pi][qj], b10 = G[qi][pj], b11 = G[qi][qj];
+          const double b00 = G(pi, pj), b01 = G(pi, qj), b10 = G(qi, pj), b11 = G(qi, qj);
           const double r00 = ci * b00 - si * b10, r01 = ci * b01 - si * b11;
           const double r10 = si * b00 + ci * b10, r11 = si * b01 + ci * b11;
-          G[pi][pj] = cj * r00 - sj * r01;
-          G[pi][qj] = sj * r00 + cj * r01;
-          G[qi][pj] = cj * r10 - sj * r11;
-          G[qi][qj] = sj * r10 + cj * r11;
+          G(pi, pj) = cj * r00 - sj * r01;
+          G(pi, qj) = sj * r00 + cj * r01;
+          G(qi, pj) = cj * r10 - sj * r11;
+          G(qi, qj) = sj * r10 + cj * r11;
         }
       }
-      for (int e = tid; e < 32 * 64; e += 256) {  // columns p, q of J
-        const int t = e >> 6, k = e & 63;
+      for (int e = tid; e < NP * PW; e += 256) {  // columns p, q of J
+        const int t = e / PW, k = e % PW;
         const double sn = cs[t][1];
         if (sn != 0.0) {
           const double c = cs[t][0];
@@ -456,7 +473,7 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
       }
       __syncthreads();
     }
-    // any rotation in this sweep (only threads < 32 decide rotations)
+    // any rotation in this sweep (only threads < NP decide rotations)
     if (any) rot = 1;
     __syncthreads();
     const int go = rot;
@@ -469,37 +486,40 @@ __global__ __launch_bounds__(256) void k_bjac_pair(TsvdBlock *blocks, const int4
   __syncthreads();
   // (3) J goes out for k_bjac_apply (one workgroup per 64-row tile of M_S
   // and of V_S), with whether it differs from the identity
-  double *Jg = jout + (size_t)blockIdx.x * 4096;
-  for (int e = tid; e < 64 * 64; e += 256) Jg[e] = J[e >> 6][e & 63];
+  double *Jg = jout + (size_t)blockIdx.x * (PW * PW);
+  for (int e = tid; e < PW * PW; e += 256) Jg[e] = J[e / PW][e % PW];
   if (tid == 0) rotated[blockIdx.x] = rot_any;
 }
 
 // X_S <- X_S J for one 64-row tile of M (side 0) or V (side 1) of one pair:
 // in place (the tile's rows are read whole before they are written; pairs
 // of one round touch disjoint columns).
+template <int BW>
 __global__ __launch_bounds__(256) void k_bjac_apply(TsvdBlock *blocks, const int4 *items,
                                                     TsvdBufs bf, const double *jin,
                                                     const int *rotated) {
-  __shared__ double lds[BjApplyTile::LDS_DOUBLES];
+  constexpr int PW = 2 * BW;
+  typedef typename BjTiles<BW>::Apply ApplyTile;
+  __shared__ double lds[ApplyTile::LDS_DOUBLES];
   if (!rotated[blockIdx.x]) return;
   const int4 it = items[blockIdx.x];
   const TsvdBlock B = blocks[it.x];
   const int nc = B.ell;
   const int r0 = blockIdx.y * 64;
   if (r0 >= nc) return;
-  const int ca = it.y * BJ_W, wa = min(BJ_W, nc - ca);
-  const int cb = it.z * BJ_W, wb = min(BJ_W, nc - cb);
+  const int ca = it.y * BW, wa = min(BW, nc - ca);
+  const int cb = it.z * BW, wb = min(BW, nc - cb);
   const int W = wa + wb;
   auto col = [&](int i) -> int { return i < wa ? ca + i : cb + (i - wa); };
   double *X = (blockIdx.z ? bf.Vj : bf.M) + B.moff;
-  const double *Jg = jin + (size_t)blockIdx.x * 4096;
-  BjApplyTile tile;
+  const double *Jg = jin + (size_t)blockIdx.x * (PW * PW);
+  ApplyTile tile;
   tile.zero();
   auto la = [&](int i, int k) -> double {
     return (r0 + i < nc && k < W) ? X[(r0 + i) + (long long)col(k) * nc] : 0.0;
   };
-  auto lb = [&](int k, int j) -> double { return Jg[k * 64 + j]; };
-  tile.template run<true, false>(lds, 64, la, lb);
+  auto lb = [&](int k, int j) -> double { return Jg[k * PW + j]; };
+  tile.template run<true, false>(lds, PW, la, lb);
   tile.store([&](int i, int j, double v) {
     if (r0 + i < nc && j < W) X[(r0 + i) + (long long)col(j) * nc] = v;
   });
@@ -557,7 +577,7 @@ static int svd_cores(TsvdBlock *d_blocks, const std::vector<TsvdBlock> &blocks,
     if (rc == SUCCESS) { mem.push_back(d_items); rc = to_device(&d_big, big, s); }
     if (rc == SUCCESS) { mem.push_back(d_big); rc = dalloc(&d_flags, blocks.size()); }
     if (rc == SUCCESS) { mem.push_back(d_flags); rc = dalloc(&d_rot, maxn); }
-    if (rc == SUCCESS) { mem.push_back(d_rot); rc = dalloc(&d_j, maxn * 4096); }
+    if (rc == SUCCESS) { mem.push_back(d_rot); rc = dalloc(&d_j, maxn * 4 * BJ_W * BJ_W); }
     if (rc == SUCCESS) mem.push_back(d_j);
     if (rc != SUCCESS) {
       cleanup();
@@ -572,10 +592,10 @@ static int svd_cores(TsvdBlock *d_blocks, const std::vector<TsvdBlock> &blocks,
       for (int r = 0; r < maxr; r++) {
         const unsigned n = (unsigned)(roff[r + 1] - roff[r]);
         if (n) {
-          hipLaunchKernelGGL(k_bjac_pair, dim3(n), dim3(256), 0, s, d_blocks, d_items + roff[r],
-                             bf, d_flags, d_j, d_rot);
-          hipLaunchKernelGGL(k_bjac_apply, dim3(n, row_tiles, 2), dim3(256), 0, s, d_blocks,
-                             d_items + roff[r], bf, d_j, d_rot);
+          hipLaunchKernelGGL(k_bjac_pair<BJ_W>, dim3(n), dim3(256), 0, s, d_blocks,
+                             d_items + roff[r], bf, d_flags, d_j, d_rot);
+          hipLaunchKernelGGL(k_bjac_apply<BJ_W>, dim3(n, row_tiles, 2), dim3(256), 0, s,
+                             d_blocks, d_items + roff[r], bf, d_j, d_rot);
         }
       }
       hipError_t e = hipMemcpyAsync(hflags.data(), d_flags, blocks.size() * sizeof(int),
