@@ -201,6 +201,11 @@ def _dense_rows(kind, pts, ell, shift, rows):
     return a
 
 
+# (format, slice count) of the mixed-precision products measured
+PRECISIONS = [("bf16", 1), ("bf16", 2), ("bf16", 3), ("bf16", 4),
+              ("int8", 1), ("int8", 2), ("int8", 3), ("int8", 4), ("fp32", 1)]
+
+
 def _time_batch(D, b, steps=20):
     b.run()
     b.tune()
@@ -216,13 +221,14 @@ def _time_batch(D, b, steps=20):
 
 def mixed_batch_table(D, np, trees, xs, fp64, t_plain, t_batched, trans=0):
     """Batched products with every tree's off-diagonal factors in S = 1..4
-    BF16 slices (diagonal leaves FP64): time per product (HIP graph replay)
-    vs the per-tree FP64 calls (the plain form) and the batched FP64
-    program, and the largest error over the trees vs their FP64 product."""
+    BF16 or INT8 slices or one FP32 copy (diagonal leaves FP64): time per
+    product (HIP graph replay) vs the per-tree FP64 calls (the plain form)
+    and the batched FP64 program, and the largest error over the trees vs
+    their FP64 product."""
     table = {}
-    for S in (1, 2, 3, 4):
+    for fmt, S in PRECISIONS:
         for t in trees:
-            t.set_precision(S)
+            t.set_precision(S, fmt)
         ys = [D.DeviceArray(x.shape) for x in xs]
         b = D.Batch(trees, xs, ys, trans=trans)
         t_s = _time_batch(D, b)
@@ -231,7 +237,7 @@ def mixed_batch_table(D, np, trees, xs, fp64, t_plain, t_batched, trans=0):
         b.free()
         for y in ys:
             y.free()
-        table[f"bf16x{S}"] = {"us_per_step": t_s * 1e6,
+        table[f"{fmt}x{S}"] = {"us_per_step": t_s * 1e6,
                               "speedup_vs_per_tree_fp64": t_plain / t_s,
                               "speedup_vs_batched_fp64": t_batched / t_s,
                               "max_rel_err_vs_fp64": err}
@@ -382,8 +388,9 @@ def batch_leg(D, WL, np):
 
 def precision_leg(D, WL, np):
     """inv2d_64k (N=65536 Morton 2-D points, 1/(r+1e-2), height 9, thr 1e-8,
-    R=256): FP64 products vs 1..4 BF16 slices for the off-diagonal factors;
-    error vs the FP64 product of the same tree and vs FP64 dense rows."""
+    R=256): FP64 products vs 1..4 BF16 or INT8 slices or one FP32 copy of the
+    off-diagonal factors; error vs the FP64 product of the same tree and vs
+    FP64 dense rows."""
     w = WL.WORKLOADS["inv2d_64k"]
     n, h, R = w.n, w.height, w.rhs
     pts = w.points()
@@ -415,11 +422,11 @@ def precision_leg(D, WL, np):
     e64 = float(np.linalg.norm(y64[rows] - ref_rows) / np.linalg.norm(ref_rows))
     table = {"f64": {"us_per_step": t64 * 1e6, "rows_rhs_per_s": n * R / t64,
                      "rel_err_vs_fp64_tree": 0.0, "rel_err_vs_dense_rows": e64}}
-    for S in (1, 2, 3, 4):
-        dt.set_precision(S)
+    for fmt, S in PRECISIONS:
+        dt.set_precision(S, fmt)
         t = timed()
         y = dy.numpy()
-        table[f"bf16x{S}"] = {
+        table[f"{fmt}x{S}"] = {
             "us_per_step": t * 1e6, "rows_rhs_per_s": n * R / t, "speedup_vs_f64": t64 / t,
             "rel_err_vs_fp64_tree": float(np.linalg.norm(y - y64) / np.linalg.norm(y64)),
             "rel_err_vs_dense_rows": float(np.linalg.norm(y[rows] - ref_rows) / np.linalg.norm(ref_rows))}

@@ -184,6 +184,8 @@ def _declare_hip_api(lib):
         "hm_hip_selftest_mfma": ([PDouble], c_int),
         "hm_hip_selftest_mfma_bf16": ([POINTER(C.c_float), POINTER(C.c_float),
                                        POINTER(C.c_float)], c_int),
+        "hm_hip_selftest_mfma_lowp": ([c_int, POINTER(C.c_float), POINTER(C.c_float),
+                                       POINTER(C.c_float)], c_int),
         "hm_hip_mlp_create": ([c_int, c_int, C.c_ulonglong, POINTER(V)], c_int),
         "hm_hip_mlp_destroy": ([V], c_int),
         "hm_hip_mlp_nparams": ([V], C.c_longlong),
@@ -206,6 +208,7 @@ def _declare_hip_api(lib):
         "hm_hip_batch_variants": ([V, POINTER(c_int), c_int], c_int),
         "hm_hip_batch_destroy": ([V], c_int),
         "hm_hip_tree_set_precision": ([V, c_int, V], c_int),
+        "hm_hip_tree_set_format": ([V, c_int, c_int, V], c_int),
         "hm_hip_kernel_timing_dump": ([C.c_char_p, c_int], c_int),
         "hm_hip_qr_batch": ([c_int, PInt, PInt, POINTER(V), POINTER(c_long), POINTER(V),
                              POINTER(c_long), POINTER(V), POINTER(c_long), V], c_int),

@@ -164,9 +164,10 @@ struct hm_dtree {
   hm::ProductPlan plan;
   double *d_scratch = nullptr;        // P then W, grown on demand
   size_t scratch_elems = 0;
-  // sliced (BF16) copies of U and V for mixed-precision products; slices =
-  // 0 means FP64 products
+  // low-precision copies of U and V for mixed-precision products (format
+  // slice_fmt: 0 BF16, 1 INT8, 2 FP32 slices); slices = 0 means FP64 products
   int slices = 0;
+  int slice_fmt = 0;
   struct MixedFactors *mixed = nullptr;
 };
 
@@ -183,7 +184,7 @@ int ensure_scratch(hm_dtree *t, int R);
 long long scratch_need(const hm_dtree *t, int R);
 void tree_free(hm_dtree *t);
 void mixed_free(MixedFactors *m);
-// Y = op(H) X with sliced BF16 factors (t->slices > 0; op = transpose
+// Y = op(H) X with sliced factors (t->slices > 0; op = transpose
 // when trans); X, Y element (row, col) at [row * rs + col * cs].
 int mixed_matmat(hm_dtree *t, int trans, int R, const double *X, long x_rs, long x_cs,
                  double *Y, long y_rs, long y_cs, hipStream_t s);

@@ -1,26 +1,45 @@
 // Mixed-precision HODLR x dense: the off-diagonal factors and products on
-// BF16 MFMA with exact FP32 accumulation ("sliced" FP64), diagonal leaves in
-// FP64.
+// low-precision MFMA with exact (or FP32) accumulation folded into FP64,
+// diagonal leaves in FP64.
 //
-// An FP64 operand is split, per row of K-chunk (at most MX_KCH entries
-// along the summed dimension), into S integer slices q_a with |q_a| <= 128:
-//   x = 2^e (q_0 2^-7 + q_1 2^-15 + ... + q_{S-1} 2^-(8S-1)) + r,
-//   |r| <= 2^(e - 8S), with 2^e > max |x| over the chunk row.
-// Integers of magnitude <= 256 are exact in BF16, products of two slices
-// are integers <= 2^14, and the MFMA sums them in FP32 exactly as long as
-// the running sum stays below 2^24: the slice pairs (a, b) with a + b = d
-// share one FP32 accumulator, at most S of them, over a chunk of MX_KCH
-// entries (S * 2^14 * 128 <= 2^23).  At the end of every chunk the S
-// accumulators are scaled by 2^(eA + eB - 14 - 8d) and added in FP64.
-// Every slice carries 8 bits: the representation error of an entry is at
-// most 2^-8S of its chunk maximum, and the dropped pairs (a + b >= S) are
-// of the same order, so S = 1 is BF16-like, S = 3 FP32-like (2^-24) and
-// S = 4 (10 MFMAs per 16 x 16 x 32 step) reaches 2^-32.
+// Formats of the factor copies (MxFmt):
+//  * BF16 slices.  An FP64 operand is split, per row of a K-chunk (at most
+//    128 entries along the summed dimension), into S integer slices q_a with
+//    |q_a| <= 128:
+//      x = 2^e (q_0 2^-7 + q_1 2^-15 + ... + q_{S-1} 2^-(8S-1)) + r,
+//      |r| <= 2^(e - 8S), with 2^e > max |x| over the chunk row.
+//    Integers of magnitude <= 256 are exact in BF16, products of two slices
+//    are integers <= 2^14, and v_mfma_f32_16x16x32_bf16 sums them in FP32
+//    exactly as long as the running sum stays below 2^24: the slice pairs
+//    (a, b) with a + b = d share one FP32 accumulator, at most S of them,
+//    over a chunk of 128 entries (S * 2^14 * 128 <= 2^23).
+//  * INT8 slices.  The same split with balanced base-254 digits, per row of
+//    a 256-entry chunk: q_0 = rint(127 x / 2^e), then each remainder
+//    (in [-1/2, 1/2]) times 254 rounded, so |q_a| <= 127 fits int8 and
+//      x = 2^e / 127 (q_0 + q_1 / 254 + ... + q_{S-1} / 254^(S-1)) + r,
+//      |r| <= 2^e / (254 * 254^(S-1)),
+//    about 8 bits per slice as for BF16.  v_mfma_i32_16x16x64_i8 sums the
+//    products in int32 exactly (S * 127^2 * 256 < 2^31), with twice the
+//    BF16 rate per instruction and one byte per slice entry.
+//  * FP32 (one "slice"): x / 2^e rounded to FP32 per row of a 32-entry
+//    chunk (the scale is exact and only keeps the layout shared), products on
+//    v_mfma_f32_16x16x4_f32 accumulated in FP32 over the chunk.  FP32
+//    rounding of the operands and of the accumulation bounds the relative
+//    error near 1e-7: FP32 MFMA runs at twice the FP64 rate, and no
+//    correction term helps, since the FP32 accumulation rounds at 2^-24
+//    whatever the operands carry.
+// At the end of every chunk the S accumulators d = S-1 .. 0 are combined in
+// FP64 (Horner in the digit base) and added with the row and column scales.
+// Pairs with a + b >= S are dropped (their size is that of the
+// representation error), so S slices cost S(S+1)/2 MFMAs per k-step and
+// carry about 8S bits of each chunk row's maximum: S = 1 is BF16-like,
+// S = 3 FP32-like (2^-24) and S = 4 reaches 2^-32.
 //
-// Operand storage ("chunk-padded"): along K every chunk starts at a
-// multiple of 32 and is zero-padded to one, so MFMA k-steps never straddle
-// chunks and loads are 16-byte aligned; slices of one operand follow each
-// other at a fixed stride.  Scales are stored per (chunk, row).
+// Operand storage ("chunk-padded", bytes): along K every chunk starts at a
+// multiple of one k-step (64 bytes: 32 BF16, 64 INT8 or 16 FP32 entries)
+// and is zero-padded to one, so MFMA k-steps never straddle chunks and loads
+// are 16-byte aligned; slices of one operand follow each other at a fixed
+// stride.  Scales are stored per (chunk, row).
 //
 // The product Y = H X (no transpose, X and Y column-major) runs:
 //   slice X   (per leaf chunk and column)
@@ -39,41 +58,81 @@
 
 namespace hm {
 
-static constexpr int MX_KCH = 128;      // exactness chunk along K
-static constexpr int MX_SEG_CHUNKS = 32;  // W split-K segment (chunks)
+enum MxFmt { MX_BF16 = 0, MX_I8 = 1, MX_F32 = 2 };
+// Entry size (bytes), entries per 64-byte k-step, chunk length (entries)
+// of each format, on the host.
+static inline int fmt_esz(int f) { return f == MX_I8 ? 1 : (f == MX_F32 ? 4 : 2); }
+static inline int fmt_ks(int f) { return 64 / fmt_esz(f); }
+static inline int fmt_kch(int f) { return f == MX_I8 ? 256 : (f == MX_F32 ? 32 : 128); }
+static constexpr int MX_SEG_K = 4096;  // W split-K segment (entries along K)
+
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
 // global (not generic) loads, so that LDS waits do not wait for them
-__device__ __forceinline__ bf8 ldg8(const __bf16 *p) {
-  return *(const __attribute__((address_space(1))) bf8 *)(p);
+__device__ __forceinline__ i4 ldg16(const unsigned char *p) {
+  return *(const __attribute__((address_space(1))) i4 *)(p);
 }
 __device__ __forceinline__ int ldgi(const int *p) {
   return *(const __attribute__((address_space(1))) int *)(p);
 }
 
-__device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
+// Per-format device side: accumulator type, the MFMAs of one 16 x 16 block
+// for one 16-byte fragment per lane of A (row L % 16) and of B (column
+// L % 16), and the fold constants (digit base, scale of a product).
+template <int F> struct MxDev;
+template <> struct MxDev<MX_BF16> {
+  static constexpr int KS = 32;
+  using acc_t = f4;
+  static constexpr double BASE = 0x1p-8, PROD = 0x1p-14;
+  __device__ static __forceinline__ acc_t mma(i4 a, i4 b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                   __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+  }
+};
+template <> struct MxDev<MX_I8> {
+  static constexpr int KS = 64;
+  using acc_t = i4;
+  static constexpr double BASE = 1.0 / 254.0, PROD = 1.0 / (127.0 * 127.0);
+  __device__ static __forceinline__ acc_t mma(i4 a, i4 b, acc_t c) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MxDev<MX_F32> {
+  static constexpr int KS = 16;
+  using acc_t = f4;
+  static constexpr double BASE = 1.0, PROD = 1.0;
+  // lane L holds entries 4(L/16) .. 4(L/16)+3 of its row (A) / column (B);
+  // MFMA j takes entry j of every lane, so each of the 16 k is summed once
+  __device__ static __forceinline__ acc_t mma(i4 a, i4 b, acc_t c) {
+    const f4 fa = __builtin_bit_cast(f4, a), fb = __builtin_bit_cast(f4, b);
+#pragma unroll
+    for (int j = 0; j < 4; j++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[j], fb[j], c, 0, 0, 0);
+    return c;
+  }
+};
 
 // ------------------------------------------------------------ slicing
 // One wavefront per item: chunk [k0, k0 + klen) of row p of the source
-// (element (p, k) at src[p * sp + k * sk]) -> S slices at
-// dst + a * dst_slice + p * dst_ld + dk0 (klen padded to a multiple of 32
-// with zeros), scale 2^e to *scale.
+// (element (p, k) at src[p * sp + k * sk]) -> S slices at byte
+// dst + (a * dst_slice + p * dst_ld + dk0) * entry size (klen padded to a
+// whole k-step with zeros), scale 2^e to *scale.
 struct SliceItem {
   const double *src;
   long long sp, sk;
-  __bf16 *dst;
+  unsigned char *dst;
   long long dst_slice, dst_ld, dk0;
   double *scale;
   int p, k0, klen, pad;
 };
 
-__global__ __launch_bounds__(256) void k_slice(const SliceItem *items, int nitems, int S) {
+__global__ __launch_bounds__(256) void k_slice(const SliceItem *items, int nitems, int S,
+                                               int fmt) {
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (w >= nitems) return;
   const SliceItem it = items[w];
-  const int kp = (it.klen + 31) & ~31;
+  const int ks = fmt == MX_I8 ? 64 : (fmt == MX_F32 ? 16 : 32);
+  const int kp = (it.klen + ks - 1) / ks * ks;
   double mx = 0.0;
   for (int k = lane; k < it.klen; k += 64)
     mx = fmax(mx, fabs(it.src[(long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk]));
@@ -85,30 +144,47 @@ __global__ __launch_bounds__(256) void k_slice(const SliceItem *items, int nitem
   const double sc = ldexp(1.0, e);
   if (lane == 0) *it.scale = sc;
   const double inv = ldexp(1.0, -e);
-  __bf16 *d = it.dst + (long long)it.p * it.dst_ld + it.dk0;
+  const long long base = (long long)it.p * it.dst_ld + it.dk0;
   for (int k = lane; k < kp; k += 64) {
     double x = k < it.klen
                    ? it.src[(long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk] * inv
                    : 0.0;
-    // x in (-1, 1): q_0 = rint(x 2^7), then the remainder scaled by 2^8
-    x *= 128.0;
-    for (int a = 0; a < S; a++) {
-      const double q = rint(x);
-      d[(long long)a * it.dst_slice + k] = (__bf16)(float)q;
-      x = (x - q) * 256.0;
+    // x in (-1, 1)
+    if (fmt == MX_BF16) {
+      // q_0 = rint(x 2^7), then the remainder scaled by 2^8
+      __bf16 *d = reinterpret_cast<__bf16 *>(it.dst) + base;
+      x *= 128.0;
+      for (int a = 0; a < S; a++) {
+        const double q = rint(x);
+        d[(long long)a * it.dst_slice + k] = (__bf16)(float)q;
+        x = (x - q) * 256.0;
+      }
+    } else if (fmt == MX_I8) {
+      // q_0 = rint(127 x), then the remainder (in [-1/2, 1/2]) times 254
+      signed char *d = reinterpret_cast<signed char *>(it.dst) + base;
+      x *= 127.0;
+      for (int a = 0; a < S; a++) {
+        const double q = rint(x);
+        d[(long long)a * it.dst_slice + k] = (signed char)(int)q;
+        x = (x - q) * 254.0;
+      }
+    } else {
+      float *d = reinterpret_cast<float *>(it.dst) + base;
+      d[k] = (float)x;
     }
   }
 }
 
 // ---------------------------------------------------------- sliced GEMM
 // One segment: C += op(A) op(B) over K entries given as chunks
-// [cs[c], cs[c+1]) (padded, multiples of 32) of the chunk table.
+// [cs[c], cs[c+1]) (padded, whole k-steps) of the chunk table.  Offsets and
+// strides count entries of the format.
 struct MxSeg {
-  const __bf16 *A;   // element (row p, k) of slice a: A[a * a_slice + p * lda + k]
+  const unsigned char *A;  // entry (row p, k) of slice a: A[a * a_slice + p * lda + k]
   long long lda, a_slice;
   const double *sA;  // scale of (chunk c, row p): sA[(a_c0 + c) * ldsa + p]
   long long ldsa;
-  const __bf16 *B;   // element (k, col j) of slice b: B[b * b_slice + j * ldb + k]
+  const unsigned char *B;  // entry (k, col j) of slice b: B[b * b_slice + j * ldb + k]
   long long ldb, b_slice;
   const double *sB;
   long long ldsb;
@@ -124,28 +200,31 @@ struct MxItem {
   int M, N, i0, j0, s0, s1, mode, pad;
 };
 
-// MX_TM x MX_TN tile per workgroup (2 x 2 waves of 32 x 32), K-steps of
-// 32: each step's S slices of the A rows and B columns go global ->
-// registers (fetched one step ahead) -> one of two LDS stages, and every
+// MX_TM x MX_TN tile per workgroup (2 x 2 waves of 32 x 32), k-steps of 64
+// bytes per row: each step's S slices of the A rows and B columns go global
+// -> registers (fetched one step ahead) -> one of two LDS stages, and every
 // wave reads its MFMA fragments from LDS; the stage written during a step is
 // read in the next, so one barrier per step orders both.  The row and
 // column scales of a chunk are loaded into registers when the chunk starts
-// and used by its fold.  LDS rows are padded to MX_RS entries.
-static constexpr int MX_RS = 40;
+// and used by its fold.  LDS rows are padded to MX_RSB bytes.
+static constexpr int MX_RSB = 80;
 static constexpr int MX_TM = 64, MX_TN = 64;
 
-template <int S>
+template <int F, int S>
 __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const MxSeg *segs) {
+  using D = MxDev<F>;
+  using acc_t = typename D::acc_t;
+  constexpr int KS = D::KS, ESZ = 64 / KS;
   constexpr int FM = 2, FN = 2;  // 16 x 16 blocks per wave
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][S][MX_TM * MX_RS];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][S][MX_TN * MX_RS];
+  __shared__ __attribute__((aligned(16))) unsigned char As[2][S][MX_TM * MX_RSB];
+  __shared__ __attribute__((aligned(16))) unsigned char Bs[2][S][MX_TN * MX_RSB];
   const MxItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
   const int L = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int li = L & 15, lk = L >> 4;
-  // global loads: A row lr, B row lr; 8 entries at kq
-  const int lr = tid >> 2, kq = (tid & 3) * 8;
+  // global loads: A row lr, B row lr; 16 bytes at byte kq
+  const int lr = tid >> 2, kq = (tid & 3) * 16;
   const bool ldA0 = it.i0 + lr < it.M, ldB = it.j0 + lr < it.N;
   double acc64[FM][FN][4];
 #pragma unroll
@@ -157,21 +236,21 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
   for (int sg = it.s0; sg < it.s1; sg++) {
     const MxSeg sd = segs[sg];
     if (sd.nch <= 0) continue;
-    const __bf16 *ga = sd.A + (long long)(it.i0 + lr) * sd.lda + kq;
-    const __bf16 *gb = sd.B + (long long)(it.j0 + lr) * sd.ldb + kq;
-    bf8 ra0[S], rb[S];
+    const unsigned char *ga = sd.A + (long long)(it.i0 + lr) * sd.lda * ESZ + kq;
+    const unsigned char *gb = sd.B + (long long)(it.j0 + lr) * sd.ldb * ESZ + kq;
+    i4 ra0[S], rb[S];
     auto fetch = [&](int k0) {
 #pragma unroll
       for (int a = 0; a < S; a++) {
-        ra0[a] = ldA0 ? ldg8(ga + a * sd.a_slice + k0) : bf8{};
-        rb[a] = ldB ? ldg8(gb + a * sd.b_slice + k0) : bf8{};
+        ra0[a] = ldA0 ? ldg16(ga + (a * sd.a_slice + k0) * ESZ) : i4{};
+        rb[a] = ldB ? ldg16(gb + (a * sd.b_slice + k0) * ESZ) : i4{};
       }
     };
     auto stage = [&](int buf) {
 #pragma unroll
       for (int a = 0; a < S; a++) {
-        *reinterpret_cast<bf8 *>(&As[buf][a][lr * MX_RS + kq]) = ra0[a];
-        *reinterpret_cast<bf8 *>(&Bs[buf][a][lr * MX_RS + kq]) = rb[a];
+        *reinterpret_cast<i4 *>(&As[buf][a][lr * MX_RSB + kq]) = ra0[a];
+        *reinterpret_cast<i4 *>(&Bs[buf][a][lr * MX_RSB + kq]) = rb[a];
       }
     };
     double sa[FM][4], sb[FN];
@@ -191,45 +270,45 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
     };
     const int kbeg = ldgi(sd.cs), kend = ldgi(sd.cs + sd.nch);
     int c = 0, cend = ldgi(sd.cs + 1);
-    f4 acc[S][FM][FN];
+    acc_t acc[S][FM][FN];
 #pragma unroll
     for (int d = 0; d < S; d++)
 #pragma unroll
       for (int f = 0; f < FM; f++)
 #pragma unroll
-        for (int g = 0; g < FN; g++) acc[d][f][g] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int g = 0; g < FN; g++) acc[d][f][g] = acc_t{};
     fetch(kbeg);
     fetch_scales(0);
     __syncthreads();  // the previous segment's last stage has been read
     stage(0);
     __syncthreads();
     int buf = 0;
-    for (int k0 = kbeg; k0 < kend; k0 += 32) {
-      const bool more = k0 + 32 < kend;
-      if (more) fetch(k0 + 32);
-      bf8 fb[S][FN];
+    for (int k0 = kbeg; k0 < kend; k0 += KS) {
+      const bool more = k0 + KS < kend;
+      if (more) fetch(k0 + KS);
+      i4 fb[S][FN];
 #pragma unroll
       for (int a = 0; a < S; a++)
 #pragma unroll
         for (int g = 0; g < FN; g++)
           fb[a][g] =
-              *reinterpret_cast<const bf8 *>(&Bs[buf][a][(wn * 32 + g * 16 + li) * MX_RS + 8 * lk]);
+              *reinterpret_cast<const i4 *>(&Bs[buf][a][(wn * 32 + g * 16 + li) * MX_RSB + 16 * lk]);
 #pragma unroll
       for (int f = 0; f < FM; f++) {
-        bf8 fa[S];
+        i4 fa[S];
 #pragma unroll
         for (int a = 0; a < S; a++)
           fa[a] =
-              *reinterpret_cast<const bf8 *>(&As[buf][a][(wm * 32 + f * 16 + li) * MX_RS + 8 * lk]);
+              *reinterpret_cast<const i4 *>(&As[buf][a][(wm * 32 + f * 16 + li) * MX_RSB + 16 * lk]);
 #pragma unroll
         for (int a = 0; a < S; a++)
 #pragma unroll
           for (int b = 0; a + b < S; b++)
 #pragma unroll
             for (int g = 0; g < FN; g++)
-              acc[a + b][f][g] = mfma_bf16(fa[a], fb[b][g], acc[a + b][f][g]);
+              acc[a + b][f][g] = D::mma(fa[a], fb[b][g], acc[a + b][f][g]);
       }
-      if (k0 + 32 == cend) {
+      if (k0 + KS == cend) {
         // fold the chunk into FP64 with its row and column scales
 #pragma unroll
         for (int f = 0; f < FM; f++)
@@ -239,15 +318,15 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
             for (int v = 0; v < 4; v++) {
               double t = 0.0;
 #pragma unroll
-              for (int d = S - 1; d >= 0; d--) t = t * 0x1p-8 + (double)acc[d][f][g][v];
-              acc64[f][g][v] += sa[f][v] * sb[g] * 0x1p-14 * t;
+              for (int d = S - 1; d >= 0; d--) t = t * D::BASE + (double)acc[d][f][g][v];
+              acc64[f][g][v] += sa[f][v] * sb[g] * D::PROD * t;
             }
 #pragma unroll
         for (int d = 0; d < S; d++)
 #pragma unroll
           for (int f = 0; f < FM; f++)
 #pragma unroll
-            for (int g = 0; g < FN; g++) acc[d][f][g] = f4{0.f, 0.f, 0.f, 0.f};
+            for (int g = 0; g < FN; g++) acc[d][f][g] = acc_t{};
         c++;
         if (c < sd.nch) {
           cend = ldgi(sd.cs + c + 1);
@@ -295,14 +374,16 @@ __global__ __launch_bounds__(256) void k_mx_reduce(const MxReduce *items) {
 }  // namespace hm
 
 // ---------------------------------------------------------- tree factors
-// Per tree: the K-chunk grid over the rows (chunks of <= MX_KCH rows inside
-// leaves, shared by both modes) and, per product mode (0: Y = H X, 1:
+// Per tree: the format and slice count, the K-chunk grid over the rows
+// (chunks of at most the format's chunk length inside leaves, shared by both
+// modes) and, per product mode (0: Y = H X, 1:
 // Y = H^T X), the sliced factors: the W side F_in^T (V^T for H X, U^T for
 // H^T X) sliced along the rows of its source node by that grid, the apply
 // side F_out (U for H X, V for H^T X) sliced along the rank by chunks of
-// MX_KCH.  Mode 1 is sliced on its first use.
+// the chunk length.  Mode 1 is sliced on its first use.
 struct MixedFactors {
-  int S = 0;
+  int S = 0, fmt = 0;
+  int esz = 2, ks = 32, kch = 128;  // entry bytes, k-step and chunk (entries)
   // chunk q covers rows [row[q], row[q] + len[q]) and padded coordinates
   // [pad[q], pad[q + 1])
   std::vector<int> row, len, leaf_chunk0;
@@ -319,10 +400,10 @@ struct MixedFactors {
   struct Mode {
     bool ready = false;
     std::vector<Blk> blk;
-    __bf16 *d_ws = nullptr, *d_as = nullptr;
+    unsigned char *d_ws = nullptr, *d_as = nullptr;
     double *d_sw = nullptr, *d_sa = nullptr;
   } mode[2];
-  std::vector<int> s_pad, nrk;  // per block: rank padded to 32, rank chunks
+  std::vector<int> s_pad, nrk;  // per block: rank padded to a k-step, rank chunks
   std::vector<long long> rk_cs;  // per block: offset of its rank chunk table
   int *d_rk = nullptr;           // rank chunk tables (nrk + 1 ints per block)
   // cached single-tree product program (operands fixed)
@@ -351,9 +432,9 @@ void mixed_free(MixedFactors *m) {
   delete m;
 }
 
-static long long pad32(long long x) { return (x + 31) & ~31LL; }
+static long long pad_to(long long x, int ks) { return (x + ks - 1) / ks * ks; }
 
-static int run_slices(const std::vector<SliceItem> &items, int S, hipStream_t s) {
+static int run_slices(const std::vector<SliceItem> &items, int S, int fmt, hipStream_t s) {
   if (items.empty()) return SUCCESS;
   SliceItem *d_items = nullptr;
   HM_OK(dalloc(&d_items, items.size()));
@@ -361,7 +442,7 @@ static int run_slices(const std::vector<SliceItem> &items, int S, hipStream_t s)
                                 hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
     const int n = (int)items.size();
-    hipLaunchKernelGGL(k_slice, dim3((n + 3) / 4), dim3(256), 0, s, d_items, n, S);
+    hipLaunchKernelGGL(k_slice, dim3((n + 3) / 4), dim3(256), 0, s, d_items, n, S, fmt);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -397,8 +478,8 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
     x.a_soff = asoff;
     asoff += (long long)m->nrk[g] * x.rows;
   }
-  HM_OK(dalloc(&md.d_ws, (size_t)woff + 8));
-  HM_OK(dalloc(&md.d_as, (size_t)aoff + 8));
+  HM_OK(dalloc(&md.d_ws, (size_t)woff * m->esz + 64));
+  HM_OK(dalloc(&md.d_as, (size_t)aoff * m->esz + 64));
   HM_OK(dalloc(&md.d_sw, (size_t)wsoff + 1));
   HM_OK(dalloc(&md.d_sa, (size_t)asoff + 1));
   std::vector<SliceItem> items;
@@ -417,7 +498,7 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
         it.src = fin;
         it.sp = fin_ld;
         it.sk = 1;
-        it.dst = md.d_ws + x.w_off;
+        it.dst = md.d_ws + x.w_off * m->esz;
         it.dst_slice = (long long)b.s * x.w_kp;
         it.dst_ld = x.w_kp;
         it.dk0 = m->pad[q] - m->pad[x.q0];
@@ -436,36 +517,40 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
         it.src = fout;
         it.sp = 1;
         it.sk = x.rows;
-        it.dst = md.d_as + x.a_off;
+        it.dst = md.d_as + x.a_off * m->esz;
         it.dst_slice = (long long)x.rows * m->s_pad[g];
         it.dst_ld = m->s_pad[g];
-        it.dk0 = (long long)c * MX_KCH;
+        it.dk0 = (long long)c * m->kch;
         it.scale = md.d_sa + x.a_soff + (long long)c * x.rows + i;
         it.p = i;
-        it.k0 = c * MX_KCH;
-        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
+        it.k0 = c * m->kch;
+        it.klen = std::max(0, std::min(m->kch, b.s - c * m->kch));
         it.pad = 0;
         items.push_back(it);
       }
   }
-  HM_OK(run_slices(items, S, s));
+  HM_OK(run_slices(items, S, m->fmt, s));
   md.ready = true;
   return SUCCESS;
 }
 
-static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
+static int mixed_prepare(hm_dtree *t, int fmt, int S, hipStream_t s) {
   MixedFactors *m = new MixedFactors();
   m->S = S;
+  m->fmt = fmt;
+  m->esz = fmt_esz(fmt);
+  m->ks = fmt_ks(fmt);
+  m->kch = fmt_kch(fmt);
   const int nl = t->nleaf;
   m->leaf_chunk0.assign(nl + 1, 0);
   m->pad.push_back(0);
   for (int c = 0; c < nl; c++) {
     m->leaf_chunk0[c] = (int)m->row.size();
     const int lo = t->leaf_off[c], mc = t->leaf_off[c + 1] - lo;
-    for (int r = 0; r < mc; r += MX_KCH) {
+    for (int r = 0; r < mc; r += m->kch) {
       m->row.push_back(lo + r);
-      m->len.push_back(std::min(MX_KCH, mc - r));
-      m->pad.push_back(m->pad.back() + pad32(m->len.back()));
+      m->len.push_back(std::min(m->kch, mc - r));
+      m->pad.push_back(m->pad.back() + pad_to(m->len.back(), m->ks));
     }
   }
   m->leaf_chunk0[nl] = (int)m->row.size();
@@ -475,10 +560,10 @@ static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
   m->nrk.assign(t->nblock, 0);
   m->rk_cs.assign(t->nblock, 0);
   for (int g = 0; g < t->nblock; g++) {
-    m->s_pad[g] = (int)pad32(t->blk[g].s);
-    m->nrk[g] = (m->s_pad[g] + MX_KCH - 1) / MX_KCH;
+    m->s_pad[g] = (int)pad_to(t->blk[g].s, m->ks);
+    m->nrk[g] = (m->s_pad[g] + m->kch - 1) / m->kch;
     m->rk_cs[g] = (long long)rktab.size();
-    for (int c = 0; c <= m->nrk[g]; c++) rktab.push_back(std::min(c * MX_KCH, m->s_pad[g]));
+    for (int c = 0; c <= m->nrk[g]; c++) rktab.push_back(std::min(c * m->kch, m->s_pad[g]));
   }
   int rc = dalloc(&m->d_rk, rktab.size() + 1);
   if (rc == SUCCESS && !rktab.empty()) {
@@ -497,6 +582,7 @@ static int mixed_prepare(hm_dtree *t, int S, hipStream_t s) {
   mixed_free(t->mixed);
   t->mixed = m;
   t->slices = S;
+  t->slice_fmt = fmt;
   return SUCCESS;
 }
 
@@ -537,9 +623,10 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
   const MixedFactors::Mode &md = m->mode[tr];
   const int S = m->S;
   const int nq = (int)m->row.size();
-  __bf16 *xs = ar.take<__bf16>((size_t)S * R * m->npad + 8);
+  const int esz = m->esz, kch = m->kch;
+  unsigned char *xs = ar.take<unsigned char>((size_t)S * R * m->npad * esz + 64);
   double *sx = ar.take<double>((size_t)nq * R + 1);
-  p.touch(xs, ((size_t)S * R * m->npad) * 2, "mixed X slices");
+  p.touch(xs, (size_t)S * R * m->npad * esz, "mixed X slices");
   for (int q = 0; q < nq; q++)
     for (int j = 0; j < R; j++) {
       SliceItem it;
@@ -571,7 +658,8 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
     if (b.s <= 0) continue;
     double *Wg = W + woff[g];
     const int nch = x.q1 - x.q0;
-    const int nseg = std::max(1, (nch + MX_SEG_CHUNKS - 1) / MX_SEG_CHUNKS);
+    const int seg_chunks = std::max(1, MX_SEG_K / kch);
+    const int nseg = std::max(1, (nch + seg_chunks - 1) / seg_chunks);
     double *part = nseg > 1 ? ar.take<double>((size_t)nseg * b.s * R) : nullptr;
     if (nch == 0) {  // empty source: W_b = 0
       mc.segs.push_back(
@@ -586,18 +674,18 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
       continue;
     }
     for (int sg = 0; sg < nseg; sg++) {
-      const int c0 = sg * MX_SEG_CHUNKS, c1 = std::min(nch, c0 + MX_SEG_CHUNKS);
+      const int c0 = sg * seg_chunks, c1 = std::min(nch, c0 + seg_chunks);
       const long long base = m->pad[x.q0 + c0] - m->pad[x.q0];
       mc.seg_ctab.push_back((long long)mc.ctab.size());
       for (int c = c0; c <= c1; c++)
         mc.ctab.push_back((int)(m->pad[x.q0 + c] - m->pad[x.q0 + c0]));
       MxSeg sd;
-      sd.A = md.d_ws + x.w_off + base;
+      sd.A = md.d_ws + (x.w_off + base) * esz;
       sd.lda = x.w_kp;
       sd.a_slice = (long long)b.s * x.w_kp;
       sd.sA = md.d_sw + x.w_soff;
       sd.ldsa = b.s;
-      sd.B = xs + m->pad[x.q0 + c0];
+      sd.B = xs + m->pad[x.q0 + c0] * esz;
       sd.ldb = m->npad;
       sd.b_slice = (long long)R * m->npad;
       sd.sB = sx;
@@ -618,14 +706,14 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
     if (nseg > 1) mc.red.push_back({part, Wg, b.s, b.s, R, nseg, 0});
   }
   // slices of W, then the apply segments (one per level with a rank)
-  std::vector<__bf16 *> Ws(t->nblock, nullptr);
+  std::vector<unsigned char *> Ws(t->nblock, nullptr);
   std::vector<double *> sW(t->nblock, nullptr);
   for (int g = 0; g < t->nblock; g++) {
     const BlockMeta &b = t->blk[g];
     if (b.s <= 0) continue;
-    Ws[g] = ar.take<__bf16>((size_t)S * R * m->s_pad[g] + 8);
+    Ws[g] = ar.take<unsigned char>((size_t)S * R * m->s_pad[g] * esz + 64);
     sW[g] = ar.take<double>((size_t)m->nrk[g] * R + 1);
-    p.touch(Ws[g], (size_t)S * R * m->s_pad[g] * 2, "mixed W slices");
+    p.touch(Ws[g], (size_t)S * R * m->s_pad[g] * esz, "mixed W slices");
     for (int c = 0; c < m->nrk[g]; c++)
       for (int j = 0; j < R; j++) {
         SliceItem it;
@@ -635,11 +723,11 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
         it.dst = Ws[g];
         it.dst_slice = (long long)R * m->s_pad[g];
         it.dst_ld = m->s_pad[g];
-        it.dk0 = (long long)c * MX_KCH;
+        it.dk0 = (long long)c * kch;
         it.scale = sW[g] + (long long)c * R + j;
         it.p = j;
-        it.k0 = c * MX_KCH;
-        it.klen = std::max(0, std::min(MX_KCH, b.s - c * MX_KCH));
+        it.k0 = c * kch;
+        it.klen = std::max(0, std::min(kch, b.s - c * kch));
         it.pad = 0;
         mc.wsl.push_back(it);
       }
@@ -658,7 +746,7 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
       if (b.s <= 0) continue;
       const int roff = lo - (tr ? b.c0 : b.r0);
       MxSeg sd;
-      sd.A = md.d_as + x.a_off + (long long)roff * m->s_pad[g];
+      sd.A = md.d_as + (x.a_off + (long long)roff * m->s_pad[g]) * esz;
       sd.lda = m->s_pad[g];
       sd.a_slice = (long long)x.rows * m->s_pad[g];
       sd.sA = md.d_sa + x.a_soff + roff;
@@ -688,12 +776,21 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
 }
 
 // The sliced GEMM over nitems work items.
-static void launch_sliced(int S, unsigned nitems, const MxItem *I, const MxSeg *G,
+template <int F>
+static void launch_sliced_f(int S, unsigned nitems, const MxItem *I, const MxSeg *G,
+                            hipStream_t st) {
+  if (S == 1) hipLaunchKernelGGL((k_sliced_gemm<F, 1>), dim3(nitems), dim3(256), 0, st, I, G);
+  if constexpr (F != MX_F32) {  // FP32 has one slice
+    if (S == 2) hipLaunchKernelGGL((k_sliced_gemm<F, 2>), dim3(nitems), dim3(256), 0, st, I, G);
+    if (S == 3) hipLaunchKernelGGL((k_sliced_gemm<F, 3>), dim3(nitems), dim3(256), 0, st, I, G);
+    if (S == 4) hipLaunchKernelGGL((k_sliced_gemm<F, 4>), dim3(nitems), dim3(256), 0, st, I, G);
+  }
+}
+static void launch_sliced(int fmt, int S, unsigned nitems, const MxItem *I, const MxSeg *G,
                           hipStream_t st) {
-  if (S == 1) hipLaunchKernelGGL(k_sliced_gemm<1>, dim3(nitems), dim3(256), 0, st, I, G);
-  if (S == 2) hipLaunchKernelGGL(k_sliced_gemm<2>, dim3(nitems), dim3(256), 0, st, I, G);
-  if (S == 3) hipLaunchKernelGGL(k_sliced_gemm<3>, dim3(nitems), dim3(256), 0, st, I, G);
-  if (S == 4) hipLaunchKernelGGL(k_sliced_gemm<4>, dim3(nitems), dim3(256), 0, st, I, G);
+  if (fmt == MX_BF16) launch_sliced_f<MX_BF16>(S, nitems, I, G, st);
+  if (fmt == MX_I8) launch_sliced_f<MX_I8>(S, nitems, I, G, st);
+  if (fmt == MX_F32) launch_sliced_f<MX_F32>(S, nitems, I, G, st);
 }
 
 // Work order of the sliced GEMM launches: workgroup b runs on XCD b % 8, and
@@ -739,7 +836,7 @@ static std::vector<MxItem> xcd_order(const std::vector<MxItem> &items,
 
 // The launches of the gathered stages: segment fix-up, slice X, W GEMMs,
 // their reduction, slice W, the apply GEMMs (Y += F_out W).
-static void mixed_emit(Program &p, const MixedCollect &mc, int S_) {
+static void mixed_emit(Program &p, const MixedCollect &mc, int F_, int S_) {
   const size_t ctab_off = p.push(mc.ctab);
   const size_t sl_off = p.push(mc.sl), wsl_off = p.push(mc.wsl);
   const size_t red_off = p.push(mc.red);
@@ -766,11 +863,11 @@ static void mixed_emit(Program &p, const MixedCollect &mc, int S_) {
   p.add("slice X", [=](hipStream_t st, const unsigned char *blob) {
     if (nsl)
       hipLaunchKernelGGL(k_slice, dim3((nsl + 3) / 4), dim3(256), 0, st,
-                         reinterpret_cast<const SliceItem *>(blob + sl_off), nsl, S_);
+                         reinterpret_cast<const SliceItem *>(blob + sl_off), nsl, S_, F_);
   });
   p.add("mixed W", [=](hipStream_t st, const unsigned char *blob) {
     if (!nwit) return;
-    launch_sliced(S_, nwit, reinterpret_cast<const MxItem *>(blob + wit_off),
+    launch_sliced(F_, S_, nwit, reinterpret_cast<const MxItem *>(blob + wit_off),
                   reinterpret_cast<const MxSeg *>(blob + seg_off), st);
   });
   p.add("mixed W reduce", [=](hipStream_t st, const unsigned char *blob) {
@@ -782,11 +879,11 @@ static void mixed_emit(Program &p, const MixedCollect &mc, int S_) {
   p.add("slice W", [=](hipStream_t st, const unsigned char *blob) {
     if (nwsl)
       hipLaunchKernelGGL(k_slice, dim3((nwsl + 3) / 4), dim3(256), 0, st,
-                         reinterpret_cast<const SliceItem *>(blob + wsl_off), nwsl, S_);
+                         reinterpret_cast<const SliceItem *>(blob + wsl_off), nwsl, S_, F_);
   });
   p.add("mixed apply", [=](hipStream_t st, const unsigned char *blob) {
     if (!nait) return;
-    launch_sliced(S_, nait, reinterpret_cast<const MxItem *>(blob + ait_off),
+    launch_sliced(F_, S_, nait, reinterpret_cast<const MxItem *>(blob + ait_off),
                   reinterpret_cast<const MxSeg *>(blob + seg_off), st);
   });
 }
@@ -802,17 +899,18 @@ int mixed_ensure_mode(hm_dtree *t, int tr, hipStream_t s) {
 }
 
 int mixed_add(Program &p, Arena &ar, const std::vector<MixedJob> &jobs) {
-  for (int S = 1; S <= 4; S++) {
-    MixedCollect mc;
-    bool any = false;
-    for (const MixedJob &j : jobs) {
-      if (!j.t->mixed || j.t->mixed->S != S || j.R <= 0) continue;
-      HM_OK(mixed_collect(mc, p, ar, j.t, j.trans, j.R, j.X, j.x_rs, j.x_cs, j.Y, j.y_rs,
-                          j.y_cs));
-      any = true;
+  for (int F = MX_BF16; F <= MX_F32; F++)
+    for (int S = 1; S <= 4; S++) {
+      MixedCollect mc;
+      bool any = false;
+      for (const MixedJob &j : jobs) {
+        if (!j.t->mixed || j.t->mixed->fmt != F || j.t->mixed->S != S || j.R <= 0) continue;
+        HM_OK(mixed_collect(mc, p, ar, j.t, j.trans, j.R, j.X, j.x_rs, j.x_cs, j.Y, j.y_rs,
+                            j.y_cs));
+        any = true;
+      }
+      if (any) mixed_emit(p, mc, F, S);
     }
-    if (any) mixed_emit(p, mc, S);
-  }
   return SUCCESS;
 }
 
@@ -869,17 +967,96 @@ int mixed_matmat(hm_dtree *t, int trans, int R, const double *X, long x_rs, long
 
 using namespace hm;
 
-extern "C" int hm_hip_tree_set_precision(hm_dtree *t, int slices, void *stream) {
+namespace hm {
+// One block of format F: lane L loads its 16-byte fragment of A row L % 16
+// and B column L % 16 (entries KS/4 * (L/16) ..), one MxDev<F>::mma.
+template <int F>
+__global__ void k_selftest_lowp(const float *A, const float *B, float *raw) {
+  constexpr int KS = MxDev<F>::KS, E = KS / 4;
+  const int L = threadIdx.x;
+  unsigned char fa[16], fb[16];
+  for (int e = 0; e < E; e++) {
+    const float a = A[(L & 15) * KS + E * (L >> 4) + e];
+    const float b = B[(E * (L >> 4) + e) * 16 + (L & 15)];
+    if constexpr (F == MX_BF16) {
+      reinterpret_cast<__bf16 *>(fa)[e] = (__bf16)a;
+      reinterpret_cast<__bf16 *>(fb)[e] = (__bf16)b;
+    } else if constexpr (F == MX_I8) {
+      reinterpret_cast<signed char *>(fa)[e] = (signed char)(int)a;
+      reinterpret_cast<signed char *>(fb)[e] = (signed char)(int)b;
+    } else {
+      reinterpret_cast<float *>(fa)[e] = a;
+      reinterpret_cast<float *>(fb)[e] = b;
+    }
+  }
+  i4 ra, rb;
+  memcpy(&ra, fa, 16);
+  memcpy(&rb, fb, 16);
+  typename MxDev<F>::acc_t c{};
+  c = MxDev<F>::mma(ra, rb, c);
+  for (int v = 0; v < 4; v++) raw[4 * L + v] = (float)c[v];
+}
+}  // namespace hm
+
+extern "C" int hm_hip_selftest_mfma_lowp(int format, const float *hA, const float *hB,
+                                         float *raw) {
   HM_OK(ensure_init());
-  if (!t || slices < 0 || slices > 4) {
-    set_error("hm_hip_tree_set_precision: slices must be 0 (FP64) or 1..4");
+  if (format < HM_FMT_BF16 || format > HM_FMT_FP32) {
+    set_error("selftest lowp: unknown format %d", format);
     return INPUT_ERROR;
   }
-  if (slices == 0) {
+  const int ks = format == HM_FMT_BF16 ? 32 : (format == HM_FMT_INT8 ? 64 : 16);
+  float *d = nullptr;
+  HM_OK(dalloc(&d, 2 * 16 * ks + 256));
+  hipError_t e = hipMemcpy(d, hA, 16 * ks * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(d + 16 * ks, hB, 16 * ks * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    float *a = d, *b = d + 16 * ks, *r = d + 32 * ks;
+    if (format == HM_FMT_BF16)
+      hipLaunchKernelGGL(k_selftest_lowp<MX_BF16>, dim3(1), dim3(64), 0, 0, a, b, r);
+    if (format == HM_FMT_INT8)
+      hipLaunchKernelGGL(k_selftest_lowp<MX_I8>, dim3(1), dim3(64), 0, 0, a, b, r);
+    if (format == HM_FMT_FP32)
+      hipLaunchKernelGGL(k_selftest_lowp<MX_F32>, dim3(1), dim3(64), 0, 0, a, b, r);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(raw, d + 32 * ks, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  dfree(d);
+  if (e != hipSuccess) {
+    set_error("selftest lowp: %s", hipGetErrorString(e));
+    return ALLOCATION_FAILURE;
+  }
+  return SUCCESS;
+}
+
+extern "C" int hm_hip_tree_set_format(hm_dtree *t, int format, int slices, void *stream) {
+  HM_OK(ensure_init());
+  if (!t || format < HM_FMT_FP64 || format > HM_FMT_FP32) {
+    set_error("hm_hip_tree_set_format: unknown format %d", format);
+    return INPUT_ERROR;
+  }
+  if (format == HM_FMT_FP64) {
     mixed_free(t->mixed);
     t->mixed = nullptr;
     t->slices = 0;
+    t->slice_fmt = 0;
     return SUCCESS;
   }
-  return mixed_prepare(t, slices, stream_of(stream));
+  const int maxs = format == HM_FMT_FP32 ? 1 : 4;
+  if (slices < 1 || slices > maxs) {
+    set_error("hm_hip_tree_set_format: %d slices (1..%d for this format)", slices, maxs);
+    return INPUT_ERROR;
+  }
+  const int fmt = format == HM_FMT_BF16 ? MX_BF16 : (format == HM_FMT_INT8 ? MX_I8 : MX_F32);
+  return mixed_prepare(t, fmt, slices, stream_of(stream));
+}
+
+extern "C" int hm_hip_tree_set_precision(hm_dtree *t, int slices, void *stream) {
+  if (slices < 0 || slices > 4) {
+    set_error("hm_hip_tree_set_precision: slices must be 0 (FP64) or 1..4");
+    return INPUT_ERROR;
+  }
+  return hm_hip_tree_set_format(t, slices ? HM_FMT_BF16 : HM_FMT_FP64, slices, stream);
 }

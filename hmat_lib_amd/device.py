@@ -160,9 +160,18 @@ class DeviceTree:
                                                           None), "multiply_hodlr_hodlr")
         return DeviceTree(out)
 
-    def set_precision(self, slices: int):
-        """0: FP64 products; 1..4: BF16-sliced off-diagonal products."""
-        capi.check(capi.hip().hm_hip_tree_set_precision(self.h, slices, None), "set_precision")
+    FORMATS = {"fp64": 0, "bf16": 1, "int8": 2, "fp32": 3}
+
+    def set_precision(self, slices: int, fmt: str = "bf16"):
+        """Off-diagonal products in FP64 (slices 0, or fmt "fp64"), on 1..4
+        BF16 or INT8 slices of the factors, or on one FP32 copy (fmt "fp32",
+        slices 1)."""
+        if slices == 0 or fmt == "fp64":
+            code, slices = 0, 0
+        else:
+            code = self.FORMATS[fmt]
+        capi.check(capi.hip().hm_hip_tree_set_format(self.h, code, slices, None),
+                   "set_precision")
 
     def download(self, lib, tree_ptr):
         capi.check(capi.hip().hm_hip_tree_download(self.h, tree_ptr, None, None),

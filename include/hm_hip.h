@@ -120,7 +120,7 @@ HM_HIP_API int hm_hip_matmat(hm_dtree *tree, int trans, int R, const double *d_X
  * no hipMalloc / hipFree or other allocating call: a tree whose apply
  * variant for this transpose mode and RHS width is not tuned yet runs
  * variant 1 (no timing runs); trees with sliced factors
- * (hm_hip_tree_set_precision > 0) give INPUT_ERROR, and so does a
+ * (hm_hip_tree_set_precision / hm_hip_tree_set_format) give INPUT_ERROR, and so does a
  * product whose scratch (partials + W, about R * (sum over levels and
  * 64-row chunks of the input-side ranks + sum of ranks) doubles) exceeds
  * 8 Mi doubles.  Every call on one device shares the array and it keeps
@@ -143,6 +143,28 @@ HM_HIP_API int hm_hip_matmat_noalloc(hm_dtree *tree, int trans, int R, const dou
  * A batch created with the tree refuses to run after its precision
  * changes. */
 HM_HIP_API int hm_hip_tree_set_precision(hm_dtree *tree, int slices, void *stream);
+
+/* Formats of the off-diagonal factor copies for hm_hip_tree_set_format. */
+#define HM_FMT_FP64 0 /* no copies: FP64 products (slices ignored) */
+#define HM_FMT_BF16 1 /* 1..4 BF16 slices, as hm_hip_tree_set_precision */
+#define HM_FMT_INT8 2 /* 1..4 INT8 slices: balanced base-254 digits per row of
+                       * 256-entry chunks (|q| <= 127), products on
+                       * v_mfma_i32_16x16x64_i8 summed exactly in int32 and
+                       * folded in FP64; about the same bits per slice as BF16
+                       * (2^-8 .. 2^-32 of the chunk maxima) with half the
+                       * operand bytes and twice the MFMA rate */
+#define HM_FMT_FP32 3 /* one FP32 copy (slices = 1), products on
+                       * v_mfma_f32_16x16x4_f32 accumulated in FP32 over
+                       * 32-entry chunks and folded in FP64: relative error
+                       * near 1e-7 (FP32 rounding of operands and sums) */
+/* Precision of the off-diagonal products of this tree, as
+ * hm_hip_tree_set_precision but with the format chosen: the copies are
+ * built on the device (those for H^T X on the first transposed product),
+ * the diagonal leaves stay FP64, every product form and batches use them,
+ * and a batch created with the tree refuses to run after they change.
+ * INPUT_ERROR for an unknown format or a slice count outside 1..4 (1 for
+ * FP32). */
+HM_HIP_API int hm_hip_tree_set_format(hm_dtree *tree, int format, int slices, void *stream);
 
 /* Apply-kernel variant the autotuner picked for this tree, transpose mode
  * and RHS width (-1 before the first product of that kind): 0-4 tile shapes
@@ -328,6 +350,16 @@ HM_HIP_API int hm_hip_selftest_mfma(double *out);
  * B[8(L/16)+e][L%16] (hA 16x32, hB 32x16 row-major host floats, exact in
  * bf16); raw (host, 256) receives value v of lane L at 4 L + v. */
 HM_HIP_API int hm_hip_selftest_mfma_bf16(const float *hA, const float *hB, float *raw);
+/* One 16 x 16 block of the sliced GEMM of a format (HM_FMT_BF16, _INT8,
+ * _FP32) from one 16-byte fragment per lane, as the kernels issue it: lane L
+ * takes A[L%16][e0 + e] and B[e0 + e][L%16] for its fragment's entries e,
+ * e0 = (entries per fragment) * (L/16), i.e. K = 32 (BF16: one
+ * v_mfma_f32_16x16x32_bf16), 64 (INT8: one v_mfma_i32_16x16x64_i8) or 16
+ * (FP32: four v_mfma_f32_16x16x4_f32, MFMA j taking entry j of every lane).
+ * hA (16 x K) and hB (K x 16) are row-major host floats holding small
+ * integers; raw (host, 256) receives value v of lane L at 4 L + v. */
+HM_HIP_API int hm_hip_selftest_mfma_lowp(int format, const float *hA, const float *hB,
+                                         float *raw);
 
 /* Host-only check (no device needed) of the wide-panel QR launch program:
  * builds it for njobs panels of m[i] x L[i] (with the triangular factor

@@ -52,6 +52,29 @@ def test_mfma_bf16_lane_layout(gpu):
     np.testing.assert_array_equal(got, A @ B)
 
 
+@pytest.mark.parametrize("fmt,ks,lim", [(1, 32, 8), (2, 64, 127), (3, 16, 512)])
+def test_mfma_lowp_lane_layout(gpu, fmt, ks, lim):
+    """One 16 x 16 block of the sliced GEMM per format, from 16-byte lane
+    fragments (lane L: A[L%16][e0 + e], B[e0 + e][L%16], e0 = ks/4 (L/16)):
+    BF16 one 16x16x32 MFMA, INT8 one v_mfma_i32_16x16x64_i8 (|q| <= 127),
+    FP32 four 16x16x4 MFMAs; value v of lane L is D[4(L/16)+v][L%16].
+    Integer inputs keep every product and sum exact."""
+    _say(f"case: mfma lowp selftest format={fmt}")
+    rng = np.random.default_rng(fmt)
+    A = rng.integers(-lim, lim + 1, size=(16, ks)).astype(np.float32)
+    B = rng.integers(-lim, lim + 1, size=(ks, 16)).astype(np.float32)
+    raw = np.zeros(256, dtype=np.float32)
+    FP = C.POINTER(C.c_float)
+    capi.check(capi.hip().hm_hip_selftest_mfma_lowp(fmt, A.ctypes.data_as(FP),
+                                                    B.ctypes.data_as(FP),
+                                                    raw.ctypes.data_as(FP)), "selftest lowp")
+    L = np.arange(64)
+    got = np.zeros((16, 16), dtype=np.float64)
+    for v in range(4):
+        got[4 * (L // 16) + v, L % 16] = raw[4 * L + v]
+    np.testing.assert_array_equal(got, A.astype(np.float64) @ B.astype(np.float64))
+
+
 CASES = [
     ("uniform h3 s2", [32] * 8, lambda l, j: 2),
     ("uniform h4 mixed", [64] * 16, lambda l, j: 1 + (l + j) % 5),
