@@ -43,9 +43,9 @@
 //
 // The product Y = H X (no transpose, X and Y column-major) runs:
 //   slice X   (per leaf chunk and column)
-//   W_b = V_b^T X_src   sliced GEMM, split-K over segments of chunks, then a
-//                       deterministic reduction of the segment partials
-//   slice W_b (per rank chunk and column)
+//   W_b = V_b^T X_src   sliced GEMM, split-K over segments of chunks
+//   slice W_b (per rank chunk and column; the segment partials summed in
+//             order first, so the result is deterministic)
 //   Y = D X   FP64 MFMA (the diagonal part of the FP64 apply kernel)
 //   Y += sum over levels U_b W_b   sliced GEMM with the levels concatenated
 //                                  along K, one pass over Y
@@ -113,17 +113,19 @@ template <> struct MxDev<MX_F32> {
 };
 
 // ------------------------------------------------------------ slicing
-// One wavefront per item: chunk [k0, k0 + klen) of row p of the source
-// (element (p, k) at src[p * sp + k * sk]) -> S slices at byte
+// One wavefront per item: chunk [k0, k0 + klen) of row p of the source,
+// element (p, k) = sum over t < nsum (in order) of
+// src[t * sum_stride + p * sp + k * sk] (nsum > 1: the split-K partials of a
+// W block, summed here instead of in a separate pass) -> S slices at byte
 // dst + (a * dst_slice + p * dst_ld + dk0) * entry size (klen padded to a
-// whole k-step with zeros), scale 2^e to *scale.
+// whole k-step with zeros), scale to *scale.  Up to 256 entries per item.
 struct SliceItem {
   const double *src;
-  long long sp, sk;
+  long long sp, sk, sum_stride;
   unsigned char *dst;
   long long dst_slice, dst_ld, dk0;
   double *scale;
-  int p, k0, klen, pad;
+  int p, k0, klen, nsum;
 };
 
 __global__ __launch_bounds__(256) void k_slice(const SliceItem *items, int nitems, int S,
@@ -133,23 +135,40 @@ __global__ __launch_bounds__(256) void k_slice(const SliceItem *items, int nitem
   const SliceItem it = items[w];
   const int ks = fmt == MX_I8 ? 64 : (fmt == MX_F32 ? 16 : 32);
   const int kp = (it.klen + ks - 1) / ks * ks;
+  // the item's entries, four per lane (k = lane + 64 i)
+  double v[4];
   double mx = 0.0;
-  for (int k = lane; k < it.klen; k += 64)
-    mx = fmax(mx, fabs(it.src[(long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk]));
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int k = lane + 64 * i;
+    double x = 0.0;
+    if (k < it.klen) {
+      const double *p = it.src + (long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk;
+      x = p[0];
+      for (int t = 1; t < it.nsum; t++) x += p[(long long)t * it.sum_stride];
+    }
+    v[i] = x;
+    mx = fmax(mx, fabs(x));
+  }
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   int e = 0;
   if (mx > 0.0) {
     (void)frexp(mx, &e);  // mx < 2^e
   }
-  const double sc = ldexp(1.0, e);
+  // BF16 and FP32: the power of two 2^e; INT8: the maximum itself, so q_0
+  // spans the whole [-127, 127] (steps of max / 127 instead of up to twice
+  // that: the INT8 products' error is about half the BF16 one)
+  const bool exact = fmt == MX_I8 && mx > 0.0;
+  const double sc = exact ? mx : ldexp(1.0, e);
   if (lane == 0) *it.scale = sc;
-  const double inv = ldexp(1.0, -e);
+  const double inv = exact ? 1.0 / mx : ldexp(1.0, -e);
   const long long base = (long long)it.p * it.dst_ld + it.dk0;
-  for (int k = lane; k < kp; k += 64) {
-    double x = k < it.klen
-                   ? it.src[(long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk] * inv
-                   : 0.0;
-    // x in (-1, 1)
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int k = lane + 64 * i;
+    if (k >= kp) break;
+    double x = v[i] * inv;  // zero beyond klen
+    // x in (-1, 1) ([-1, 1] for INT8)
     if (fmt == MX_BF16) {
       // q_0 = rint(x 2^7), then the remainder scaled by 2^8
       __bf16 *d = reinterpret_cast<__bf16 *>(it.dst) + base;
@@ -200,8 +219,9 @@ struct MxItem {
   int M, N, i0, j0, s0, s1, mode, pad;
 };
 
-// MX_TM x MX_TN tile per workgroup (2 x 2 waves of 32 x 32), k-steps of 64
-// bytes per row: each step's S slices of the A rows and B columns go global
+// MX_TM x TN tile per workgroup (2 x TN/32 waves of 32 x 32; TN = MX_TN, 4
+// waves, measured faster than 128 wide with 8 waves), k-steps of 64 bytes per
+// row: each step's S slices of the A rows and B columns go global
 // -> registers (fetched one step ahead) -> one of two LDS stages, and every
 // wave reads its MFMA fragments from LDS; the stage written during a step is
 // read in the next, so one barrier per step orders both.  The row and
@@ -210,22 +230,24 @@ struct MxItem {
 static constexpr int MX_RSB = 80;
 static constexpr int MX_TM = 64, MX_TN = 64;
 
-template <int F, int S>
-__global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const MxSeg *segs) {
+template <int F, int S, int TN>
+__global__ __launch_bounds__(2 * TN * 2) void k_sliced_gemm(const MxItem *items,
+                                                             const MxSeg *segs) {
   using D = MxDev<F>;
   using acc_t = typename D::acc_t;
   constexpr int KS = D::KS, ESZ = 64 / KS;
   constexpr int FM = 2, FN = 2;  // 16 x 16 blocks per wave
   __shared__ __attribute__((aligned(16))) unsigned char As[2][S][MX_TM * MX_RSB];
-  __shared__ __attribute__((aligned(16))) unsigned char Bs[2][S][MX_TN * MX_RSB];
+  __shared__ __attribute__((aligned(16))) unsigned char Bs[2][S][TN * MX_RSB];
   const MxItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
   const int L = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int li = L & 15, lk = L >> 4;
-  // global loads: A row lr, B row lr; 16 bytes at byte kq
+  // global loads: B row lr (every thread), A row lr (the first 256 threads);
+  // 16 bytes at byte kq
   const int lr = tid >> 2, kq = (tid & 3) * 16;
-  const bool ldA0 = it.i0 + lr < it.M, ldB = it.j0 + lr < it.N;
+  const bool ldA = TN == MX_TM || lr < MX_TM;  // uniform per wave
   double acc64[FM][FN][4];
 #pragma unroll
   for (int f = 0; f < FM; f++)
@@ -236,20 +258,38 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
   for (int sg = it.s0; sg < it.s1; sg++) {
     const MxSeg sd = segs[sg];
     if (sd.nch <= 0) continue;
-    const unsigned char *ga = sd.A + (long long)(it.i0 + lr) * sd.lda * ESZ + kq;
-    const unsigned char *gb = sd.B + (long long)(it.j0 + lr) * sd.ldb * ESZ + kq;
+    // Operands through raw buffer loads: one descriptor per slice whose
+    // range ends after row M - 1 (A) / column N - 1 (B), so rows beyond the
+    // tile's edge read zeros without a branch, and a 32-bit lane offset
+    // (the host keeps M * lda and N * ldb bytes below 2^31).
+    const unsigned nA = (unsigned)((long long)it.M * sd.lda * ESZ);
+    const unsigned nB = (unsigned)((long long)it.N * sd.ldb * ESZ);
+    __amdgpu_buffer_rsrc_t rA[S], rB[S];
+#pragma unroll
+    for (int a = 0; a < S; a++) {
+      rA[a] = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(sd.A + (long long)a * sd.a_slice * ESZ), 0, (int)nA, 0x00020000);
+      rB[a] = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(sd.B + (long long)a * sd.b_slice * ESZ), 0, (int)nB, 0x00020000);
+    }
+    const unsigned offA = (unsigned)((long long)(it.i0 + lr) * sd.lda * ESZ) + kq;
+    const unsigned offB = (unsigned)((long long)(it.j0 + lr) * sd.ldb * ESZ) + kq;
     i4 ra0[S], rb[S];
     auto fetch = [&](int k0) {
+      const unsigned kb = (unsigned)k0 * ESZ;
 #pragma unroll
       for (int a = 0; a < S; a++) {
-        ra0[a] = ldA0 ? ldg16(ga + (a * sd.a_slice + k0) * ESZ) : i4{};
-        rb[a] = ldB ? ldg16(gb + (a * sd.b_slice + k0) * ESZ) : i4{};
+        if (ldA)
+          ra0[a] = __builtin_bit_cast(
+              i4, __builtin_amdgcn_raw_buffer_load_b128(rA[a], (int)(offA + kb), 0, 0));
+        rb[a] = __builtin_bit_cast(
+            i4, __builtin_amdgcn_raw_buffer_load_b128(rB[a], (int)(offB + kb), 0, 0));
       }
     };
     auto stage = [&](int buf) {
 #pragma unroll
       for (int a = 0; a < S; a++) {
-        *reinterpret_cast<i4 *>(&As[buf][a][lr * MX_RSB + kq]) = ra0[a];
+        if (ldA) *reinterpret_cast<i4 *>(&As[buf][a][lr * MX_RSB + kq]) = ra0[a];
         *reinterpret_cast<i4 *>(&Bs[buf][a][lr * MX_RSB + kq]) = rb[a];
       }
     };
@@ -351,24 +391,6 @@ __global__ __launch_bounds__(256) void k_sliced_gemm(const MxItem *items, const 
           stg(cp, it.mode ? ldg(cp) + acc64[f][g][v] : acc64[f][g][v]);
         }
       }
-}
-
-// sum of nseg partials (each M x N, ld M) into C (ld ldc)
-struct MxReduce {
-  const double *part;
-  double *C;
-  long long ldc;
-  int M, N, nseg, pad;
-};
-__global__ __launch_bounds__(256) void k_mx_reduce(const MxReduce *items) {
-  const MxReduce r = items[blockIdx.x];
-  const long long E = (long long)r.M * r.N;
-  for (long long e = (long long)blockIdx.y * 256 + threadIdx.x; e < E;
-       e += (long long)gridDim.y * 256) {
-    double s = 0.0;
-    for (int k = 0; k < r.nseg; k++) s += r.part[(long long)k * E + e];
-    r.C[(e % r.M) + (e / r.M) * r.ldc] = s;
-  }
 }
 
 }  // namespace hm
@@ -506,7 +528,8 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
         it.p = r;
         it.k0 = m->row[q] - node_lo;
         it.klen = m->len[q];
-        it.pad = 0;
+        it.nsum = 1;
+        it.sum_stride = 0;
         items.push_back(it);
       }
     // apply side: row i of F_out, K = rank
@@ -525,7 +548,8 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
         it.p = i;
         it.k0 = c * m->kch;
         it.klen = std::max(0, std::min(m->kch, b.s - c * m->kch));
-        it.pad = 0;
+        it.nsum = 1;
+        it.sum_stride = 0;
         items.push_back(it);
       }
   }
@@ -601,7 +625,6 @@ struct MixedCollect {
   std::vector<SliceItem> sl, wsl;
   std::vector<MxSeg> segs;
   std::vector<MxItem> wit, ait;
-  std::vector<MxReduce> red;
   std::vector<int> ctab;           // chunk tables of the W segments
   std::vector<long long> seg_ctab;  // per segment: its table in ctab, or -1
 };
@@ -624,6 +647,22 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
   const int S = m->S;
   const int nq = (int)m->row.size();
   const int esz = m->esz, kch = m->kch;
+  // the sliced GEMM addresses an operand's rows (or columns) with 32-bit byte
+  // offsets below 2^31
+  auto fits = [&](long long rows, long long ld) { return rows * ld * esz < (1LL << 31) - (1LL << 24); };
+  if (!fits(R, m->npad)) {
+    set_error("mixed product: %d right-hand sides x %lld padded rows exceed the sliced "
+              "kernels' 2 GiB operand range", R, m->npad);
+    return INPUT_ERROR;
+  }
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    if (b.s > 0 && (!fits(b.s, md.blk[g].w_kp) || !fits(md.blk[g].rows, m->s_pad[g]) ||
+                    !fits(R, m->s_pad[g]))) {
+      set_error("mixed product: block %d exceeds the sliced kernels' 2 GiB operand range", g);
+      return INPUT_ERROR;
+    }
+  }
   unsigned char *xs = ar.take<unsigned char>((size_t)S * R * m->npad * esz + 64);
   double *sx = ar.take<double>((size_t)nq * R + 1);
   p.touch(xs, (size_t)S * R * m->npad * esz, "mixed X slices");
@@ -641,7 +680,8 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
       it.p = j;
       it.k0 = m->row[q];
       it.klen = m->len[q];
-      it.pad = 0;
+      it.nsum = 1;
+      it.sum_stride = 0;
       mc.sl.push_back(it);
     }
   // W = F_in^T X per block and segment
@@ -652,6 +692,10 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
     wtot += (long long)t->blk[g].s * R;
   }
   double *W = ar.take<double>((size_t)wtot + 1);
+  // per block: its W (the GEMM's output for one segment) or the segments'
+  // partials, summed by the slicing of W
+  std::vector<const double *> wsrc(t->nblock, nullptr);
+  std::vector<int> wnsum(t->nblock, 1);
   for (int g = 0; g < t->nblock; g++) {
     const BlockMeta &b = t->blk[g];
     const MixedFactors::Blk &x = md.blk[g];
@@ -661,6 +705,8 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
     const int seg_chunks = std::max(1, MX_SEG_K / kch);
     const int nseg = std::max(1, (nch + seg_chunks - 1) / seg_chunks);
     double *part = nseg > 1 ? ar.take<double>((size_t)nseg * b.s * R) : nullptr;
+    wsrc[g] = nseg > 1 ? part : Wg;
+    wnsum[g] = nseg;
     if (nch == 0) {  // empty source: W_b = 0
       mc.segs.push_back(
           {nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, 0});
@@ -703,7 +749,6 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
           mc.wit.push_back({C, b.s, 1, b.s, R, i0, j0, sgi, sgi + 1, 0, 0});
       p.touch(C, (size_t)b.s * R * sizeof(double), "mixed W");
     }
-    if (nseg > 1) mc.red.push_back({part, Wg, b.s, b.s, R, nseg, 0});
   }
   // slices of W, then the apply segments (one per level with a rank)
   std::vector<unsigned char *> Ws(t->nblock, nullptr);
@@ -717,7 +762,7 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
     for (int c = 0; c < m->nrk[g]; c++)
       for (int j = 0; j < R; j++) {
         SliceItem it;
-        it.src = W + woff[g];
+        it.src = wsrc[g];
         it.sp = b.s;
         it.sk = 1;
         it.dst = Ws[g];
@@ -728,7 +773,8 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
         it.p = j;
         it.k0 = c * kch;
         it.klen = std::max(0, std::min(kch, b.s - c * kch));
-        it.pad = 0;
+        it.nsum = wnsum[g];
+        it.sum_stride = (long long)b.s * R;
         mc.wsl.push_back(it);
       }
   }
@@ -776,21 +822,22 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
 }
 
 // The sliced GEMM over nitems work items.
-template <int F>
+template <int F, int TN>
 static void launch_sliced_f(int S, unsigned nitems, const MxItem *I, const MxSeg *G,
                             hipStream_t st) {
-  if (S == 1) hipLaunchKernelGGL((k_sliced_gemm<F, 1>), dim3(nitems), dim3(256), 0, st, I, G);
+  const dim3 b(4 * TN);
+  if (S == 1) hipLaunchKernelGGL((k_sliced_gemm<F, 1, TN>), dim3(nitems), b, 0, st, I, G);
   if constexpr (F != MX_F32) {  // FP32 has one slice
-    if (S == 2) hipLaunchKernelGGL((k_sliced_gemm<F, 2>), dim3(nitems), dim3(256), 0, st, I, G);
-    if (S == 3) hipLaunchKernelGGL((k_sliced_gemm<F, 3>), dim3(nitems), dim3(256), 0, st, I, G);
-    if (S == 4) hipLaunchKernelGGL((k_sliced_gemm<F, 4>), dim3(nitems), dim3(256), 0, st, I, G);
+    if (S == 2) hipLaunchKernelGGL((k_sliced_gemm<F, 2, TN>), dim3(nitems), b, 0, st, I, G);
+    if (S == 3) hipLaunchKernelGGL((k_sliced_gemm<F, 3, TN>), dim3(nitems), b, 0, st, I, G);
+    if (S == 4) hipLaunchKernelGGL((k_sliced_gemm<F, 4, TN>), dim3(nitems), b, 0, st, I, G);
   }
 }
 static void launch_sliced(int fmt, int S, unsigned nitems, const MxItem *I, const MxSeg *G,
                           hipStream_t st) {
-  if (fmt == MX_BF16) launch_sliced_f<MX_BF16>(S, nitems, I, G, st);
-  if (fmt == MX_I8) launch_sliced_f<MX_I8>(S, nitems, I, G, st);
-  if (fmt == MX_F32) launch_sliced_f<MX_F32>(S, nitems, I, G, st);
+  if (fmt == MX_BF16) launch_sliced_f<MX_BF16, MX_TN>(S, nitems, I, G, st);
+  if (fmt == MX_I8) launch_sliced_f<MX_I8, MX_TN>(S, nitems, I, G, st);
+  if (fmt == MX_F32) launch_sliced_f<MX_F32, MX_TN>(S, nitems, I, G, st);
 }
 
 // Work order of the sliced GEMM launches: workgroup b runs on XCD b % 8, and
@@ -835,20 +882,17 @@ static std::vector<MxItem> xcd_order(const std::vector<MxItem> &items,
 }
 
 // The launches of the gathered stages: segment fix-up, slice X, W GEMMs,
-// their reduction, slice W, the apply GEMMs (Y += F_out W).
+// slice W (summing the split-K partials), the apply GEMMs (Y += F_out W).
 static void mixed_emit(Program &p, const MixedCollect &mc, int F_, int S_) {
   const size_t ctab_off = p.push(mc.ctab);
   const size_t sl_off = p.push(mc.sl), wsl_off = p.push(mc.wsl);
-  const size_t red_off = p.push(mc.red);
   const std::vector<MxItem> wit = xcd_order(mc.wit, mc.segs);
   const std::vector<MxItem> ait = xcd_order(mc.ait, mc.segs);
   const size_t wit_off = p.push(wit), ait_off = p.push(ait);
   const size_t seg_off = p.push(mc.segs);
-  const int nsl = (int)mc.sl.size(), nwsl = (int)mc.wsl.size(), nred = (int)mc.red.size();
+  const int nsl = (int)mc.sl.size(), nwsl = (int)mc.wsl.size();
   const unsigned nwit = (unsigned)wit.size(), nait = (unsigned)ait.size();
   const int nsegs = (int)mc.segs.size();
-  long long maxE = 1;
-  for (const auto &r : mc.red) maxE = std::max(maxE, (long long)r.M * r.N);
   std::vector<long long> fix;
   for (int i = 0; i < nsegs; i++)
     fix.push_back(mc.seg_ctab[i] < 0 ? -1
@@ -869,12 +913,6 @@ static void mixed_emit(Program &p, const MixedCollect &mc, int F_, int S_) {
     if (!nwit) return;
     launch_sliced(F_, S_, nwit, reinterpret_cast<const MxItem *>(blob + wit_off),
                   reinterpret_cast<const MxSeg *>(blob + seg_off), st);
-  });
-  p.add("mixed W reduce", [=](hipStream_t st, const unsigned char *blob) {
-    if (nred)
-      hipLaunchKernelGGL(k_mx_reduce,
-                         dim3(nred, (unsigned)std::min<long long>(64, (maxE + 255) / 256)),
-                         dim3(256), 0, st, reinterpret_cast<const MxReduce *>(blob + red_off));
   });
   p.add("slice W", [=](hipStream_t st, const unsigned char *blob) {
     if (nwsl)
