@@ -74,6 +74,11 @@ struct BlockMeta {
 // chunk, so the partials of one off-diagonal block are contiguous.
 static constexpr int CHUNK_ROWS = 64;
 static constexpr int REDUCE_SEG = 16;
+// H x: the apply kernel sums a leaf's chunk partials itself (no reduction
+// launches) when they are at most this many values, of at most this many
+// ranks over the leaf's levels
+static constexpr int MV_FUSE_VALUES = 4096;
+static constexpr int MV_FUSE_RANKS = 256;
 
 // Block path (trees whose per-chunk rank sum is large): W_b = F_in^T X_src
 // is computed per block as a split-K GEMM over K segments of WSEG_ROWS
@@ -122,6 +127,13 @@ struct ProductPlan {
   std::vector<RedItem> red_seg[2], red_w[2];
   RedItem *d_red_seg[2] = {nullptr, nullptr};
   RedItem *d_red_w[2] = {nullptr, nullptr};
+  // H x (R = 1) with the reduction in the apply kernel: per block, its chunk
+  // partials as one item (src, n chunks, s); usable when every leaf's sum of
+  // n * s over its levels is at most MV_FUSE_VALUES and of s at most
+  // MV_FUSE_RANKS (mv_fused)
+  std::vector<RedItem> fsum;
+  RedItem *d_fsum = nullptr;
+  bool mv_fused = false;
   int4 *d_ch = nullptr;                      // (leaf, r0, rows, first global row)
   // stacked input-factor rows of each chunk, 16 per M tile, per transpose
   // mode: [(k * part_mtiles[tr] + mt) * 16 + r]

@@ -34,6 +34,7 @@ struct ProdArgs {
   const PartRow *prow;
   const long long *pofs;
   const long long *wofs;
+  const RedItem *fsum;  // per block: its chunk partials as one item (H x, fused)
   double *P, *W, *Wp;
   const double *X;
   long long x_rs, x_cs;
@@ -98,35 +99,79 @@ __global__ __launch_bounds__(256) void k_mv_partials(ProdArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_mv_apply(ProdArgs a, int rtiles) {
+// FUSED: W_b of the leaf's levels summed here from the chunk partials (one
+// fixed-order sum per rank: lane-strided, then the wavefront butterfly) into
+// LDS, instead of by the two reduction launches, by four more waves while
+// waves 0-3 stream the diagonal block; for trees whose leaves have at most
+// MV_FUSE_VALUES partial values (plan.mv_fused).
+template <bool FUSED>
+__global__ __launch_bounds__(FUSED ? 512 : 256) void k_mv_apply(ProdArgs a, int rtiles) {
   __shared__ double red[4][64];
   __shared__ LevelInfo li[MAX_LEVELS];
+  __shared__ double wl[FUSED ? MV_FUSE_RANKS : 1];
+  __shared__ int wofs_l[MAX_LEVELS + 1];
   const int c = blockIdx.x / rtiles, rt = blockIdx.x % rtiles;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lo = a.leaf_off[c], mc = a.leaf_off[c + 1] - lo;
   const int r0 = rt * 64;
   if (r0 >= mc) return;
   load_levels(a, c, 0, 1, li);
+  if (FUSED) {
+    if (threadIdx.x == 0) {
+      int o = 0;
+      for (int l = 1; l <= a.h; l++) {
+        wofs_l[l - 1] = o;
+        o += a.fsum[blk_index(l, c >> (a.h - l))].s;
+      }
+      wofs_l[a.h] = o;
+    }
+    __syncthreads();
+    for (int l = wave - 4; l >= 0 && l < a.h; l += 4) {
+      const RedItem r = a.fsum[blk_index(l + 1, c >> (a.h - l - 1))];
+      const double *P = a.P + r.src;
+      for (int kk = 0; kk < r.s; kk++) {
+        double t0 = 0.0, t1 = 0.0;
+        int i = lane;
+        for (; i + 64 < r.n; i += 128) {
+          t0 += ldg(P + (long long)i * r.s + kk);
+          t1 += ldg(P + (long long)(i + 64) * r.s + kk);
+        }
+        if (i < r.n) t0 += ldg(P + (long long)i * r.s + kk);
+        const double t = wave_sum(t0 + t1);
+        if (lane == 0) wl[wofs_l[l] + kk] = t;
+      }
+    }
+  }
   const int i = r0 + lane;
   const bool valid = i < mc;
   const int ii = valid ? i : 0;
   const double *D = a.diag + a.diag_off[c];
   const int cq = (mc + 3) / 4;
-  const int j0 = wave * cq, j1 = min(mc, j0 + cq);
-  double acc0 = 0.0, acc1 = 0.0;
-  int j = j0;
-  for (; j + 1 < j1; j += 2) {
-    acc0 += D[ii + (long long)j * mc] * a.X[(long long)(lo + j) * a.x_rs];
-    acc1 += D[ii + (long long)(j + 1) * mc] * a.X[(long long)(lo + j + 1) * a.x_rs];
+  double acc = 0.0;
+  if (wave < 4) {  // the diagonal block, columns split over waves 0-3
+    const int j0 = wave * cq, j1 = min(mc, j0 + cq);
+    double acc0 = 0.0, acc1 = 0.0;
+    int j = j0;
+    for (; j + 1 < j1; j += 2) {
+      acc0 += D[ii + (long long)j * mc] * a.X[(long long)(lo + j) * a.x_rs];
+      acc1 += D[ii + (long long)(j + 1) * mc] * a.X[(long long)(lo + j + 1) * a.x_rs];
+    }
+    if (j < j1) acc0 += D[ii + (long long)j * mc] * a.X[(long long)(lo + j) * a.x_rs];
+    acc = acc0 + acc1;
   }
-  if (j < j1) acc0 += D[ii + (long long)j * mc] * a.X[(long long)(lo + j) * a.x_rs];
-  double acc = acc0 + acc1;
   __syncthreads();
-  for (int l = wave; l < a.h; l += 4) {
-    const LevelInfo x = li[l];
-    for (int k = 0; k < x.s; k++) acc += ldg(x.F + ii + (long long)k * x.ld) * ldg(x.P + k);
+  if (wave < 4) {
+    for (int l = wave; l < a.h; l += 4) {
+      const LevelInfo x = li[l];
+      if (FUSED) {
+        const double *w = wl + wofs_l[l];
+        for (int k = 0; k < x.s; k++) acc += ldg(x.F + ii + (long long)k * x.ld) * w[k];
+      } else {
+        for (int k = 0; k < x.s; k++) acc += ldg(x.F + ii + (long long)k * x.ld) * ldg(x.P + k);
+      }
+    }
+    red[wave][lane] = acc;
   }
-  red[wave][lane] = acc;
   __syncthreads();
   if (wave == 0 && valid) {
     double y = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
@@ -428,6 +473,7 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   a.prow = t->plan.d_prow[trans ? 1 : 0];
   a.pofs = t->plan.d_pofs[trans];
   a.wofs = t->plan.d_wofs;
+  a.fsum = t->plan.d_fsum;
   long long pmax = std::max(std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back()),
                             std::max(t->plan.wpofs[0].back(), t->plan.wpofs[1].back()));
   a.P = scratch_base ? scratch_base : t->d_scratch;
@@ -1051,6 +1097,13 @@ static int run_block_w(hm_dtree *t, const ProdArgs &a, bool nfast, hipStream_t s
   return SUCCESS;
 }
 
+// HM_MV_FUSE=0 forces H x through the reduction launches (for tests that
+// compare both paths).
+static bool mv_fuse_disabled() {
+  const char *e = getenv("HM_MV_FUSE");
+  return e && e[0] == '0';
+}
+
 static int run_reduce(hm_dtree *t, const ProdArgs &a, hipStream_t s) {
   const ProductPlan &p = t->plan;
   const int tr_ = a.trans ? 1 : 0;
@@ -1120,10 +1173,14 @@ static int products_run_ex(hm_dtree *t, int trans, int R, const double *X, long 
       timing_end(tp, s);
       HM_LAUNCH_CHECK();
     }
-    HM_OK(run_reduce(t, a, s));
+    const bool fused = t->plan.mv_fused && !mv_fuse_disabled();
+    if (!fused) HM_OK(run_reduce(t, a, s));
     const int rtiles = (t->max_leaf + 63) / 64;
     Timed ta = timing_begin("mv_apply", s);
-    hipLaunchKernelGGL(k_mv_apply, dim3(t->nleaf * rtiles), dim3(256), 0, s, a, rtiles);
+    if (fused)
+      hipLaunchKernelGGL(k_mv_apply<true>, dim3(t->nleaf * rtiles), dim3(512), 0, s, a, rtiles);
+    else
+      hipLaunchKernelGGL(k_mv_apply<false>, dim3(t->nleaf * rtiles), dim3(256), 0, s, a, rtiles);
     timing_end(ta, s);
     HM_LAUNCH_CHECK();
     return SUCCESS;

@@ -25,6 +25,7 @@ void tree_free(hm_dtree *t) {
     dfree(t->plan.d_prow[k]);
     dfree(t->plan.d_red_seg[k]);
     dfree(t->plan.d_red_w[k]);
+    if (k == 0) dfree(t->plan.d_fsum);
   }
   dfree(t->plan.d_wofs);
   for (int k = 0; k < 2; k++) {
@@ -172,6 +173,27 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
       p.red_w[tr].push_back(r);
     }
   }
+  // H x with the reduction in the apply kernel (small per-leaf partial sets)
+  p.fsum.assign(t->nblock, RedItem{0, 0, 0, 0, 0, 0});
+  for (int g = 0; g < t->nblock; g++) {
+    const BlockMeta &b = t->blk[g];
+    const int span = 1 << (h - b.level);
+    const int cb = p.leaf_chunk0[(b.j ^ 1) * span], ce = p.leaf_chunk0[((b.j ^ 1) + 1) * span];
+    RedItem &r = p.fsum[g];
+    r.src = ce > cb ? p.pofs[0][(size_t)(b.level - 1) * p.nchunk + cb] : 0;
+    r.n = b.s > 0 ? ce - cb : 0;
+    r.s = b.s;
+  }
+  p.mv_fused = true;
+  for (int c = 0; c < nl && p.mv_fused; c++) {
+    long long vals = 0, ranks = 0;
+    for (int l = 1; l <= h; l++) {
+      const RedItem &r = p.fsum[blk_index(l, c >> (h - l))];
+      vals += (long long)r.n * r.s;
+      ranks += r.s;
+    }
+    p.mv_fused = vals <= MV_FUSE_VALUES && ranks <= MV_FUSE_RANKS;
+  }
   // block path plan
   for (int tr = 0; tr < 2; tr++) {
     p.witems[tr].clear();
@@ -243,6 +265,7 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     HM_OK(upload_vec(&p.d_prow[tr], p.prow[tr], s));
     HM_OK(upload_vec(&p.d_red_seg[tr], p.red_seg[tr], s));
     HM_OK(upload_vec(&p.d_red_w[tr], p.red_w[tr], s));
+    if (tr == 0) HM_OK(upload_vec(&p.d_fsum, p.fsum, s));
   }
   for (int tr = 0; tr < 2; tr++) HM_OK(upload_vec(&p.d_pofs[tr], p.pofs[tr], s));
   HM_OK(upload_vec(&p.d_wofs, p.wofs, s));

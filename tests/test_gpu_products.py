@@ -131,6 +131,37 @@ def test_products_vs_dense(gpu, monkeypatch, name, leaves, ranks, R, wpath):
         T.detach_and_free(lib, t)
 
 
+@pytest.mark.parametrize("name,leaves,ranks", CASES, ids=[c[0] for c in CASES])
+def test_matvec_fused_and_reduced(gpu, monkeypatch, name, leaves, ranks):
+    """H x with the chunk partials summed inside the apply kernel (trees
+    whose leaves have few partial values) and through the two reduction
+    launches (HM_MV_FUSE=0; the path larger trees take): both vs the dense
+    product, and against each other."""
+    from hmat_lib_amd import device as D
+    _say(f"case: matvec fused/reduced {name}")
+    rng = np.random.default_rng(len(leaves) * 17 + 3)
+    ft = H.random_flat_tree(rng, leaves, ranks)
+    lib = capi.hmat()
+    keep = T.Keep()
+    t = T.fill_tree(lib, ft, keep)
+    try:
+        dense = ft.to_dense()
+        x = rng.standard_normal(ft.n)
+        dt = D.DeviceTree.upload(t.contents.root, ft.height)
+        dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((ft.n,))
+        ys = {}
+        for fuse in ("1", "0"):
+            monkeypatch.setenv("HM_MV_FUSE", fuse)
+            dt.matvec(dx, dy)
+            dt.matvec(dx, dy)
+            ys[fuse] = dy.numpy()
+            assert H.rel_err(ys[fuse], dense @ x) < TOL, fuse
+        assert H.rel_err(ys["1"], ys["0"]) < 1e-14
+        dt.free()
+    finally:
+        T.detach_and_free(lib, t)
+
+
 @pytest.mark.parametrize("variant", range(7))
 def test_every_apply_tile_variant(gpu, monkeypatch, variant):
     """Each tile shape of the apply kernel (forced instead of tuned), on a
