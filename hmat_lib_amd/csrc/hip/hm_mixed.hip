@@ -113,12 +113,14 @@ template <> struct MxDev<MX_F32> {
 };
 
 // ------------------------------------------------------------ slicing
-// One wavefront per item: chunk [k0, k0 + klen) of row p of the source,
-// element (p, k) = sum over t < nsum (in order) of
-// src[t * sum_stride + p * sp + k * sk] (nsum > 1: the split-K partials of a
-// W block, summed here instead of in a separate pass) -> S slices at byte
+// One workgroup per item: columns p .. p + ncol - 1 (ncol <= 16, four per
+// wave, their loads issued before their reductions), each the chunk
+// [k0, k0 + klen) (klen <= 256) of one row of the source, element (p, k) =
+// sum over t < nsum (in order) of src[t * sum_stride + p * sp + k * sk]
+// (nsum > 1: the split-K partials of a W block, summed here instead of in a
+// separate pass) -> S slices at byte
 // dst + (a * dst_slice + p * dst_ld + dk0) * entry size (klen padded to a
-// whole k-step with zeros), scale to *scale.  Up to 256 entries per item.
+// whole k-step with zeros), its scale to scale[p - first p].
 struct SliceItem {
   const double *src;
   long long sp, sk, sum_stride;
@@ -126,72 +128,104 @@ struct SliceItem {
   long long dst_slice, dst_ld, dk0;
   double *scale;
   int p, k0, klen, nsum;
+  int ncol, pad;
 };
+static constexpr int SLICE_COLS = 16, SLICE_CPW = 4;
 
-__global__ __launch_bounds__(256) void k_slice(const SliceItem *items, int nitems, int S,
-                                               int fmt) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (w >= nitems) return;
-  const SliceItem it = items[w];
-  const int ks = fmt == MX_I8 ? 64 : (fmt == MX_F32 ? 16 : 32);
+template <int F, int S>
+__global__ __launch_bounds__(256) void k_slice(const SliceItem *items) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const SliceItem it = items[blockIdx.x];
+  constexpr int ks = F == MX_I8 ? 64 : (F == MX_F32 ? 16 : 32);
   const int kp = (it.klen + ks - 1) / ks * ks;
-  // the item's entries, four per lane (k = lane + 64 i)
-  double v[4];
-  double mx = 0.0;
+  // the columns' entries, four per lane (k = lane + 64 i)
+  double v[SLICE_CPW][4], mx[SLICE_CPW];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int k = lane + 64 * i;
-    double x = 0.0;
-    if (k < it.klen) {
-      const double *p = it.src + (long long)it.p * it.sp + (long long)(it.k0 + k) * it.sk;
-      x = p[0];
-      for (int t = 1; t < it.nsum; t++) x += p[(long long)t * it.sum_stride];
-    }
-    v[i] = x;
-    mx = fmax(mx, fabs(x));
-  }
-  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-  int e = 0;
-  if (mx > 0.0) {
-    (void)frexp(mx, &e);  // mx < 2^e
-  }
-  // BF16 and FP32: the power of two 2^e; INT8: the maximum itself, so q_0
-  // spans the whole [-127, 127] (steps of max / 127 instead of up to twice
-  // that: the INT8 products' error is about half the BF16 one)
-  const bool exact = fmt == MX_I8 && mx > 0.0;
-  const double sc = exact ? mx : ldexp(1.0, e);
-  if (lane == 0) *it.scale = sc;
-  const double inv = exact ? 1.0 / mx : ldexp(1.0, -e);
-  const long long base = (long long)it.p * it.dst_ld + it.dk0;
+  for (int c = 0; c < SLICE_CPW; c++) {
+    const int jj = wv * SLICE_CPW + c;
+    mx[c] = 0.0;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int k = lane + 64 * i;
-    if (k >= kp) break;
-    double x = v[i] * inv;  // zero beyond klen
-    // x in (-1, 1) ([-1, 1] for INT8)
-    if (fmt == MX_BF16) {
-      // q_0 = rint(x 2^7), then the remainder scaled by 2^8
-      __bf16 *d = reinterpret_cast<__bf16 *>(it.dst) + base;
-      x *= 128.0;
-      for (int a = 0; a < S; a++) {
-        const double q = rint(x);
-        d[(long long)a * it.dst_slice + k] = (__bf16)(float)q;
-        x = (x - q) * 256.0;
+    for (int i = 0; i < 4; i++) {
+      const int k = lane + 64 * i;
+      double x = 0.0;
+      if (jj < it.ncol && k < it.klen) {
+        const double *q =
+            it.src + (long long)(it.p + jj) * it.sp + (long long)(it.k0 + k) * it.sk;
+        x = q[0];
+        for (int t = 1; t < it.nsum; t++) x += q[(long long)t * it.sum_stride];
       }
-    } else if (fmt == MX_I8) {
-      // q_0 = rint(127 x), then the remainder (in [-1/2, 1/2]) times 254
-      signed char *d = reinterpret_cast<signed char *>(it.dst) + base;
-      x *= 127.0;
-      for (int a = 0; a < S; a++) {
-        const double q = rint(x);
-        d[(long long)a * it.dst_slice + k] = (signed char)(int)q;
-        x = (x - q) * 254.0;
-      }
-    } else {
-      float *d = reinterpret_cast<float *>(it.dst) + base;
-      d[k] = (float)x;
+      v[c][i] = x;
+      mx[c] = fmax(mx[c], fabs(x));
     }
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int c = 0; c < SLICE_CPW; c++) mx[c] = fmax(mx[c], __shfl_xor(mx[c], o, 64));
+#pragma unroll
+  for (int c = 0; c < SLICE_CPW; c++) {
+    const int jj = wv * SLICE_CPW + c;
+    if (jj >= it.ncol) break;
+    int e = 0;
+    if (mx[c] > 0.0) {
+      (void)frexp(mx[c], &e);  // mx < 2^e
+    }
+    // BF16 and FP32: the power of two 2^e; INT8: the maximum itself, so q_0
+    // spans the whole [-127, 127] (steps of max / 127 instead of up to twice
+    // that: the INT8 products' error is about half the BF16 one)
+    const bool exact = F == MX_I8 && mx[c] > 0.0;
+    const double sc = exact ? mx[c] : ldexp(1.0, e);
+    if (lane == 0) it.scale[jj] = sc;
+    const double inv = exact ? 1.0 / mx[c] : ldexp(1.0, -e);
+    const long long base = (long long)(it.p + jj) * it.dst_ld + it.dk0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int k = lane + 64 * i;
+      if (k >= kp) break;
+      double x = v[c][i] * inv;  // zero beyond klen
+      // x in (-1, 1) ([-1, 1] for INT8)
+      if constexpr (F == MX_BF16) {
+        // q_0 = rint(x 2^7), then the remainder scaled by 2^8
+        __bf16 *d = reinterpret_cast<__bf16 *>(it.dst) + base;
+        x *= 128.0;
+#pragma unroll
+        for (int a = 0; a < S; a++) {
+          const double q = rint(x);
+          d[(long long)a * it.dst_slice + k] = (__bf16)(float)q;
+          x = (x - q) * 256.0;
+        }
+      } else if constexpr (F == MX_I8) {
+        // q_0 = rint(127 x), then the remainder (in [-1/2, 1/2]) times 254
+        signed char *d = reinterpret_cast<signed char *>(it.dst) + base;
+        x *= 127.0;
+#pragma unroll
+        for (int a = 0; a < S; a++) {
+          const double q = rint(x);
+          d[(long long)a * it.dst_slice + k] = (signed char)(int)q;
+          x = (x - q) * 254.0;
+        }
+      } else {
+        float *d = reinterpret_cast<float *>(it.dst) + base;
+        d[k] = (float)x;
+      }
+    }
+  }
+}
+
+template <int F>
+static void launch_slice_f(int S, unsigned n, const SliceItem *items, hipStream_t st) {
+  if (S == 1) hipLaunchKernelGGL((k_slice<F, 1>), dim3(n), dim3(256), 0, st, items);
+  if constexpr (F != MX_F32) {
+    if (S == 2) hipLaunchKernelGGL((k_slice<F, 2>), dim3(n), dim3(256), 0, st, items);
+    if (S == 3) hipLaunchKernelGGL((k_slice<F, 3>), dim3(n), dim3(256), 0, st, items);
+    if (S == 4) hipLaunchKernelGGL((k_slice<F, 4>), dim3(n), dim3(256), 0, st, items);
+  }
+}
+static void launch_slice(int fmt, int S, unsigned n, const SliceItem *items, hipStream_t st) {
+  if (n == 0) return;
+  if (fmt == MX_BF16) launch_slice_f<MX_BF16>(S, n, items, st);
+  if (fmt == MX_I8) launch_slice_f<MX_I8>(S, n, items, st);
+  if (fmt == MX_F32) launch_slice_f<MX_F32>(S, n, items, st);
 }
 
 // ---------------------------------------------------------- sliced GEMM
@@ -464,7 +498,7 @@ static int run_slices(const std::vector<SliceItem> &items, int S, int fmt, hipSt
                                 hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
     const int n = (int)items.size();
-    hipLaunchKernelGGL(k_slice, dim3((n + 3) / 4), dim3(256), 0, s, d_items, n, S, fmt);
+    launch_slice(fmt, S, (unsigned)n, d_items, s);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -514,8 +548,8 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
     const double *fin = tr ? t->d_U + b.uoff : t->d_V + b.voff;
     const int fin_ld = tr ? b.m : b.n;
     const int node_lo = tr ? b.r0 : b.c0;
-    for (int r = 0; r < b.s; r++)
-      for (int q = x.q0; q < x.q1; q++) {
+    for (int q = x.q0; q < x.q1; q++)
+      for (int r = 0; r < b.s; r += SLICE_COLS) {
         SliceItem it;
         it.src = fin;
         it.sp = fin_ld;
@@ -530,12 +564,14 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
         it.klen = m->len[q];
         it.nsum = 1;
         it.sum_stride = 0;
+        it.ncol = std::min(SLICE_COLS, b.s - r);
+        it.pad = 0;
         items.push_back(it);
       }
     // apply side: row i of F_out, K = rank
     const double *fout = tr ? t->d_V + b.voff : t->d_U + b.uoff;
-    for (int i = 0; i < x.rows; i++)
-      for (int c = 0; c < m->nrk[g]; c++) {
+    for (int c = 0; c < m->nrk[g]; c++)
+      for (int i = 0; i < x.rows; i += SLICE_COLS) {
         SliceItem it;
         it.src = fout;
         it.sp = 1;
@@ -550,6 +586,8 @@ static int prepare_mode(hm_dtree *t, MixedFactors *m, int tr, hipStream_t s) {
         it.klen = std::max(0, std::min(m->kch, b.s - c * m->kch));
         it.nsum = 1;
         it.sum_stride = 0;
+        it.ncol = std::min(SLICE_COLS, x.rows - i);
+        it.pad = 0;
         items.push_back(it);
       }
   }
@@ -667,7 +705,7 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
   double *sx = ar.take<double>((size_t)nq * R + 1);
   p.touch(xs, (size_t)S * R * m->npad * esz, "mixed X slices");
   for (int q = 0; q < nq; q++)
-    for (int j = 0; j < R; j++) {
+    for (int j = 0; j < R; j += SLICE_COLS) {
       SliceItem it;
       it.src = X;
       it.sp = x_cs;
@@ -682,6 +720,8 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
       it.klen = m->len[q];
       it.nsum = 1;
       it.sum_stride = 0;
+      it.ncol = std::min(SLICE_COLS, R - j);
+      it.pad = 0;
       mc.sl.push_back(it);
     }
   // W = F_in^T X per block and segment
@@ -760,7 +800,7 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
     sW[g] = ar.take<double>((size_t)m->nrk[g] * R + 1);
     p.touch(Ws[g], (size_t)S * R * m->s_pad[g] * esz, "mixed W slices");
     for (int c = 0; c < m->nrk[g]; c++)
-      for (int j = 0; j < R; j++) {
+      for (int j = 0; j < R; j += SLICE_COLS) {
         SliceItem it;
         it.src = wsrc[g];
         it.sp = b.s;
@@ -775,6 +815,8 @@ static int mixed_collect(MixedCollect &mc, Program &p, Arena &ar, hm_dtree *t, i
         it.klen = std::max(0, std::min(kch, b.s - c * kch));
         it.nsum = wnsum[g];
         it.sum_stride = (long long)b.s * R;
+        it.ncol = std::min(SLICE_COLS, R - j);
+        it.pad = 0;
         mc.wsl.push_back(it);
       }
   }
@@ -905,9 +947,7 @@ static void mixed_emit(Program &p, const MixedCollect &mc, int F_, int S_) {
                          reinterpret_cast<const long long *>(blob + fix_off), blob, nsegs);
   });
   p.add("slice X", [=](hipStream_t st, const unsigned char *blob) {
-    if (nsl)
-      hipLaunchKernelGGL(k_slice, dim3((nsl + 3) / 4), dim3(256), 0, st,
-                         reinterpret_cast<const SliceItem *>(blob + sl_off), nsl, S_, F_);
+    launch_slice(F_, S_, (unsigned)nsl, reinterpret_cast<const SliceItem *>(blob + sl_off), st);
   });
   p.add("mixed W", [=](hipStream_t st, const unsigned char *blob) {
     if (!nwit) return;
@@ -915,9 +955,8 @@ static void mixed_emit(Program &p, const MixedCollect &mc, int F_, int S_) {
                   reinterpret_cast<const MxSeg *>(blob + seg_off), st);
   });
   p.add("slice W", [=](hipStream_t st, const unsigned char *blob) {
-    if (nwsl)
-      hipLaunchKernelGGL(k_slice, dim3((nwsl + 3) / 4), dim3(256), 0, st,
-                         reinterpret_cast<const SliceItem *>(blob + wsl_off), nwsl, S_, F_);
+    launch_slice(F_, S_, (unsigned)nwsl, reinterpret_cast<const SliceItem *>(blob + wsl_off),
+                 st);
   });
   p.add("mixed apply", [=](hipStream_t st, const unsigned char *blob) {
     if (!nait) return;

@@ -244,8 +244,20 @@ __global__ __launch_bounds__(256) void k_gemm_chain(const GemmChain *chains,
     const int i0 = t.y * Tile::TM, j0 = t.z * Tile::TN;
     Tile tile;
     tile.zero();
+    const bool full = i0 + Tile::TM <= ch.M && j0 + Tile::TN <= ch.N;
     for (int q = ch.link0; q < ch.link0 + ch.nlink; q++) {
       const GemmLink lk = links[q];
+      // a link whose tile lies inside its operands with K a whole number of
+      // chunks (the diagonal leaves of uniform trees): the branch-free,
+      // double-buffered interior loop
+      if (full && !lk.ta && lk.K > 0 && lk.K % Tile::KC_ == 0) {
+        if (lk.tb)
+          tile.template run_interior<true>(lds, lk.K, lk.A + i0, lk.lda, lk.B + j0, lk.ldb, 1);
+        else
+          tile.template run_interior<false>(lds, lk.K, lk.A + i0, lk.lda,
+                                            lk.B + (long long)j0 * lk.ldb, 1, lk.ldb);
+        continue;
+      }
       auto la_n = [&](int i, int k) -> double {
         return (i0 + i < ch.M && k < lk.K) ? ldg(lk.A + (i0 + i) + (long long)k * lk.lda) : 0.0;
       };
