@@ -205,8 +205,9 @@ def test_inv2d_64k_fp64_and_slices(gpu):
     """inv2d_64k, R=256: the compression's later rounds continue the failed
     rounds' sketches (blocks of ranks to 662); the FP64 product equals the
     reference's product on the same tree (1e-12) and the FP64 dense rows to
-    the threshold's error; the sliced products' error vs the FP64 product
-    of the same tree per slice count, 4 slices within 1e-8."""
+    the threshold's error; the sliced products' error (BF16 and INT8 slices,
+    one FP32 copy) vs the FP64 product of the same tree per slice count, 4
+    slices within 1e-8."""
     from hmat_lib_amd import device as D
     w = WL.WORKLOADS["inv2d_64k"]
     _say(f"case: config {w.name} compress")
@@ -238,13 +239,20 @@ def test_inv2d_64k_fp64_and_slices(gpu):
     assert err <= 1e-12
     chk.free(ht)
     tols = {1: 3e-2, 2: 5e-4, 3: 3e-6, 4: 1e-8}
-    for S in (1, 2, 3, 4):
-        _say(f"case: config {w.name} slices={S}")
-        dt.set_precision(S)
-        dt.matmat(dx, dy)
-        e = H.rel_err(dy.numpy(), y64)
-        _say(f"  {S} slices vs FP64 product: {e:.2e}")
-        assert e <= tols[S], (S, e)
+    for fmt in ("bf16", "int8"):
+        for S in (1, 2, 3, 4):
+            _say(f"case: config {w.name} {fmt} slices={S}")
+            dt.set_precision(S, fmt)
+            dt.matmat(dx, dy)
+            e = H.rel_err(dy.numpy(), y64)
+            _say(f"  {fmt} x{S} vs FP64 product: {e:.2e}")
+            assert e <= tols[S], (fmt, S, e)
+    _say(f"case: config {w.name} fp32")
+    dt.set_precision(1, "fp32")
+    dt.matmat(dx, dy)
+    e = H.rel_err(dy.numpy(), y64)
+    _say(f"  fp32 vs FP64 product: {e:.2e}")
+    assert e <= 1e-6, e
     dt.set_precision(0)
     dt.free()
 
