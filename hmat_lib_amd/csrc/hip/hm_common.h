@@ -174,6 +174,9 @@ struct hm_dtree {
   long long *d_diag_off = nullptr;
   hm::BlockMeta *d_blk = nullptr;
   hm::ProductPlan plan;
+  // one device allocation holding every metadata and plan array above (the
+  // d_* pointers of the tree and of the plan point into it)
+  char *d_meta = nullptr;
   double *d_scratch = nullptr;        // P then W, grown on demand
   size_t scratch_elems = 0;
   // low-precision copies of U and V for mixed-precision products (format

@@ -1,5 +1,6 @@
 // Runtime services of libhmat_hip: device selection, error text, memory
 // helpers and optional HIP-event timing of named kernels.
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -37,10 +38,16 @@ hipStream_t stream_of(void *s) {
   return s ? reinterpret_cast<hipStream_t>(s) : g_stream;
 }
 
+static bool timing_on();
 int dmalloc(void **p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 16;
+  const bool tm = timing_on();
+  const auto t0 = std::chrono::steady_clock::now();
   hipError_t e = hipMalloc(p, bytes);
+  if (tm)
+    timing_host("hipMalloc (host)", std::chrono::duration<double, std::milli>(
+                                        std::chrono::steady_clock::now() - t0).count());
   if (e != hipSuccess) {
     *p = nullptr;
     set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
@@ -50,7 +57,13 @@ int dmalloc(void **p, size_t bytes) {
 }
 
 void dfree(void *p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  const bool tm = timing_on();
+  const auto t0 = std::chrono::steady_clock::now();
+  (void)hipFree(p);
+  if (tm)
+    timing_host("hipFree (host)", std::chrono::duration<double, std::milli>(
+                                      std::chrono::steady_clock::now() - t0).count());
 }
 
 // ---------------------------------------------------------------- timing
@@ -64,6 +77,7 @@ struct TimingAcc {
   long count = 0;
 };
 static bool g_timing = false;
+static bool timing_on() { return g_timing; }
 static std::string g_timing_only;  // empty: time every named kernel
 static std::vector<TimingSlot> g_slots;
 static std::map<std::string, TimingAcc> g_acc;

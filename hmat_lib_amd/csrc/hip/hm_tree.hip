@@ -16,35 +16,43 @@ void tree_free(hm_dtree *t) {
   dfree(t->d_diag);
   dfree(t->d_U);
   dfree(t->d_V);
-  dfree(t->d_leaf_off);
-  dfree(t->d_diag_off);
-  dfree(t->d_blk);
-  for (int k = 0; k < 2; k++) dfree(t->plan.d_pofs[k]);
-  dfree(t->plan.d_ch);
-  for (int k = 0; k < 2; k++) {
-    dfree(t->plan.d_prow[k]);
-    dfree(t->plan.d_red_seg[k]);
-    dfree(t->plan.d_red_w[k]);
-    if (k == 0) dfree(t->plan.d_fsum);
-  }
-  dfree(t->plan.d_wofs);
-  for (int k = 0; k < 2; k++) {
-    dfree(t->plan.d_witems[k]);
-    dfree(t->plan.d_wpofs[k]);
-    dfree(t->plan.d_wmulti[k]);
-  }
+  dfree(t->d_meta);  // every metadata and plan array
   dfree(t->d_scratch);
   delete t;
 }
 
-template <class T>
-static int upload_vec(T **d, const std::vector<T> &h, hipStream_t s) {
-  HM_OK(dalloc(d, h.size()));
-  if (!h.empty())
-    HM_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T),
-                          hipMemcpyHostToDevice, s));
-  return SUCCESS;
-}
+// The tree's metadata and plan arrays packed into one device allocation
+// and one host -> device copy (each array at a 256-byte boundary): a tree
+// has about twenty of them, and one allocation and copy each cost more host
+// time than the compression's last round.
+struct MetaPack {
+  struct Part {
+    void **dst;
+    const void *src;
+    size_t off, bytes;
+  };
+  std::vector<Part> parts;
+  size_t total = 0;
+  template <class T>
+  void add(T **dst, const std::vector<T> &h) {
+    total = (total + 255) & ~(size_t)255;
+    parts.push_back({reinterpret_cast<void **>(dst), h.data(), total, h.size() * sizeof(T)});
+    total += h.size() * sizeof(T);
+  }
+  int upload(char **blob, hipStream_t s) {
+    dfree(*blob);
+    *blob = nullptr;
+    HM_OK(dalloc(blob, total + 256));
+    std::vector<char> host(total + 256, 0);
+    for (const Part &p : parts)
+      if (p.bytes) memcpy(host.data() + p.off, p.src, p.bytes);
+    HM_HIP(hipMemcpyAsync(*blob, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    for (const Part &p : parts) *p.dst = *blob + p.off;
+    // the staging copy must finish before host goes out of scope
+    HM_HIP(hipStreamSynchronize(s));
+    return SUCCESS;
+  }
+};
 
 // Requires leaf_off (nleaf+1) and blk[g].{r0,m,c0,n,s,level,j}.
 int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
@@ -257,25 +265,25 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     }
   }
 
-  HM_OK(upload_vec(&t->d_leaf_off, t->leaf_off, s));
-  HM_OK(upload_vec(&t->d_diag_off, t->diag_off, s));
-  HM_OK(upload_vec(&t->d_blk, t->blk, s));
-  HM_OK(upload_vec(&p.d_ch, ch, s));
+  MetaPack mp;
+  mp.add(&t->d_leaf_off, t->leaf_off);
+  mp.add(&t->d_diag_off, t->diag_off);
+  mp.add(&t->d_blk, t->blk);
+  mp.add(&p.d_ch, ch);
   for (int tr = 0; tr < 2; tr++) {
-    HM_OK(upload_vec(&p.d_prow[tr], p.prow[tr], s));
-    HM_OK(upload_vec(&p.d_red_seg[tr], p.red_seg[tr], s));
-    HM_OK(upload_vec(&p.d_red_w[tr], p.red_w[tr], s));
-    if (tr == 0) HM_OK(upload_vec(&p.d_fsum, p.fsum, s));
+    mp.add(&p.d_prow[tr], p.prow[tr]);
+    mp.add(&p.d_red_seg[tr], p.red_seg[tr]);
+    mp.add(&p.d_red_w[tr], p.red_w[tr]);
   }
-  for (int tr = 0; tr < 2; tr++) HM_OK(upload_vec(&p.d_pofs[tr], p.pofs[tr], s));
-  HM_OK(upload_vec(&p.d_wofs, p.wofs, s));
+  mp.add(&p.d_fsum, p.fsum);
+  for (int tr = 0; tr < 2; tr++) mp.add(&p.d_pofs[tr], p.pofs[tr]);
+  mp.add(&p.d_wofs, p.wofs);
   for (int tr = 0; tr < 2; tr++) {
-    HM_OK(upload_vec(&p.d_witems[tr], p.witems[tr], s));
-    HM_OK(upload_vec(&p.d_wpofs[tr], p.wpofs[tr], s));
-    HM_OK(upload_vec(&p.d_wmulti[tr], p.wmulti[tr], s));
+    mp.add(&p.d_witems[tr], p.witems[tr]);
+    mp.add(&p.d_wpofs[tr], p.wpofs[tr]);
+    mp.add(&p.d_wmulti[tr], p.wmulti[tr]);
   }
-  HM_HIP(hipStreamSynchronize(s));
-  return SUCCESS;
+  return mp.upload(&t->d_meta, s);
 }
 
 int tree_alloc_pools(hm_dtree *t) {

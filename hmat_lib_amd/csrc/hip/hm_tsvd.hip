@@ -300,6 +300,34 @@ __global__ __launch_bounds__(256) void k_jacobi(TsvdBlock *blocks,
   }
 }
 
+// Several host arrays in one device allocation and one copy (each at a
+// 256-byte boundary); the host staging buffer is kept until the copy ends.
+struct HostPack {
+  struct Part {
+    void **dst;
+    const void *src;
+    size_t off, bytes;
+  };
+  std::vector<Part> parts;
+  size_t total = 0;
+  template <class T>
+  void add(T **dst, const std::vector<T> &h) {
+    total = (total + 255) & ~(size_t)255;
+    parts.push_back({reinterpret_cast<void **>(dst), h.data(), total, h.size() * sizeof(T)});
+    total += h.size() * sizeof(T);
+  }
+  int upload(char **blob, hipStream_t s) {
+    HM_OK(dalloc(blob, total + 256));
+    std::vector<char> host(total + 256, 0);
+    for (const Part &p : parts)
+      if (p.bytes) memcpy(host.data() + p.off, p.src, p.bytes);
+    HM_HIP(hipMemcpyAsync(*blob, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    for (const Part &p : parts) *p.dst = *blob + p.off;
+    HM_HIP(hipStreamSynchronize(s));
+    return SUCCESS;
+  }
+};
+
 template <class T>
 static int to_device(T **d, const std::vector<T> &h, hipStream_t s) {
   HM_OK(dalloc(d, h.size()));
@@ -883,8 +911,6 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
       for (int q = 0; q < 11; q++) *dst[q] = reinterpret_cast<double *>(rbuf.base + offs[q]);
       if (!any_carry) Ms = nullptr;  // no region was carved for it
     }
-    TsvdBlock *d_blocks = nullptr;
-    HM_OK(to_device(&d_blocks, blocks, s)); mem.keep(d_blocks);
     // work lists, biggest K first for balance
     std::vector<int> order = active;
     std::sort(order.begin(), order.end(), [&](int x, int y) {
@@ -919,15 +945,24 @@ int tsvd_run(std::vector<TsvdBlock> &blocks, const TsvdOptions &opt,
     ws = xcd_grouped(ws);
     wp = xcd_grouped(wp);
     wp16 = xcd_grouped(wp16);
+    // the block table and the work lists: one allocation, one copy
+    TsvdBlock *d_blocks = nullptr;
     int2 *d_wo;
     int4 *d_ws, *d_wp;
     int *d_list;
-    HM_OK(to_device(&d_wo, wo, s)); mem.keep(d_wo);
-    HM_OK(to_device(&d_ws, ws, s)); mem.keep(d_ws);
-    HM_OK(to_device(&d_wp, wp, s)); mem.keep(d_wp);
     int4 *d_wp16;
-    HM_OK(to_device(&d_wp16, wp16, s)); mem.keep(d_wp16);
-    HM_OK(to_device(&d_list, order, s)); mem.keep(d_list);
+    {
+      HostPack hp;
+      hp.add(&d_blocks, blocks);
+      hp.add(&d_wo, wo);
+      hp.add(&d_ws, ws);
+      hp.add(&d_wp, wp);
+      hp.add(&d_wp16, wp16);
+      hp.add(&d_list, order);
+      char *pack = nullptr;
+      HM_OK(hp.upload(&pack, s));
+      mem.keep(pack);
+    }
     timing_host("round setup (host)", std::chrono::duration<double, std::milli>(
                                           std::chrono::steady_clock::now() - r0).count());
     // the round as one launch program: sketch, QR of the sketch, posterior
