@@ -13,11 +13,17 @@
 //
 // so a product over the whole batch is 2-3 launches of the persistent
 // batched GEMM kernels, whatever the number of trees or levels, and Y is
-// written once.  (A launch per level was measured first: Y is then read and
-// written once per level, 2.25 ms for the 512-tree workload.)  Tile shapes
-// of every launch are chosen by timing (program_tune), and the program can
-// be captured once into a HIP graph and replayed.
+// written once.  Trees whose W goes through the chunk path of a single
+// product (rank sums within its partial tiles) take that path instead,
+// all of them in the same launches (tree_prod_add): partials of every level
+// per row chunk with X read once, the fixed-order reductions, and apply
+// tiles per leaf, one apply launch per RHS width group.  (A launch per
+// level was measured first: Y is then read and written once per level,
+// 2.25 ms for the 512-tree workload.)  Tile shapes of every launch are
+// chosen by timing (program_tune), and the program can be captured once
+// into a HIP graph and replayed.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "hm_prog.h"
@@ -128,33 +134,51 @@ static void diag_chains(const hm_dtree *H, int trans, const double *X, long long
   }
 }
 
+// HM_BATCH_CHAINS=1 sends every FP64 tree through the chained GEMMs (for
+// tests and timings that compare the two routes).
+static bool chain_route_forced() {
+  const char *e = getenv("HM_BATCH_CHAINS");
+  return e && e[0] == '1';
+}
+
 // trans: 0 Y = H X, 1 Y = H^T X, 2 Y = X H (X, Y: R x n), taken as
 // Y^T = H^T X^T.  Trees with sliced factors (hm_hip_tree_set_precision)
-// take their diagonal from the FP64 chains and their off-diagonal blocks
-// from the sliced stages, which run after the chains, one launch per
-// stage for all of them.
+// take their FP64 diagonal from the batched apply tiles (with no levels; the
+// chained GEMMs when HM_BATCH_CHAINS=1) and their off-diagonal blocks from
+// the sliced stages, which run after it, one launch per stage for all of
+// them.
 static int build_batch(Program &p, Arena &ar, const std::vector<hm_dtree *> &trees, int trans,
                        const std::vector<BatchOperands> &ops) {
   std::vector<GemmJob> wj;
   std::vector<GemmChain> chains;
   std::vector<GemmLink> links;
   std::vector<MixedJob> mj;
+  std::vector<TreeProdJob> tj;
   const bool rowmajor = trans == 2;
+  const bool chains_only = chain_route_forced();
   for (size_t i = 0; i < trees.size(); i++) {
     const BatchOperands &o = ops[i];
-    if (trees[i]->slices > 0) {
+    const bool sliced = trees[i]->slices > 0;
+    if (!chains_only && o.R > 0 && tree_prod_batchable(trees[i], trans ? 1 : 0, sliced)) {
+      // X H = (H^T X^T)^T: X^T's element (row r, column j) is X[j + r ldx]
+      if (rowmajor)
+        tj.push_back({trees[i], 1, o.R, o.X, o.ldx, 1, o.Y, o.ldy, 1, sliced});
+      else
+        tj.push_back({trees[i], trans ? 1 : 0, o.R, o.X, 1, o.ldx, o.Y, 1, o.ldy, sliced});
+    } else if (sliced) {
       diag_chains(trees[i], trans ? 1 : 0, o.X, o.ldx, o.R, o.Y, o.ldy, chains, links,
                   rowmajor);
-      if (o.R > 0)
-        mj.push_back({trees[i], trans ? 1 : 0, o.R, o.X, rowmajor ? o.ldx : 1,
-                      rowmajor ? 1 : o.ldx, o.Y, rowmajor ? o.ldy : 1, rowmajor ? 1 : o.ldy});
     } else {
       subtree_product(ar, trees[i], 0, 0, trans ? 1 : 0, o.X, o.ldx, o.R, o.Y, o.ldy, wj,
                       chains, links, rowmajor);
     }
+    if (sliced && o.R > 0)
+      mj.push_back({trees[i], trans ? 1 : 0, o.R, o.X, rowmajor ? o.ldx : 1,
+                    rowmajor ? 1 : o.ldx, o.Y, rowmajor ? o.ldy : 1, rowmajor ? 1 : o.ldy});
   }
   HM_OK(gemm_add(p, ar, wj, true, rowmajor, "batch W", -1));
   HM_OK(gemm_chain_add(p, chains, links, "batch apply", -1));
+  HM_OK(tree_prod_add(p, ar, tj, rowmajor));
   if (!mj.empty()) HM_OK(mixed_add(p, ar, mj));
   return SUCCESS;
 }

@@ -21,6 +21,7 @@
 #include <mutex>
 
 #include "hm_common.h"
+#include "hm_prog.h"
 
 namespace hm {
 
@@ -222,31 +223,65 @@ __device__ __forceinline__ void group_sum(const double *base, long long stride, 
 // only dependent load is the item itself): stage 1 sums chunk partials into
 // segment sums, stage 2 segment sums (or a block's few chunk partials)
 // into W_b; both in fixed order.
-__global__ __launch_bounds__(256) void k_reduce_items(ProdArgs a, const RedItem *items,
-                                                      int to_w) {
+__device__ __forceinline__ void reduce_item(const ProdArgs &a, const RedItem &it, int to_w) {
   __shared__ double red[256];
-  const RedItem it = items[blockIdx.x];
   const long long E = (long long)it.s * a.R;
   const double *base = (it.from_seg ? a.Wp : a.P) + it.src * a.R;
   double *out = (to_w ? a.W : a.Wp) + it.dst * a.R;
   group_sum(base, E, it.n, E, out, red);
 }
 
-// ------------------------------------------------------------ MFMA path
-typedef MfmaTile<1, 4, 1, 1, 16> PartTile;  // 16 x 64
+__global__ __launch_bounds__(256) void k_reduce_items(ProdArgs a, const RedItem *items,
+                                                      int to_w) {
+  reduce_item(a, items[blockIdx.x], to_w);
+}
 
-template <bool B_NFAST>
-__global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
-                                                     int ntiles) {
+// One tree of a batched product (a device table entry): its product
+// arguments, its two reduction lists (segment sums, then the sums into W)
+// and its partials' rank tiles.
+struct BatchProd {
+  ProdArgs a;
+  const RedItem *red[2];
+  int mtiles, pad;
+};
+
+// Batched reduction, one wave per work item (tree, item of list to_w):
+// each lane sums its elements over the item's parts in order (fixed order,
+// deterministic).  A batch's items are short (at most REDUCE_SEG parts of
+// s R values, mostly a few dozen), so one-wave workgroups with no LDS and no
+// barrier keep the most of them in flight.
+__global__ __launch_bounds__(64) void k_reduce_items_b(const BatchProd *bp, const int2 *wl,
+                                                       int to_w) {
+  const int2 w = wl[blockIdx.x];
+  const ProdArgs a = bp[w.x].a;
+  const RedItem it = bp[w.x].red[to_w][w.y];
+  const long long E = (long long)it.s * a.R;
+  const double *base = (it.from_seg ? a.Wp : a.P) + it.src * a.R;
+  double *out = (to_w ? a.W : a.Wp) + it.dst * a.R;
+  for (long long e = threadIdx.x; e < E; e += 64) {
+    double s0 = 0.0, s1 = 0.0;
+    int q = 0;
+    for (; q + 1 < it.n; q += 2) {
+      s0 += ldg(base + q * E + e);
+      s1 += ldg(base + (q + 1) * E + e);
+    }
+    if (q < it.n) s0 += ldg(base + q * E + e);
+    stg(out + e, s0 + s1);
+  }
+}
+
+// ------------------------------------------------------------ MFMA path
+// Partials of chunk k, rank tile mt (16 stacked factor rows), RHS tile nt
+// of PartTile::TN columns (16 x 64 tile, four waves; or 16 x 16, one wave).
+typedef MfmaTile<1, 4, 1, 1, 16> PartTile64;
+typedef MfmaTile<1, 1, 1, 1, 16> PartTile16;
+template <bool B_NFAST, class PartTile = PartTile64>
+__device__ __forceinline__ void mm_partials_tile(const ProdArgs &a, int k, int mt, int nt,
+                                                 int mtiles) {
   __shared__ double lds[PartTile::LDS_DOUBLES];
   __shared__ const double *rowptr[16];
   __shared__ double *outptr[16];
   __shared__ int outld[16];
-  int bid = blockIdx.x;
-  const int nt = bid % ntiles;
-  bid /= ntiles;
-  const int mt = bid % mtiles;
-  const int k = bid / mtiles;
   const int4 chk = a.ch[k];
   const int rows = chk.z;
   const int lo = chk.w;
@@ -262,7 +297,7 @@ __global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
   bool any = false;
   for (int r = 0; r < 16; r++) any |= (rowptr[r] != nullptr);
   if (!any) return;
-  const int n0 = nt * 64;
+  const int n0 = nt * PartTile::TN;
   auto la = [&](int i, int kx) -> double {
     const double *p = rowptr[i];
     return (p && kx < rows) ? ldg(p + kx) : 0.0;
@@ -279,6 +314,28 @@ __global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
     double *o = outptr[i];
     if (o && n0 + j < a.R) stg(o + (long long)(n0 + j) * outld[i], v);
   });
+}
+
+template <bool B_NFAST>
+__global__ __launch_bounds__(256) void k_mm_partials(ProdArgs a, int mtiles,
+                                                     int ntiles) {
+  int bid = blockIdx.x;
+  const int nt = bid % ntiles;
+  bid /= ntiles;
+  const int mt = bid % mtiles;
+  mm_partials_tile<B_NFAST>(a, bid / mtiles, mt, nt, mtiles);
+}
+
+// Batched partials: work item (tree, chunk, rank tile, RHS tile).  The
+// batch is many short chunks with few RHS each, so a workgroup's time is
+// mostly its dependent round trips: the one-wave 16 x 16 tile runs four
+// times as many of them per CU as the 16 x 64 one (chosen by timing).
+template <bool B_NFAST, class PartTile>
+__global__ __launch_bounds__(PartTile::NT) void k_mm_partials_b(const BatchProd *bp,
+                                                                const int4 *wl) {
+  const int4 w = wl[blockIdx.x];
+  const ProdArgs a = bp[w.x].a;
+  mm_partials_tile<B_NFAST, PartTile>(a, w.y, w.z, w.w, bp[w.x].mtiles);
 }
 
 // ------------------------------------------------------ block W path
@@ -356,17 +413,16 @@ __device__ __forceinline__ bool apply_tile_of(int bid, int nleaf, int rtiles, in
 // FAST: every leaf is a whole number of TM x KC tiles, R is a multiple of
 // TN and there is no transpose (checked on the host); only the
 // interior form of the diagonal GEMM is compiled in.
-template <bool B_NFAST, int WM, int FM, int FN, int KC, bool FAST>
-__global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles, int ntiles) {
-  typedef MfmaTile<WM, 4 / WM, FM, FN, KC> Tile;
+// Apply tile (leaf c, row tile rt, RHS tile nt) of Tile::TM x Tile::TN.
+template <bool B_NFAST, class Tile, bool FAST>
+__device__ __forceinline__ void mm_apply_tile(const ProdArgs &a, int c, int rt, int nt) {
+  constexpr int KC = Tile::KC_;
   __shared__ double lds[Tile::LDS_DOUBLES];
   __shared__ int lp[MAX_LEVELS + 1];
   __shared__ LevelInfo li[MAX_LEVELS];
   __shared__ const double *tabA[KC];
   __shared__ const double *tabB[KC];
   __shared__ int tabS[KC];
-  int c, rt, nt;
-  if (!apply_tile_of(blockIdx.x, a.nleaf, rtiles, ntiles, c, rt, nt)) return;
   const int lo = a.leaf_off[c], mc = a.leaf_off[c + 1] - lo;
   const int r0 = rt * Tile::TM;
   if (r0 >= mc) return;
@@ -451,6 +507,23 @@ __global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles, int nt
     if (FAST || (r0 + i < mc && n0 + j < a.R))
       a.Y[(long long)(lo + r0 + i) * a.y_rs + (long long)(n0 + j) * a.y_cs] = v;
   });
+}
+
+template <bool B_NFAST, int WM, int FM, int FN, int KC, bool FAST>
+__global__ __launch_bounds__(256) void k_mm_apply(ProdArgs a, int rtiles, int ntiles) {
+  int c, rt, nt;
+  if (!apply_tile_of(blockIdx.x, a.nleaf, rtiles, ntiles, c, rt, nt)) return;
+  mm_apply_tile<B_NFAST, MfmaTile<WM, 4 / WM, FM, FN, KC>, FAST>(a, c, rt, nt);
+}
+
+// Batched apply: work item (tree, leaf, row tile, RHS tile); tree < 0 pads
+// the per-XCD queues.
+template <bool B_NFAST, class Tile>
+__global__ __launch_bounds__(Tile::NT) void k_mm_apply_b(const BatchProd *bp, const int4 *wl) {
+  const int4 w = wl[blockIdx.x];
+  if (w.x < 0) return;
+  const ProdArgs a = bp[w.x].a;
+  mm_apply_tile<B_NFAST, Tile, false>(a, w.y, w.z, w.w);
 }
 
 static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
@@ -1207,6 +1280,187 @@ static int products_run_ex(hm_dtree *t, int trans, int R, const double *X, long 
   run_apply(t, a, v, nfast, s);
   timing_end(ta, s);
   HM_LAUNCH_CHECK();
+  return SUCCESS;
+}
+
+// ------------------------------------------------ batched tree products
+// Apply tile shapes of a batch (WM, FM, FN, KC: TM x TN = 16 WM FM x
+// 16 (4 / WM) FN); the narrow ones serve small RHS counts, whose 64-wide
+// tiles would be mostly padding.
+struct BatchApplyVariant {
+  int tm, tn;
+  void (*launch)(bool nfast, unsigned grid, const BatchProd *bp, const int4 *wl,
+                 hipStream_t s);
+};
+
+template <int WM, int WN, int FM, int FN, int KC>
+static void launch_apply_b(bool nfast, unsigned grid, const BatchProd *bp, const int4 *wl,
+                           hipStream_t s) {
+  typedef MfmaTile<WM, WN, FM, FN, KC> Tile;
+  if (nfast)
+    hipLaunchKernelGGL((k_mm_apply_b<true, Tile>), dim3(grid), dim3(Tile::NT), 0, s, bp, wl);
+  else
+    hipLaunchKernelGGL((k_mm_apply_b<false, Tile>), dim3(grid), dim3(Tile::NT), 0, s, bp, wl);
+}
+
+// (WM, WN, FM, FN, KC): WM x WN waves, tile 16 WM FM x 16 WN FN.
+static const BatchApplyVariant kBatchVariants[] = {
+    {64, 64, launch_apply_b<2, 2, 2, 2, 16>},   // 64 x 64
+    {128, 64, launch_apply_b<4, 1, 2, 4, 16>},  // 128 x 64, 32 x 64 per wave
+    {128, 32, launch_apply_b<4, 1, 2, 2, 32>},  // 128 x 32, KC 32
+    {64, 32, launch_apply_b<2, 2, 2, 1, 16>},   // 64 x 32
+    {64, 16, launch_apply_b<4, 1, 1, 1, 16>},   // 64 x 16
+    {32, 32, launch_apply_b<1, 1, 2, 2, 16>},   // 32 x 32, one wave
+};
+static constexpr int kBatchVariantCount =
+    sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+
+bool tree_prod_batchable(const hm_dtree *t, int trans, bool diag_only) {
+  if (!t || t->nleaf <= 0) return false;
+  if (diag_only) return true;
+  return t->slices == 0 && t->height >= 1 && t->plan.nchunk > 0 &&
+         choose_wpath(t, trans ? 1 : 0) == 0;
+}
+
+// Bytes spanned by a rows x cols operand with element strides rs, cs.
+static size_t span_bytes(long long rows, long long cols, long long rs, long long cs) {
+  if (rows <= 0 || cols <= 0) return 0;
+  return (size_t)((rows - 1) * rs + (cols - 1) * cs + 1) * sizeof(double);
+}
+
+int tree_prod_add(Program &p, Arena &ar, const std::vector<TreeProdJob> &jobs, bool nfast) {
+  if (jobs.empty()) return SUCCESS;
+  const int nj = (int)jobs.size();
+  std::vector<BatchProd> bp(nj);
+  std::vector<int4> pw[2];  // partials work: 64-column tiles, 16-column tiles
+  std::vector<int2> rw[2];
+  for (int i = 0; i < nj; i++) {
+    const TreeProdJob &j = jobs[i];
+    hm_dtree *t = j.t;
+    const int tr = j.trans ? 1 : 0;
+    const long long need = j.diag_only ? 0 : scratch_need(t, j.R);
+    double *scr = j.diag_only ? nullptr : ar.take<double>((size_t)need);
+    BatchProd &b = bp[i];
+    memset(&b, 0, sizeof(b));
+    b.a = make_args(t, tr, j.R, j.X, j.x_rs, j.x_cs, j.Y, j.y_rs, j.y_cs, scr);
+    if (j.diag_only) b.a.h = 0;  // no off-diagonal levels: op(D_c) X_c only
+    b.red[0] = t->plan.d_red_seg[tr];
+    b.red[1] = t->plan.d_red_w[tr];
+    b.mtiles = t->plan.part_mtiles[tr];
+    long long u = 0, v = 0;
+    for (const auto &blk : t->blk) {
+      u += (long long)blk.m * blk.s;
+      v += (long long)blk.n * blk.s;
+    }
+    p.touch(j.X, span_bytes(t->n, j.R, j.x_rs, j.x_cs), "batch tree products");
+    p.touch(j.Y, span_bytes(t->n, j.R, j.y_rs, j.y_cs), "batch tree products");
+    p.touch(t->d_diag, (size_t)t->diag_off[t->nleaf] * sizeof(double), "batch tree products");
+    if (j.diag_only) continue;
+    p.touch(scr, (size_t)need * sizeof(double), "batch tree products");
+    p.touch(t->d_U, (size_t)u * sizeof(double), "batch tree products");
+    p.touch(t->d_V, (size_t)v * sizeof(double), "batch tree products");
+    if (t->plan.s_in_max[tr] <= 0) continue;
+    // same order as the single-tree partials grid: chunk, rank tile, RHS tile
+    for (int v = 0; v < 2; v++) {
+      const int nt = v ? (j.R + 15) / 16 : (j.R + 63) / 64;
+      for (int k = 0; k < t->plan.nchunk; k++)
+        for (int mt = 0; mt < b.mtiles; mt++)
+          for (int n = 0; n < nt; n++) pw[v].push_back(make_int4(i, k, mt, n));
+    }
+    for (size_t q = 0; q < t->plan.red_seg[tr].size(); q++) rw[0].push_back(make_int2(i, (int)q));
+    for (size_t q = 0; q < t->plan.red_w[tr].size(); q++) rw[1].push_back(make_int2(i, (int)q));
+  }
+  const size_t bp_off = p.push(bp);
+  const size_t pw0_off = p.push(pw[0]), pw1_off = p.push(pw[1]);
+  const size_t r0_off = p.push(rw[0]), r1_off = p.push(rw[1]);
+  const unsigned npw0 = (unsigned)pw[0].size(), npw1 = (unsigned)pw[1].size();
+  const unsigned nr0 = (unsigned)rw[0].size(), nr1 = (unsigned)rw[1].size();
+  if (npw0) {
+    auto pchoice = std::make_shared<int>(1);
+    p.tunables.push_back({p.ops.size(), 2, pchoice});
+    p.add("batch partials", [=](hipStream_t s, const unsigned char *blob) {
+      const BatchProd *B = reinterpret_cast<const BatchProd *>(blob + bp_off);
+      const int4 *W0 = reinterpret_cast<const int4 *>(blob + pw0_off);
+      const int4 *W1 = reinterpret_cast<const int4 *>(blob + pw1_off);
+      if (*pchoice == 0) {
+        if (nfast)
+          hipLaunchKernelGGL((k_mm_partials_b<true, PartTile64>), dim3(npw0), dim3(256), 0, s,
+                             B, W0);
+        else
+          hipLaunchKernelGGL((k_mm_partials_b<false, PartTile64>), dim3(npw0), dim3(256), 0, s,
+                             B, W0);
+      } else {
+        if (nfast)
+          hipLaunchKernelGGL((k_mm_partials_b<true, PartTile16>), dim3(npw1), dim3(64), 0, s,
+                             B, W1);
+        else
+          hipLaunchKernelGGL((k_mm_partials_b<false, PartTile16>), dim3(npw1), dim3(64), 0, s,
+                             B, W1);
+      }
+    });
+  }
+  if (nr0 || nr1)
+    p.add("batch reduce", [=](hipStream_t s, const unsigned char *blob) {
+      const BatchProd *B = reinterpret_cast<const BatchProd *>(blob + bp_off);
+      if (nr0)
+        hipLaunchKernelGGL(k_reduce_items_b, dim3(nr0), dim3(64), 0, s, B,
+                           reinterpret_cast<const int2 *>(blob + r0_off), 0);
+      if (nr1)
+        hipLaunchKernelGGL(k_reduce_items_b, dim3(nr1), dim3(64), 0, s, B,
+                           reinterpret_cast<const int2 *>(blob + r1_off), 1);
+    });
+  // apply: one launch per RHS width group, tile shape tuned per group; the
+  // leaves of a group are dealt to the eight XCD queues in turn (workgroup
+  // b runs on XCD b % 8) with all tiles of one leaf in one queue, so X_c
+  // and its W_b are shared through one L2
+  static const int kDefault[3] = {4, 3, 0};
+  static const char *kName[3] = {"batch apply R<=16", "batch apply R<=32", "batch apply R>32"};
+  for (int g = 0; g < 3; g++) {
+    std::vector<int> mem;
+    for (int i = 0; i < nj; i++) {
+      const int R = jobs[i].R;
+      if ((R <= 16 ? 0 : (R <= 32 ? 1 : 2)) == g) mem.push_back(i);
+    }
+    if (mem.empty()) continue;
+    size_t toff[kBatchVariantCount];
+    unsigned ntile[kBatchVariantCount];
+    for (int v = 0; v < kBatchVariantCount; v++) {
+      const int tm = kBatchVariants[v].tm, tn = kBatchVariants[v].tn;
+      std::vector<std::vector<int4>> q(8);
+      long long leaves = 0;
+      for (int i : mem) {
+        const hm_dtree *t = jobs[i].t;
+        const int ntl = (jobs[i].R + tn - 1) / tn;
+        for (int c = 0; c < t->nleaf; c++) {
+          const int mc = t->leaf_off[c + 1] - t->leaf_off[c];
+          if (mc <= 0) continue;
+          std::vector<int4> &dst = q[leaves++ % 8];
+          for (int rt = 0; rt < (mc + tm - 1) / tm; rt++)
+            for (int n = 0; n < ntl; n++) dst.push_back(make_int4(i, c, rt, n));
+        }
+      }
+      size_t len = 0;
+      for (const auto &x : q) len = std::max(len, x.size());
+      std::vector<int4> wl(len * 8, make_int4(-1, 0, 0, 0));
+      for (int x = 0; x < 8; x++)
+        for (size_t u = 0; u < q[x].size(); u++) wl[u * 8 + x] = q[x][u];
+      toff[v] = p.push(wl);
+      ntile[v] = (unsigned)wl.size();
+    }
+    // HM_BATCH_APPLY_VARIANT=v fixes every group's tile shape (not tuned)
+    const char *force = getenv("HM_BATCH_APPLY_VARIANT");
+    const int fv = force ? std::max(0, std::min(kBatchVariantCount - 1, atoi(force))) : -1;
+    auto choice = std::make_shared<int>(fv >= 0 ? fv : kDefault[g]);
+    if (fv < 0) p.tunables.push_back({p.ops.size(), kBatchVariantCount, choice});
+    std::vector<size_t> to(toff, toff + kBatchVariantCount);
+    std::vector<unsigned> nt(ntile, ntile + kBatchVariantCount);
+    p.add(kName[g], [=](hipStream_t s, const unsigned char *blob) {
+      const int v = *choice;
+      if (nt[v])
+        kBatchVariants[v].launch(nfast, nt[v], reinterpret_cast<const BatchProd *>(blob + bp_off),
+                                 reinterpret_cast<const int4 *>(blob + to[v]), s);
+    });
+  }
   return SUCCESS;
 }
 

@@ -198,4 +198,27 @@ struct MixedJob {
 // count, whatever the number of trees).
 int mixed_add(Program &p, Arena &ar, const std::vector<MixedJob> &jobs);
 
+// ------------------------------------ batched tree products (FP64 trees)
+// One product op(H) X (op = transpose when trans), X / Y element (row, col)
+// at [row * rs + col * cs], run by the tree-product kernels of a single
+// product (chunk partials with X read once, fixed-order reductions, apply
+// tiles over the leaves) with every tree of the batch in the same launches.
+struct TreeProdJob {
+  hm_dtree *t;
+  int trans, R;
+  const double *X;
+  long x_rs, x_cs;
+  double *Y;
+  long y_rs, y_cs;
+  bool diag_only;  // Y = op(D) X only (the FP64 diagonal of a sliced tree)
+};
+// diag_only: any tree with leaves; else FP64 factors and the chunk W path
+// (rank sum per chunk within the partial kernel's tiles) in mode trans.
+bool tree_prod_batchable(const hm_dtree *t, int trans, bool diag_only = false);
+// Appends partials, the two reduction stages and one apply launch per RHS
+// width group (<= 16, <= 32, wider), each apply tunable over its tile shape;
+// product scratch of every tree is taken from `ar`.  nfast: X is row-major
+// (X H as H^T X^T), the same for every job.
+int tree_prod_add(Program &p, Arena &ar, const std::vector<TreeProdJob> &jobs, bool nfast);
+
 }  // namespace hm

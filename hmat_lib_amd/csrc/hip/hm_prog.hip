@@ -1,6 +1,8 @@
 // Launch programs and the batched kernels they are built from (FP64):
 // MFMA GEMM tiles with deterministic split-K, strided copies, column norms.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "hm_prog.h"
@@ -351,6 +353,9 @@ int gemm_chain_add(Program &p, const std::vector<GemmChain> &chains,
 
 int program_tune(Program &p, hipStream_t s) {
   if (p.tunables.empty()) return SUCCESS;
+  // HM_TUNE_VERBOSE=1 prints every variant's time to stderr
+  const char *ev = getenv("HM_TUNE_VERBOSE");
+  const bool verbose = ev && ev[0] == '1';
   hipEvent_t e0, e1;
   HM_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -375,6 +380,9 @@ int program_tune(Program &p, hipStream_t s) {
       }
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, e0, e1);
+      if (verbose)
+        fprintf(stderr, "program_tune: %s variant %d: %.1f us\n", p.names[t.op].c_str(), v,
+                ms * 1e3f / 3);
       if (ms < best) {
         best = ms;
         bestv = v;

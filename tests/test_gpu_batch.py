@@ -45,10 +45,25 @@ def _forest(seed, count):
     return out, rng
 
 
+ROUTES = ["tree", "chains"]
+
+
+def _route(monkeypatch, route):
+    """"tree": trees with the chunk W path run the batched tree-product
+    kernels (partials, reductions, apply tiles), the rest the chained GEMMs;
+    "chains": every tree the chained GEMMs."""
+    if route == "chains":
+        monkeypatch.setenv("HM_BATCH_CHAINS", "1")
+    else:
+        monkeypatch.delenv("HM_BATCH_CHAINS", raising=False)
+
+
+@pytest.mark.parametrize("route", ROUTES)
 @pytest.mark.parametrize("trans", [0, 1])
-def test_batch_products_vs_dense(gpu, trans):
+def test_batch_products_vs_dense(gpu, trans, route, monkeypatch):
     from hmat_lib_amd import device as D
-    _say(f"case: batch of 14 uneven trees trans={trans}")
+    _route(monkeypatch, route)
+    _say(f"case: batch of 14 uneven trees trans={trans} route={route}")
     forest, rng = _forest(7 + trans, 14)
     xs, ys, dense, xh = [], [], [], []
     for ft, dt, R in forest:
@@ -69,7 +84,7 @@ def test_batch_products_vs_dense(gpu, trans):
     check()
     b.tune()
     v = b.variants()
-    assert len(v) >= 2 and all(0 <= k < 3 for k in v)
+    assert len(v) >= 2 and all(0 <= k < 6 for k in v)
     b.run()
     D.synchronize()
     check()
@@ -93,12 +108,43 @@ def test_batch_products_vs_dense(gpu, trans):
     b.free()
 
 
-def test_batch_dense_times_hodlr(gpu):
+@pytest.mark.parametrize("variant", range(6))
+def test_batch_apply_variants(gpu, variant, monkeypatch):
+    """Every apply tile shape of the batched tree products (64 x 64,
+    128 x 64, 128 x 32, 64 x 32, 64 x 16 on four waves; 32 x 32 on one),
+    fixed with HM_BATCH_APPLY_VARIANT,
+    on ragged leaves (zero-size ones included) and R = 1 .. 100, vs dense
+    algebra; H X and H^T X."""
+    from hmat_lib_amd import device as D
+    _route(monkeypatch, "tree")
+    monkeypatch.setenv("HM_BATCH_APPLY_VARIANT", str(variant))
+    for trans in (0, 1):
+        _say(f"case: batch apply variant {variant} trans={trans}")
+        forest, rng = _forest(31 + trans, 12)
+        xs, ys, xh, dense = [], [], [], []
+        for ft, dt, R in forest:
+            x = np.asfortranarray(rng.standard_normal((ft.n, R)))
+            xh.append(x)
+            xs.append(D.DeviceArray.from_numpy(x))
+            ys.append(D.DeviceArray((ft.n, R)))
+            d = ft.to_dense()
+            dense.append(d.T if trans else d)
+        b = D.Batch([f[1] for f in forest], xs, ys, trans=trans)
+        b.run()
+        D.synchronize()
+        for i in range(len(forest)):
+            assert H.rel_err(ys[i].numpy(), dense[i] @ xh[i]) < TOL, (trans, i)
+        b.free()
+
+
+@pytest.mark.parametrize("route", ROUTES)
+def test_batch_dense_times_hodlr(gpu, route, monkeypatch):
     """Batched X_i H_i (X_i, Y_i: R_i x n_i, the batched form of
     multiply_dense_hodlr): taken as Y^T = H^T X^T with transposed operand
     access and transposed stores; untuned, tuned and replayed from a graph."""
     from hmat_lib_amd import device as D
-    _say("case: batch of 14 uneven trees, X H")
+    _route(monkeypatch, route)
+    _say(f"case: batch of 14 uneven trees, X H, route={route}")
     forest, rng = _forest(23, 14)
     xs, ys, xh, dense = [], [], [], []
     for ft, dt, R in forest:
