@@ -320,11 +320,13 @@ HM_HIP_API int hm_hip_compress_batch(int nmat, const double *const *d_A, const l
 /* A batched product over ntrees trees, fixed at creation: trans 0:
  * Y_i = H_i X_i, 1: Y_i = H_i^T X_i (X_i, Y_i n_i x R[i]); 2: Y_i = X_i H_i
  * (X_i, Y_i R[i] x n_i); device arrays, column-major, leading dimensions
- * ldx[i], ldy[i].  One launch program of two batched-GEMM launches
- * whatever the number of trees or levels; trees with sliced factors
- * (hm_hip_tree_set_precision) add one launch per sliced stage for all of
- * them.  The batch keeps pointers to the trees and operands (it owns
- * neither).  Replaces per-tree loops of
+ * ldx[i], ldy[i].  One launch program whatever the number of trees or
+ * levels: low-rank trees share the launches of the single-tree product
+ * kernels (partials, two reductions, one apply per RHS width group),
+ * high-rank trees two batched-GEMM launches, and trees with sliced factors
+ * (hm_hip_tree_set_precision) one launch per sliced stage for all of them.
+ * The batch keeps pointers to the trees and operands (it owns neither).
+ * Replaces per-tree loops of
  * multiply_hodlr_dense / multiply_hodlr_transpose_dense /
  * multiply_dense_hodlr (src/dense_algebra.c:361-648, 957-1026). */
 typedef struct hm_batch hm_batch;
