@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -157,6 +158,16 @@ struct ProductPlan {
 }  // namespace hm
 
 struct MixedFactors;
+namespace hm {
+// Device pools of the trees of one batched compression (U, V, diagonal
+// blocks, metadata of all of them in one allocation each), freed with the
+// last of those trees.
+struct SharedPools {
+  std::vector<void *> ptrs;
+  ~SharedPools();
+};
+}  // namespace hm
+
 struct hm_dtree {
   int height = 0;
   int n = 0;
@@ -184,6 +195,9 @@ struct hm_dtree {
   int slices = 0;
   int slice_fmt = 0;
   struct MixedFactors *mixed = nullptr;
+  // set when d_diag, d_U, d_V and the metadata point into pools shared with
+  // other trees (hm_hip_compress_batch); null: the tree owns them
+  std::shared_ptr<hm::SharedPools> shared;
 };
 
 namespace hm {
@@ -193,6 +207,10 @@ __host__ __device__ inline int blk_index(int level, int j) { return (1 << level)
 // Fill sizes/offsets of a tree from its leaf sizes and block ranks;
 // allocates and uploads metadata and the product plan (not the data pools).
 int tree_finalize_meta(hm_dtree *t, hipStream_t s);
+// The same for several trees, their metadata in one device allocation and
+// one copy, returned in *blob (the caller owns it; the trees' d_meta stay
+// null).
+int trees_finalize_meta(const std::vector<hm_dtree *> &ts, char **blob, hipStream_t s);
 int tree_alloc_pools(hm_dtree *t);
 int ensure_scratch(hm_dtree *t, int R);
 // Doubles of product scratch (partials, W, segment sums) for R columns.

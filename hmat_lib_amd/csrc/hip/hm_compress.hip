@@ -398,33 +398,38 @@ int hm_hip_compress_batch(int nmat, const double *const *d_A, const long *lda, c
   double *up = nullptr, *vp = nullptr;
   int rc = tsvd_run(all, tsvd_options(opts), &up, &vp, &ts, nullptr, 0, s);
   if (rc != SUCCESS) return fail(rc);
-  // per-tree pools: the tree's blocks are contiguous in the batch pools
+  // The trees keep their factors where the batched SVD left them (each
+  // tree's blocks are contiguous in the batch pools, in the order its
+  // metadata lays them out), their diagonal blocks in one pool and their
+  // metadata in one allocation: four allocations and one metadata copy for
+  // the whole batch, shared by its trees and freed with the last of them.
+  auto pools = std::make_shared<SharedPools>();
+  pools->ptrs = {up, vp};
+  for (int i = 0; i < nmat; i++) {
+    hm_dtree *t = trees[i];
+    for (int g = 0; g < t->nblock; g++) t->blk[g].s = all[first[i] + g].k;
+    t->d_U = all[first[i]].uout;
+    t->d_V = all[first[i]].vout;
+    t->shared = pools;
+  }
+  char *meta = nullptr;
+  rc = trees_finalize_meta(trees, &meta, s);
+  if (rc == SUCCESS) pools->ptrs.push_back(meta);
+  std::vector<long long> doff(nmat + 1, 0);
+  for (int i = 0; i < nmat; i++) doff[i + 1] = doff[i] + trees[i]->diag_off[trees[i]->nleaf];
+  double *dpool = nullptr;
+  if (rc == SUCCESS) rc = dalloc(&dpool, (size_t)std::max(1LL, doff[nmat]));
+  if (rc == SUCCESS) pools->ptrs.push_back(dpool);
   for (int i = 0; i < nmat && rc == SUCCESS; i++) {
     hm_dtree *t = trees[i];
-    long long u = 0, v = 0;
-    for (int g = 0; g < t->nblock; g++) {
-      const TsvdBlock &b = all[first[i] + g];
-      t->blk[g].s = b.k;
-      u += (long long)b.m * b.k;
-      v += (long long)b.n * b.k;
+    t->d_diag = dpool + doff[i];
+    hipLaunchKernelGGL(k_extract_diag, dim3(t->nleaf, 64), dim3(256), 0, s, d_A[i],
+                       (long long)lda[i], t->d_leaf_off, t->d_diag_off, t->d_diag);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error("hm_hip_compress_batch: diagonal blocks: %s", hipGetErrorString(e));
+      rc = ALLOCATION_FAILURE;
     }
-    rc = dalloc(&t->d_U, (size_t)u);
-    if (rc == SUCCESS) rc = dalloc(&t->d_V, (size_t)v);
-    if (rc != SUCCESS) break;
-    const TsvdBlock &b0 = all[first[i]];
-    {
-      hipError_t e = hipSuccess;
-      if (u > 0)
-        e = hipMemcpyAsync(t->d_U, b0.uout, u * sizeof(double), hipMemcpyDeviceToDevice, s);
-      if (e == hipSuccess && v > 0)
-        e = hipMemcpyAsync(t->d_V, b0.vout, v * sizeof(double), hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) {
-        set_error("hm_hip_compress_batch: %s", hipGetErrorString(e));
-        rc = ALLOCATION_FAILURE;
-        break;
-      }
-    }
-    rc = finish_tree(t, d_A[i], lda[i], s);
   }
   if (rc == SUCCESS) {
     hipError_t e = hipStreamSynchronize(s);
@@ -433,8 +438,7 @@ int hm_hip_compress_batch(int nmat, const double *const *d_A, const long *lda, c
       rc = ALLOCATION_FAILURE;
     }
   }
-  dfree(up);
-  dfree(vp);
+  pools.reset();  // the trees hold it now
   if (rc != SUCCESS) return fail(rc);
   auto t1 = std::chrono::steady_clock::now();
   fill_stats(stats, ts, std::chrono::duration<double, std::milli>(t1 - t0).count());

@@ -56,7 +56,7 @@ struct Program {
   bool own_blob = true;  // false: d_blob belongs to the caller (not freed here)
   // audit data: address ranges touched by items / buffers they may touch
   bool record = false;
-  std::vector<std::pair<MemRange, std::string>> touched;
+  std::vector<std::pair<MemRange, const char *>> touched;  // labels: string literals
   std::vector<MemRange> allowed;
 
   ~Program() {
@@ -73,6 +73,8 @@ struct Program {
     names.push_back(name);
     ops.push_back(std::move(f));
   }
+  // what: a string literal (kept by pointer; recording runs for every item
+  // of a program, hundreds of thousands for a batched compression)
   void touch(const void *lo, size_t bytes, const char *what) {
     if (record && bytes) touched.push_back({{lo, bytes}, what});
   }

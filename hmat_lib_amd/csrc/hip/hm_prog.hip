@@ -36,7 +36,7 @@ int Program::audit(std::string *msg) const {
       if (msg) {
         char buf[256];
         snprintf(buf, sizeof(buf), "%s touches [%p, %p) outside every buffer",
-                 t.second.c_str(), (const void *)lo, (const void *)hi);
+                 t.second, (const void *)lo, (const void *)hi);
         *msg = buf;
       }
       return INPUT_ERROR;

@@ -13,10 +13,14 @@ void tree_free(hm_dtree *t) {
   if (!t) return;
   mixed_free(t->mixed);
   t->mixed = nullptr;
-  dfree(t->d_diag);
-  dfree(t->d_U);
-  dfree(t->d_V);
-  dfree(t->d_meta);  // every metadata and plan array
+  if (t->shared) {
+    t->shared.reset();  // the pools go with the last tree holding them
+  } else {
+    dfree(t->d_diag);
+    dfree(t->d_U);
+    dfree(t->d_V);
+    dfree(t->d_meta);  // every metadata and plan array
+  }
   dfree(t->d_scratch);
   delete t;
 }
@@ -54,8 +58,15 @@ struct MetaPack {
   }
 };
 
-// Requires leaf_off (nleaf+1) and blk[g].{r0,m,c0,n,s,level,j}.
-int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
+SharedPools::~SharedPools() {
+  for (void *p : ptrs) dfree(static_cast<char *>(p));
+}
+
+// Sizes, offsets and the product plan of a tree; its metadata and plan
+// arrays are added to mp (ch: storage for the chunk table, which must live
+// until mp is uploaded).  Requires leaf_off (nleaf+1) and
+// blk[g].{r0,m,c0,n,s,level,j}.
+static void meta_plan(hm_dtree *t, MetaPack &mp, std::vector<int4> &ch) {
   const int h = t->height, nl = t->nleaf;
   t->n = t->leaf_off[nl];
   t->diag_off.assign(nl + 1, 0);
@@ -235,7 +246,7 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     std::stable_sort(p.witems[tr].begin(), p.witems[tr].end(),
                      [](const int4 &x, const int4 &y) { return x.w > y.w; });
   }
-  std::vector<int4> ch(p.nchunk);
+  ch.assign(p.nchunk, make_int4(0, 0, 0, 0));
   for (int k = 0; k < p.nchunk; k++)
     ch[k] = make_int4(p.ch_leaf[k], p.ch_r0[k], p.ch_rows[k],
                       t->leaf_off[p.ch_leaf[k]] + p.ch_r0[k]);
@@ -265,7 +276,6 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     }
   }
 
-  MetaPack mp;
   mp.add(&t->d_leaf_off, t->leaf_off);
   mp.add(&t->d_diag_off, t->diag_off);
   mp.add(&t->d_blk, t->blk);
@@ -283,7 +293,21 @@ int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
     mp.add(&p.d_wpofs[tr], p.wpofs[tr]);
     mp.add(&p.d_wmulti[tr], p.wmulti[tr]);
   }
+}
+
+int tree_finalize_meta(hm_dtree *t, hipStream_t s) {
+  MetaPack mp;
+  std::vector<int4> ch;
+  meta_plan(t, mp, ch);
   return mp.upload(&t->d_meta, s);
+}
+
+int trees_finalize_meta(const std::vector<hm_dtree *> &ts, char **blob, hipStream_t s) {
+  MetaPack mp;
+  std::vector<std::vector<int4>> ch(ts.size());
+  for (size_t i = 0; i < ts.size(); i++) meta_plan(ts[i], mp, ch[i]);
+  *blob = nullptr;
+  return mp.upload(blob, s);
 }
 
 int tree_alloc_pools(hm_dtree *t) {
