@@ -312,7 +312,7 @@ HM_HIP_API int hm_hip_rank_features(int nblock, int n_extra, const float *extra,
  * halving, rank rule as hm_hip_compress_dense.  out[i] receives tree i;
  * stats (optional) covers the whole batch.  The trees of one call share
  * their device pools (factors, diagonal blocks, metadata): each is freed
- * with hm_hip_tree_free as usual, and the pools go with the last of them.
+ * with hm_hip_tree_destroy as usual, and the pools go with the last of them.
  * Replaces nmat calls of dense_to_tree_hodlr (src/constructors.c:629-704). */
 HM_HIP_API int hm_hip_compress_batch(int nmat, const double *const *d_A, const long *lda,
                                      const int *m, const int *height, double thr,
