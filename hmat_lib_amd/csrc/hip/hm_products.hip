@@ -550,8 +550,9 @@ static ProdArgs make_args(hm_dtree *t, int trans, int R, const double *X,
   long long pmax = std::max(std::max(t->plan.pofs[0].back(), t->plan.pofs[1].back()),
                             std::max(t->plan.wpofs[0].back(), t->plan.wpofs[1].back()));
   a.P = scratch_base ? scratch_base : t->d_scratch;
-  a.W = a.P + pmax * R;
-  a.Wp = a.W + t->plan.wofs[t->nblock] * R;
+  // no scratch (a product of the diagonal blocks only): no W either
+  a.W = a.P ? a.P + pmax * R : nullptr;
+  a.Wp = a.P ? a.W + t->plan.wofs[t->nblock] * R : nullptr;
   a.X = X;
   a.x_rs = x_rs;
   a.x_cs = x_cs;
@@ -1284,9 +1285,8 @@ static int products_run_ex(hm_dtree *t, int trans, int R, const double *X, long 
 }
 
 // ------------------------------------------------ batched tree products
-// Apply tile shapes of a batch (WM, FM, FN, KC: TM x TN = 16 WM FM x
-// 16 (4 / WM) FN); the narrow ones serve small RHS counts, whose 64-wide
-// tiles would be mostly padding.
+// Apply tile shapes of a batch; the narrow ones serve small RHS counts,
+// whose 64-wide tiles would be mostly padding.
 struct BatchApplyVariant {
   int tm, tn;
   void (*launch)(bool nfast, unsigned grid, const BatchProd *bp, const int4 *wl,
