@@ -210,3 +210,35 @@ def test_batch_compress_exp1d(gpu):
     b.free()
     for t in trees:
         t.free()
+
+
+def test_batch_compress_shared_pools(gpu):
+    """The trees of one batched compression share their device pools: free
+    every other tree, the others' products still match the FP64 dense
+    product, and the pools go with the last tree."""
+    from hmat_lib_amd import device as D
+    sizes = [384, 512, 768, 1024, 1280, 640]
+    _say(f"case: shared pools of a batch compression {sizes}")
+    mats, dens, hs = [], [], []
+    for i, n in enumerate(sizes):
+        pts = WL.points_1d_sorted(1700 + i, n)
+        da = D.fill_kernel_matrix(0, pts, 0.05 + 0.03 * i, 0.0)
+        mats.append(da)
+        dens.append(da.numpy())
+        hs.append(max(1, int(round(np.log2(n / 128)))))
+    trees, _ = D.compress_batch(mats, hs, 1e-8)
+    for a in mats:
+        a.free()
+    for i in (0, 2, 4):
+        trees[i].free()
+    D.synchronize()
+    for i in (1, 3, 5):
+        n = sizes[i]
+        x = np.asfortranarray(np.random.default_rng(i).standard_normal((n, 4)))
+        dx, dy = D.DeviceArray.from_numpy(x), D.DeviceArray((n, 4))
+        trees[i].matmat(dx, dy)
+        D.synchronize()
+        assert H.rel_err(dy.numpy(), dens[i] @ x) < 1e-8, i
+    for i in (1, 3, 5):
+        trees[i].free()
+    D.synchronize()
