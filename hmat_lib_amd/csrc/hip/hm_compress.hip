@@ -414,7 +414,7 @@ int hm_hip_compress_batch(int nmat, const double *const *d_A, const long *lda, c
   }
   char *meta = nullptr;
   rc = trees_finalize_meta(trees, &meta, s);
-  if (rc == SUCCESS) pools->ptrs.push_back(meta);
+  if (meta) pools->ptrs.push_back(meta);  // also when its copy failed
   std::vector<long long> doff(nmat + 1, 0);
   for (int i = 0; i < nmat; i++) doff[i + 1] = doff[i] + trees[i]->diag_off[trees[i]->nleaf];
   double *dpool = nullptr;
