@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include <mutex>
+#include <thread>
 
 #include "hm_tsqr.h"
 
@@ -825,8 +826,22 @@ int program_exec(const std::function<int(Program &, Arena &)> &build,
   HM_OK(build(dry, count));
   for (const auto &r : inputs) dry.allow(r.lo, r.bytes);
   dry.allow(Arena::fake_base(), count.off + 256);
+  // The audit only reads `dry`: for large programs (a batched compression
+  // records hundreds of thousands of ranges) it runs on a helper thread while
+  // the real program is built, and is joined before anything is uploaded or
+  // launched; small ones are audited here (a thread costs more than they do).
   std::string msg;
-  if (dry.audit(&msg) != SUCCESS) {
+  int audit_rc = SUCCESS;
+  std::thread auditor;
+  struct Join {
+    std::thread &t;
+    ~Join() {
+      if (t.joinable()) t.join();
+    }
+  } join_auditor{auditor};
+  if (dry.touched.size() >= 100000) {
+    auditor = std::thread([&]() { audit_rc = dry.audit(&msg); });
+  } else if (dry.audit(&msg) != SUCCESS) {
     set_error("launch program audit: %s", msg.c_str());
     return INPUT_ERROR;
   }
@@ -861,6 +876,13 @@ int program_exec(const std::function<int(Program &, Arena &)> &build,
   }
   Program prog;
   rc = build(prog, ar);
+  if (auditor.joinable()) {
+    auditor.join();
+    if (audit_rc != SUCCESS) {
+      set_error("launch program audit: %s", msg.c_str());
+      rc = INPUT_ERROR;
+    }
+  }
   if (rc == SUCCESS && ar.off > ar.cap) {
     set_error("launch program: scratch grew between passes");
     rc = INPUT_ERROR;
