@@ -875,6 +875,7 @@ int program_exec(const std::function<int(Program &, Arena &)> &build,
     return rc;
   }
   Program prog;
+  prog.blob.reserve(dry.blob.size() + 64);  // the same items: no regrowth copies
   rc = build(prog, ar);
   if (auditor.joinable()) {
     auditor.join();
